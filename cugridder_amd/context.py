@@ -1,0 +1,193 @@
+"""Device-resident execution: a reusable workspace bound to a HIP stream.
+
+Inputs and outputs are torch tensors already on the GPU (torch is used only
+for device memory and streams).  Every call goes through libcurafft.so's
+cg_*_dev entry points.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+STAGES = ("plan", "sort", "records", "grid", "fft", "image")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """Wraps cg_ctx_create; `stream` is a torch.cuda.Stream (None = torch's
+    current stream at construction)."""
+
+    def __init__(self, stream=None, timing=False, **options):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("cugridder_amd.Context needs a GPU")
+        self._torch = torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        self.stream = stream
+        self._h = ctypes.c_void_p()
+        check(lib().cg_ctx_create(ctypes.byref(self._h), ctypes.c_void_p(stream.cuda_stream)),
+              "cg_ctx_create")
+        if timing:
+            self.set_option("timing", 1)
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def set_option(self, name, value):
+        check(lib().cg_ctx_set_option(self._h, name.encode(), int(value)), "set_option " + name)
+
+    def close(self):
+        if self._h:
+            lib().cg_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ imaging
+    @staticmethod
+    def _prec(vis_dtype, uvw_dtype, wgt):
+        import torch
+        if vis_dtype == torch.complex64:
+            prec = 0
+        elif vis_dtype == torch.complex128:
+            prec = 1
+        else:
+            raise TypeError("vis/image must be complex64 or complex128")
+        up = 1 if uvw_dtype == torch.float64 else 0
+        wp = prec if wgt is None else (1 if wgt.dtype == torch.float64 else 0)
+        return prec, up, wp
+
+    def ms2dirty(self, uvw, vis, wgt, nxdirty, nydirty, fov, freq, epsilon, sigma=2.0, sign=-1,
+                 out=None):
+        """Complex dirty image [nx, ny] from device visibilities."""
+        torch = self._torch
+        prec, up, wp = self._prec(vis.dtype, uvw.dtype, wgt)
+        if out is None:
+            out = torch.empty((nxdirty, nydirty), dtype=vis.dtype, device=vis.device)
+        ier = lib().cg_ms2dirty_dev(self._h, prec, uvw.shape[0], nxdirty, nydirty, float(fov),
+                                    float(freq), _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
+                                    _ptr(out), float(epsilon), float(sigma), int(sign))
+        check(ier, "ms2dirty")
+        return out
+
+    def dirty2ms(self, uvw, dirty, wgt, fov, freq, epsilon, sigma=2.0, sign=-1, out=None):
+        """Complex visibilities [nrow] from a device image [nx, ny]."""
+        torch = self._torch
+        prec, up, wp = self._prec(dirty.dtype, uvw.dtype, wgt)
+        nx, ny = dirty.shape
+        if out is None:
+            out = torch.empty(uvw.shape[0], dtype=dirty.dtype, device=dirty.device)
+        ier = lib().cg_dirty2ms_dev(self._h, prec, uvw.shape[0], nx, ny, float(fov), float(freq),
+                                    _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
+                                    float(epsilon), float(sigma), int(sign))
+        check(ier, "dirty2ms")
+        return out
+
+    def nufft2d1(self, x, y, c, N1, N2, epsilon, sigma=2.0, out=None):
+        torch = self._torch
+        prec = 0 if c.dtype == torch.complex64 else 1
+        if out is None:
+            out = torch.empty((N2, N1), dtype=c.dtype, device=c.device)
+        ier = lib().cg_nufft2d_dev(self._h, 1, prec, x.shape[0], _ptr(x), _ptr(y), _ptr(c), N1, N2,
+                                   _ptr(out), float(epsilon), float(sigma))
+        check(ier, "nufft2d1")
+        return out
+
+    def nufft2d2(self, x, y, fk, epsilon, sigma=2.0, out=None):
+        torch = self._torch
+        prec = 0 if fk.dtype == torch.complex64 else 1
+        N2, N1 = fk.shape
+        if out is None:
+            out = torch.empty(x.shape[0], dtype=fk.dtype, device=fk.device)
+        ier = lib().cg_nufft2d_dev(self._h, 2, prec, x.shape[0], _ptr(x), _ptr(y), _ptr(out), N1,
+                                   N2, _ptr(fk), float(epsilon), float(sigma))
+        check(ier, "nufft2d2")
+        return out
+
+    # ------------------------------------------------------------ introspection
+    def stage_times(self):
+        """{'ms2dirty': {stage: ms}, 'dirty2ms': {...}, 'calls': (n1, n2)} summed."""
+        out = np.zeros(12)
+        calls = (ctypes.c_longlong * 2)()
+        check(lib().cg_ctx_stage_times(self._h, out, calls), "stage_times")
+        return {"ms2dirty": dict(zip(STAGES, out[:6].tolist())),
+                "dirty2ms": dict(zip(STAGES, out[6:].tolist())),
+                "calls": (calls[0], calls[1])}
+
+    def reset_times(self):
+        lib().cg_ctx_reset_times(self._h)
+
+    def last_plan(self):
+        out = np.zeros(16)
+        lib().cg_ctx_last_plan(self._h, out)
+        keys = ("nf1", "nf2", "nf3", "kw", "beta", "es_c", "pix_x", "pix_y", "i_center",
+                "i_half_width", "o_center", "o_half_width", "gamma", "h", "tile_x", "tile_y")
+        d = dict(zip(keys, out.tolist()))
+        for k in ("nf1", "nf2", "nf3", "kw", "tile_x", "tile_y"):
+            d[k] = int(d[k])
+        return d
+
+    # ------------------------------------------------------------ test hooks
+    def grid_view(self, dtype):
+        """The workspace w-plane grid of the last call as a torch tensor view
+        [nf3, nf2, nf1] (copied)."""
+        torch = self._torch
+        p = ctypes.c_void_p()
+        cells = ctypes.c_longlong()
+        check(lib().cg_ctx_grid(self._h, ctypes.byref(p), ctypes.byref(cells)), "ctx_grid")
+        plan = self.last_plan()
+        n = cells.value
+        itemsize = 8 if dtype == torch.complex64 else 16
+        buf = torch.empty(n, dtype=dtype, device="cuda")
+        _memcpy_d2d(buf.data_ptr(), p.value, n * itemsize)
+        return buf.view(plan["nf3"], plan["nf2"], plan["nf1"])
+
+    def sorted_view(self):
+        """(keys uint32-as-int64, perm int32, tile_start int32) of the last call."""
+        torch = self._torch
+        k, pm, ts = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        n = (ctypes.c_longlong * 2)()
+        check(lib().cg_ctx_sorted(self._h, ctypes.byref(k), ctypes.byref(pm), ctypes.byref(ts), n),
+              "ctx_sorted")
+        m, nt = n[0], n[1]
+        keys = torch.empty(m, dtype=torch.int32, device="cuda")
+        perm = torch.empty(m, dtype=torch.int32, device="cuda")
+        tst = torch.empty(nt + 1, dtype=torch.int32, device="cuda")
+        if m:
+            _memcpy_d2d(keys.data_ptr(), k.value, 4 * m)
+            _memcpy_d2d(perm.data_ptr(), pm.value, 4 * m)
+        _memcpy_d2d(tst.data_ptr(), ts.value, 4 * (nt + 1))
+        return keys.to(torch.int64) & 0xFFFFFFFF, perm, tst
+
+    def fft2d(self, data, direction):
+        """In-place unnormalised 2-D FFT of data [batch, n2, n1] (or [n2, n1])."""
+        torch = self._torch
+        prec = 0 if data.dtype == torch.complex64 else 1
+        d = data if data.dim() == 3 else data.unsqueeze(0)
+        check(lib().cg_fft2d_dev(self._h, prec, _ptr(d), d.shape[0], d.shape[1], d.shape[2],
+                                 int(direction)), "fft2d")
+        return data
+
+    def sort_pairs(self, keys, vals, key_bits):
+        """Stable in-place sort of int32 keys (as uint32) / int32 values."""
+        check(lib().cg_sort_pairs_dev(self._h, _ptr(keys), _ptr(vals), keys.shape[0],
+                                      int(key_bits)), "sort_pairs")
+        return keys, vals
+
+
+def _memcpy_d2d(dst, src, nbytes):
+    import torch
+    if nbytes == 0:
+        return
+    s = torch.cuda.current_stream()
+    check(lib().cg_memcpy_dev(ctypes.c_void_p(dst), ctypes.c_void_p(src),
+                              ctypes.c_void_p(s.cuda_stream), nbytes), "memcpy")
+    s.synchronize()

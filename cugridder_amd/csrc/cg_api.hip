@@ -1,0 +1,890 @@
+// Orchestration of ms2dirty / dirty2ms on one MI355X and the C-ABI.
+//
+// The eight reference entry points (include/cugridder.h:20-46, defined at
+// src/astro/cugridder.cu:540-571) keep their names, argument order and
+// meaning: host buffers in, host buffers out, every device allocation made
+// and released inside the call, int ier returned (0 ok, 1 eps clamped).
+// Unlike the reference, failures return a non-zero code instead of exit().
+//
+// Device-pointer entry points (cg_*_dev) run the same pipeline on inputs
+// already resident in HBM with a reusable workspace held by a context.
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "cg_internal.hpp"
+#include "../../include/curafft.h"
+
+namespace cg {
+
+void report_hip_error(hipError_t e, const char *what, const char *file, int line)
+{
+    if (std::getenv("CG_QUIET")) return;
+    std::fprintf(stderr, "curafft: HIP error '%s' (%d) at %s:%d in %s\n", hipGetErrorString(e),
+                 (int)e, file, line, what);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_IMAGE, ST_COUNT };
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    int lds_budget = 65536;
+    int tile_grid = 16;
+    int tile_degrid = 32;
+    int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
+    int64_t last_m = 0;
+    // workspace (grow-only)
+    DevBuf keys, keys2, vals, vals2, hist, scan, tstart, geo, val, grid, zq, phi3, phi1, phi2,
+        part, tw1, tw2;
+    int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
+    // stage timing (HIP events on the context stream)
+    bool timing = false;
+    hipEvent_t ev[ST_COUNT + 1] = {};
+    bool ev_ok = false;
+    double stage_ms[2][ST_COUNT] = {};
+    int64_t stage_calls[2] = {0, 0};
+    WPlan last_plan;
+    Tiling last_tiling;
+
+    ~Ctx()
+    {
+        DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
+                          &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2};
+        for (DevBuf *b : bufs)
+            if (b->p) (void)hipFree(b->p);
+        if (ev_ok)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+static int ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (b.bytes >= bytes) return CG_OK;
+    if (b.p) {
+        CG_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return CG_ERR_ALLOC;
+    }
+    b.bytes = bytes;
+    return CG_OK;
+}
+
+template <typename X>
+static X *as(DevBuf &b)
+{
+    return reinterpret_cast<X *>(b.p);
+}
+
+static int key_bits_for(int64_t ntiles)
+{
+    int b = 1;
+    while (((int64_t)1 << b) < ntiles) b++;
+    return b;
+}
+
+static void env_overrides(Ctx &C)
+{
+    if (const char *e = std::getenv("CG_LDS_BUDGET")) C.lds_budget = std::atoi(e);
+    if (const char *e = std::getenv("CG_TILE_GRID")) C.tile_grid = std::atoi(e);
+    if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
+}
+
+struct StageClock {
+    Ctx &C;
+    int dir;
+    explicit StageClock(Ctx &c, int d) : C(c), dir(d) {}
+    int mark(int i)
+    {
+        if (!C.timing) return CG_OK;
+        if (!C.ev_ok) {
+            for (auto &e : C.ev) CG_HIP(hipEventCreate(&e));
+            C.ev_ok = true;
+        }
+        CG_HIP(hipEventRecord(C.ev[i], C.stream));
+        return CG_OK;
+    }
+    int finish()
+    {
+        if (!C.timing) return CG_OK;
+        CG_HIP(hipEventSynchronize(C.ev[ST_COUNT]));
+        for (int i = 0; i < ST_COUNT; i++) {
+            float ms = 0.f;
+            CG_HIP(hipEventElapsedTime(&ms, C.ev[i], C.ev[i + 1]));
+            C.stage_ms[dir][i] += ms;
+        }
+        C.stage_calls[dir]++;
+        return CG_OK;
+    }
+};
+
+#define CG_TRY(x)              \
+    do {                       \
+        int _r = (x);          \
+        if (_r) return _r;     \
+    } while (0)
+
+// twiddle tables (cached per length and precision)
+template <typename T>
+static int twiddles(Ctx &C, int n1, int n2)
+{
+    const int eb = (int)sizeof(cx_t<T>);
+    if (C.tw_bytes != eb || C.tw1_n != n1) {
+        CG_TRY(ensure(C.tw1, (size_t)n1 * eb));
+        CG_TRY(launch_twiddles<T>(as<cx_t<T>>(C.tw1), n1, C.stream));
+        C.tw1_n = n1;
+    }
+    if (C.tw_bytes != eb || C.tw2_n != n2) {
+        CG_TRY(ensure(C.tw2, (size_t)n2 * eb));
+        CG_TRY(launch_twiddles<T>(as<cx_t<T>>(C.tw2), n2, C.stream));
+        C.tw2_n = n2;
+    }
+    C.tw_bytes = eb;
+    return CG_OK;
+}
+
+// sort rows by the tile of their footprint start; leaves the permutation in
+// C.vals and the tile offsets in C.tstart
+template <typename TU>
+static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const Tiling &t)
+{
+    const int64_t ntiles = t.ntiles();
+    CG_TRY(ensure(C.keys, sizeof(uint32_t) * M));
+    CG_TRY(ensure(C.keys2, sizeof(uint32_t) * M));
+    CG_TRY(ensure(C.vals, sizeof(int32_t) * M));
+    CG_TRY(ensure(C.vals2, sizeof(int32_t) * M));
+    const int64_t hs = radix_hist_size(M);
+    CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
+    CG_TRY(ensure(C.scan, sizeof(int32_t) * std::max(scan_tmp_size(hs), scan_tmp_size(ntiles + 1))));
+    CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
+    CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
+    CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
+                             as<int32_t>(C.vals2), M, key_bits_for(ntiles), as<int32_t>(C.hist),
+                             as<int32_t>(C.scan), C.stream));
+    CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart), C.stream));
+    return CG_OK;
+}
+
+static GridArgs grid_args(const WPlan &P, const Tiling &t, int64_t m)
+{
+    GridArgs g{};
+    g.m = (int)m;
+    g.nf1 = P.nf1;
+    g.nf2 = P.nf2;
+    g.nf3 = P.nf3;
+    g.tx = t.tx;
+    g.ty = t.ty;
+    g.ntx = t.ntx;
+    g.nty = t.nty;
+    g.sbx = t.sbx;
+    g.sby = t.sby;
+    g.pc = t.pc;
+    g.nchunk = t.nchunk;
+    g.beta = P.conv.beta;
+    g.es_c = P.conv.es_c;
+    return g;
+}
+
+template <typename T, typename TU>
+static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nx, int ny, double fov, double freq,
+                     const TU *uvw, double eps, double sigma, int sign, bool single)
+{
+    if (nrow < 0 || nrow > (int64_t)INT32_MAX - 1) return CG_ERR_SIZE;
+    CG_TRY(ensure(C.part, sizeof(double) * 2056));
+    double wmin = 0, wmax = 0;
+    const double scale_w = 2.0 * kPi * (freq / kC) * sign;
+    CG_TRY(launch_wminmax<TU>(uvw, nrow, scale_w, as<double>(C.part), wmin, wmax, C.stream));
+    int ier = make_wplan(P, nrow, nx, ny, fov, freq, eps, sigma, sign, single, wmin, wmax);
+    if (ier >= 2) return ier;
+    if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
+    return ier;
+}
+
+// ------------------------------------------------------------ ms2dirty
+
+template <typename T, typename TU, typename TW>
+int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, const TU *uvw,
+                 const cx_t<T> *vis, const TW *wgt, cx_t<T> *dirty, double eps, double sigma,
+                 int sign, bool single)
+{
+    StageClock clk(C, 0);
+    CG_TRY(clk.mark(ST_PLAN));
+    WPlan P;
+    int ier = plan_call<T, TU>(C, P, nrow, nx, ny, fov, freq, uvw, eps, sigma, sign, single);
+    if (ier >= 2) return ier;
+    const int eb = (int)sizeof(cx_t<T>);
+    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, eb, C.lds_budget, C.tile_grid, 0);
+    C.last_plan = P;
+    C.last_tiling = t;
+    C.last_m = nrow;
+    CoordArgs a = make_coord_args(P, t);
+    const int64_t M = nrow;
+    CG_TRY(clk.mark(ST_SORT));
+    if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
+    else {
+        CG_TRY(ensure(C.tstart, sizeof(int32_t) * (t.ntiles() + 1)));
+        CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (t.ntiles() + 1), C.stream));
+    }
+    CG_TRY(clk.mark(ST_RECORDS));
+    CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
+    CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
+    CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, vis, wgt, as<int32_t>(C.vals), M, a, 0,
+                                                as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
+                                                C.stream)));
+    CG_TRY(clk.mark(ST_GRID));
+    CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
+    GridArgs g = grid_args(P, t, M);
+    CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
+                          as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
+    if (C.stop_after == 1) return P.ier;
+    CG_TRY(clk.mark(ST_FFT));
+    CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
+    CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, as<cx_t<T>>(C.tw1),
+                           as<cx_t<T>>(C.tw2), C.stream));
+    if (C.stop_after == 2) return P.ier;
+    CG_TRY(clk.mark(ST_IMAGE));
+    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    CG_TRY(ensure(C.zq, sizeof(double) * nq));
+    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
+    CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
+    CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
+    WArgs w = make_wargs(P);
+    CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
+    CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
+    CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
+    CG_TRY((launch_ms2dirty_epilogue<T, cx_t<T>>(as<cx_t<T>>(C.grid), as<double>(C.zq),
+                                                  as<double>(C.phi3), as<double>(C.phi1),
+                                                  as<double>(C.phi2), dirty, w, C.stream)));
+    CG_TRY(clk.mark(ST_COUNT));
+    CG_TRY(clk.finish());
+    return P.ier;
+}
+
+// ------------------------------------------------------------ dirty2ms
+
+template <typename T, typename TU, typename TW>
+int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, const TU *uvw,
+                 cx_t<T> *vis, const TW *wgt, const cx_t<T> *dirty, double eps, double sigma,
+                 int sign, bool single)
+{
+    StageClock clk(C, 1);
+    CG_TRY(clk.mark(ST_PLAN));
+    WPlan P;
+    int ier = plan_call<T, TU>(C, P, nrow, nx, ny, fov, freq, uvw, eps, sigma, sign, single);
+    if (ier >= 2) return ier;
+    const int eb = (int)sizeof(cx_t<T>);
+    const int kw = P.conv.kw;
+    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
+    C.last_plan = P;
+    C.last_tiling = t;
+    C.last_m = nrow;
+    CoordArgs a = make_coord_args(P, t);
+    const int64_t M = nrow;
+    CG_TRY(clk.mark(ST_SORT));
+    if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
+    CG_TRY(clk.mark(ST_RECORDS));
+    CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
+    CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
+    CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, (const cx_t<T> *)nullptr, wgt,
+                                                as<int32_t>(C.vals), M, a, 1, as<geo_t<T>>(C.geo),
+                                                as<cx_t<T>>(C.val), C.stream)));
+    CG_TRY(clk.mark(ST_IMAGE));
+    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    CG_TRY(ensure(C.zq, sizeof(double) * nq));
+    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
+    CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
+    CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
+    CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
+    WArgs w = make_wargs(P);
+    CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
+    CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
+    CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
+    CG_TRY((launch_dirty2ms_prologue<T, cx_t<T>>(dirty, as<double>(C.zq), as<double>(C.phi3),
+                                                  as<double>(C.phi1), as<double>(C.phi2),
+                                                  as<cx_t<T>>(C.grid), w, C.stream)));
+    if (C.stop_after == 1) return P.ier;
+    CG_TRY(clk.mark(ST_FFT));
+    CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
+    CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, as<cx_t<T>>(C.tw1),
+                           as<cx_t<T>>(C.tw2), C.stream));
+    if (C.stop_after == 2) return P.ier;
+    CG_TRY(clk.mark(ST_GRID));
+    GridArgs g = grid_args(P, t, M);
+    if (M > 0)
+        CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
+                                          as<int32_t>(C.vals), as<int32_t>(C.tstart),
+                                          as<cx_t<T>>(C.grid), vis, g, kw, C.stream)));
+    CG_TRY(clk.mark(ST_COUNT));
+    // stage order differs from ms2dirty: report the grid stage under ST_GRID
+    if (C.timing) {
+        CG_HIP(hipEventSynchronize(C.ev[ST_COUNT]));
+        float ms[ST_COUNT] = {};
+        CG_HIP(hipEventElapsedTime(&ms[ST_PLAN], C.ev[ST_PLAN], C.ev[ST_SORT]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_SORT], C.ev[ST_SORT], C.ev[ST_RECORDS]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_RECORDS], C.ev[ST_RECORDS], C.ev[ST_IMAGE]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_IMAGE], C.ev[ST_IMAGE], C.ev[ST_FFT]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_GRID]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_GRID], C.ev[ST_GRID], C.ev[ST_COUNT]));
+        for (int i = 0; i < ST_COUNT; i++) C.stage_ms[1][i] += ms[i];
+        C.stage_calls[1]++;
+    }
+    return P.ier;
+}
+
+// ------------------------------------------------------------ 2-D NUFFT
+
+template <typename T, typename TU>
+int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *c, int N1, int N2,
+                cx_t<T> *fk, double eps, double sigma, bool single)
+{
+    if (M < 0 || M > (int64_t)INT32_MAX - 1 || N1 <= 0 || N2 <= 0) return CG_ERR_ARG;
+    StageClock clk(C, type == 1 ? 0 : 1);
+    CG_TRY(clk.mark(ST_PLAN));
+    ConvOpts conv;
+    int ier = setup_conv_opts(eps, sigma, single, conv);
+    if (ier >= 2) return ier;
+    WPlan P;
+    P.N1 = N1;
+    P.N2 = N2;
+    P.nf1 = num_cells(N1, sigma, conv.kw);
+    P.nf2 = num_cells(N2, sigma, conv.kw);
+    P.nf3 = 1;
+    P.conv = conv;
+    P.ier = ier;
+    if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
+    const int eb = (int)sizeof(cx_t<T>);
+    const int kw = conv.kw;
+    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_grid, 0)
+                         : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
+    C.last_plan = P;
+    C.last_tiling = t;
+    CoordArgs a{};
+    a.nf1 = P.nf1;
+    a.nf2 = P.nf2;
+    a.nf3 = 1;
+    a.kw = kw;
+    a.tx = t.tx;
+    a.ty = t.ty;
+    a.ntx = t.ntx;
+    a.nty = t.nty;
+    a.pirange = 1;
+    a.has_w = 0;
+    const int64_t ntiles = t.ntiles();
+    CG_TRY(clk.mark(ST_SORT));
+    CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
+    if (M > 0) {
+        CG_TRY(ensure(C.keys, sizeof(uint32_t) * M));
+        CG_TRY(ensure(C.keys2, sizeof(uint32_t) * M));
+        CG_TRY(ensure(C.vals, sizeof(int32_t) * M));
+        CG_TRY(ensure(C.vals2, sizeof(int32_t) * M));
+        const int64_t hs = radix_hist_size(M);
+        CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
+        CG_TRY(ensure(C.scan, sizeof(int32_t) * scan_tmp_size(hs)));
+        CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
+        CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
+                                 as<int32_t>(C.vals2), M, key_bits_for(ntiles), as<int32_t>(C.hist),
+                                 as<int32_t>(C.scan), C.stream));
+        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart), C.stream));
+    } else {
+        CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (ntiles + 1), C.stream));
+    }
+    CG_TRY(clk.mark(ST_RECORDS));
+    CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
+    CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
+    CG_TRY((launch_records_xy<T, TU, cx_t<T>>(x, y, type == 1 ? c : (const cx_t<T> *)nullptr,
+                                              as<int32_t>(C.vals), M, a, as<geo_t<T>>(C.geo),
+                                              as<cx_t<T>>(C.val), C.stream)));
+    CG_TRY(ensure(C.grid, (size_t)P.nf1 * P.nf2 * eb));
+    CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
+    CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
+    Quad qd = make_quad(conv);
+    CG_TRY(launch_fseries_grid(qd, P.nf1, as<double>(C.phi1), C.stream));
+    CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
+    CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
+    GridArgs g = grid_args(P, t, M);
+    if (type == 1) {
+        CG_TRY(clk.mark(ST_GRID));
+        CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
+                              as<cx_t<T>>(C.grid), g, kw, C.stream));
+        CG_TRY(clk.mark(ST_FFT));
+        CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, as<cx_t<T>>(C.tw1),
+                               as<cx_t<T>>(C.tw2), C.stream));
+        CG_TRY(clk.mark(ST_IMAGE));
+        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
+                                            as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 1, C.stream)));
+    } else {
+        CG_TRY(clk.mark(ST_GRID));
+        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
+                                            as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 2, C.stream)));
+        CG_TRY(clk.mark(ST_FFT));
+        CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, as<cx_t<T>>(C.tw1),
+                               as<cx_t<T>>(C.tw2), C.stream));
+        CG_TRY(clk.mark(ST_IMAGE));
+        if (M > 0)
+            CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
+                                              as<int32_t>(C.vals), as<int32_t>(C.tstart),
+                                              as<cx_t<T>>(C.grid), c, g, kw, C.stream)));
+    }
+    CG_TRY(clk.mark(ST_COUNT));
+    CG_TRY(clk.finish());
+    return ier;
+}
+
+// ------------------------------------------------------------ host wrappers
+
+struct HostBuf {
+    void *p = nullptr;
+    ~HostBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes)
+    {
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return CG_ERR_ALLOC;
+        }
+        return CG_OK;
+    }
+};
+
+// host-pointer ms2dirty / dirty2ms: H2D, run, D2H, free (reference semantics)
+template <typename T, typename TU, typename TW>
+static int host_ms2dirty(int nrow, int nx, int ny, double fov, double freq, const TU *uvw,
+                         const void *vis, const TW *wgt, void *dirty, double eps, double sigma,
+                         int sign, bool single)
+{
+    if (nrow < 0 || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis)) return CG_ERR_ARG;
+    try {
+        Ctx C;
+        env_overrides(C);
+        const size_t M = (size_t)nrow;
+        HostBuf du, dv, dw, dd;
+        CG_TRY(du.alloc(sizeof(TU) * 3 * M));
+        CG_TRY(dv.alloc(sizeof(cx_t<T>) * M));
+        if (wgt) CG_TRY(dw.alloc(sizeof(TW) * M));
+        CG_TRY(dd.alloc(sizeof(cx_t<T>) * (size_t)nx * ny));
+        if (M) {
+            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * M, hipMemcpyHostToDevice));
+            CG_HIP(hipMemcpy(dv.p, vis, sizeof(cx_t<T>) * M, hipMemcpyHostToDevice));
+            if (wgt) CG_HIP(hipMemcpy(dw.p, wgt, sizeof(TW) * M, hipMemcpyHostToDevice));
+        }
+        int ier = ms2dirty_dev<T, TU, TW>(C, nrow, nx, ny, fov, freq, (const TU *)du.p,
+                                          (const cx_t<T> *)dv.p, wgt ? (const TW *)dw.p : nullptr,
+                                          (cx_t<T> *)dd.p, eps, sigma, sign, single);
+        if (ier >= 2) return ier;
+        CG_HIP(hipMemcpy(dirty, dd.p, sizeof(cx_t<T>) * (size_t)nx * ny, hipMemcpyDeviceToHost));
+        return ier;
+    } catch (const std::bad_alloc &) {
+        return CG_ERR_ALLOC;
+    }
+}
+
+template <typename T, typename TU, typename TW>
+static int host_dirty2ms(int nrow, int nx, int ny, double fov, double freq, const TU *uvw,
+                         void *vis, const TW *wgt, const void *dirty, double eps, double sigma,
+                         int sign, bool single)
+{
+    if (nrow < 0 || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis)) return CG_ERR_ARG;
+    try {
+        Ctx C;
+        env_overrides(C);
+        const size_t M = (size_t)nrow;
+        HostBuf du, dv, dw, dd;
+        CG_TRY(du.alloc(sizeof(TU) * 3 * M));
+        CG_TRY(dv.alloc(sizeof(cx_t<T>) * M));
+        if (wgt) CG_TRY(dw.alloc(sizeof(TW) * M));
+        CG_TRY(dd.alloc(sizeof(cx_t<T>) * (size_t)nx * ny));
+        if (M) {
+            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * M, hipMemcpyHostToDevice));
+            if (wgt) CG_HIP(hipMemcpy(dw.p, wgt, sizeof(TW) * M, hipMemcpyHostToDevice));
+        }
+        CG_HIP(hipMemcpy(dd.p, dirty, sizeof(cx_t<T>) * (size_t)nx * ny, hipMemcpyHostToDevice));
+        int ier = dirty2ms_dev<T, TU, TW>(C, nrow, nx, ny, fov, freq, (const TU *)du.p,
+                                          (cx_t<T> *)dv.p, wgt ? (const TW *)dw.p : nullptr,
+                                          (const cx_t<T> *)dd.p, eps, sigma, sign, single);
+        if (ier >= 2) return ier;
+        if (M) CG_HIP(hipMemcpy(vis, dv.p, sizeof(cx_t<T>) * M, hipMemcpyDeviceToHost));
+        return ier;
+    } catch (const std::bad_alloc &) {
+        return CG_ERR_ALLOC;
+    }
+}
+
+template <typename T, typename TU>
+static int host_nufft2d(int type, int M, const TU *x, const TU *y, void *c, int N1, int N2,
+                        void *fk, double eps, double sigma, bool single)
+{
+    if (M < 0 || N1 <= 0 || N2 <= 0 || !fk || (M > 0 && (!x || !y || !c))) return CG_ERR_ARG;
+    try {
+        Ctx C;
+        env_overrides(C);
+        HostBuf dx, dy, dc, dk;
+        CG_TRY(dx.alloc(sizeof(TU) * (size_t)M));
+        CG_TRY(dy.alloc(sizeof(TU) * (size_t)M));
+        CG_TRY(dc.alloc(sizeof(cx_t<T>) * (size_t)M));
+        CG_TRY(dk.alloc(sizeof(cx_t<T>) * (size_t)N1 * N2));
+        if (M) {
+            CG_HIP(hipMemcpy(dx.p, x, sizeof(TU) * (size_t)M, hipMemcpyHostToDevice));
+            CG_HIP(hipMemcpy(dy.p, y, sizeof(TU) * (size_t)M, hipMemcpyHostToDevice));
+            if (type == 1) CG_HIP(hipMemcpy(dc.p, c, sizeof(cx_t<T>) * (size_t)M, hipMemcpyHostToDevice));
+        }
+        if (type == 2)
+            CG_HIP(hipMemcpy(dk.p, fk, sizeof(cx_t<T>) * (size_t)N1 * N2, hipMemcpyHostToDevice));
+        int ier = nufft2d_dev<T, TU>(C, type, M, (const TU *)dx.p, (const TU *)dy.p,
+                                     (cx_t<T> *)dc.p, N1, N2, (cx_t<T> *)dk.p, eps, sigma, single);
+        if (ier >= 2) return ier;
+        if (type == 1)
+            CG_HIP(hipMemcpy(fk, dk.p, sizeof(cx_t<T>) * (size_t)N1 * N2, hipMemcpyDeviceToHost));
+        else if (M)
+            CG_HIP(hipMemcpy(c, dc.p, sizeof(cx_t<T>) * (size_t)M, hipMemcpyDeviceToHost));
+        return ier;
+    } catch (const std::bad_alloc &) {
+        return CG_ERR_ALLOC;
+    }
+}
+
+}  // namespace cg
+
+using namespace cg;
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+// reference entry points (include/cugridder.h:20-46), double precision
+int ms2dirty_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
+               double *vis, double *wgt, double *dirty, double epsilon, double sigma, int sign)
+{
+    return host_ms2dirty<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                                 dirty, epsilon, sigma, sign, false);
+}
+int ms2dirty_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
+               double *vis, double *dirty, double epsilon, double sigma, int sign)
+{
+    return host_ms2dirty<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis,
+                                                 nullptr, dirty, epsilon, sigma, sign, false);
+}
+int dirty2ms_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
+               double *vis, double *wgt, double *dirty, double epsilon, double sigma, int sign)
+{
+    return host_dirty2ms<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                                 dirty, epsilon, sigma, sign, false);
+}
+int dirty2ms_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
+               double *vis, double *dirty, double epsilon, double sigma, int sign)
+{
+    return host_dirty2ms<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis,
+                                                 nullptr, dirty, epsilon, sigma, sign, false);
+}
+// single precision (float ABI, FP32 uvw)
+int ms2dirtyf_2(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
+                float *wgt, float *dirty, float epsilon, float sigma, int sign)
+{
+    return host_ms2dirty<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                              dirty, epsilon, sigma, sign, true);
+}
+int ms2dirtyf_1(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
+                float *dirty, float epsilon, float sigma, int sign)
+{
+    return host_ms2dirty<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr,
+                                              dirty, epsilon, sigma, sign, true);
+}
+int dirty2msf_2(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
+                float *wgt, float *dirty, float epsilon, float sigma, int sign)
+{
+    return host_dirty2ms<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                              dirty, epsilon, sigma, sign, true);
+}
+int dirty2msf_1(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
+                float *dirty, float epsilon, float sigma, int sign)
+{
+    return host_dirty2ms<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr,
+                                              dirty, epsilon, sigma, sign, true);
+}
+
+// mixed precision: FP64 uvw, complex64 vis/dirty, FP32 wgt (may be NULL)
+int ms2dirty_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw,
+                   const float *vis, const float *wgt, float *dirty, double epsilon, double sigma,
+                   int sign)
+{
+    return host_ms2dirty<float, double, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                               dirty, epsilon, sigma, sign, true);
+}
+int dirty2ms_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw,
+                   float *vis, const float *wgt, const float *dirty, double epsilon, double sigma,
+                   int sign)
+{
+    return host_dirty2ms<float, double, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+                                               dirty, epsilon, sigma, sign, true);
+}
+
+// 2-D NUFFT type 1 / 2 (points in [-pi,pi)), host pointers
+int nufft2d1(int M, const double *x, const double *y, const double *c, int N1, int N2, double *fk,
+             double epsilon, double sigma)
+{
+    return host_nufft2d<double, double>(1, M, x, y, (void *)c, N1, N2, fk, epsilon, sigma, false);
+}
+int nufft2d2(int M, const double *x, const double *y, double *c, int N1, int N2, const double *fk,
+             double epsilon, double sigma)
+{
+    return host_nufft2d<double, double>(2, M, x, y, c, N1, N2, (void *)fk, epsilon, sigma, false);
+}
+int nufft2d1f(int M, const float *x, const float *y, const float *c, int N1, int N2, float *fk,
+              float epsilon, float sigma)
+{
+    return host_nufft2d<float, float>(1, M, x, y, (void *)c, N1, N2, fk, epsilon, sigma, true);
+}
+int nufft2d2f(int M, const float *x, const float *y, float *c, int N1, int N2, const float *fk,
+              float epsilon, float sigma)
+{
+    return host_nufft2d<float, float>(2, M, x, y, c, N1, N2, (void *)fk, epsilon, sigma, true);
+}
+
+// ---- contexts and device-pointer entry points
+
+int cg_ctx_create(void **ctx, void *stream)
+{
+    if (!ctx) return CG_ERR_ARG;
+    try {
+        Ctx *C = new Ctx();
+        C->stream = (hipStream_t)stream;
+        env_overrides(*C);
+        *ctx = C;
+        return CG_OK;
+    } catch (const std::bad_alloc &) {
+        return CG_ERR_ALLOC;
+    }
+}
+
+int cg_ctx_destroy(void *ctx)
+{
+    delete (Ctx *)ctx;
+    return CG_OK;
+}
+
+int cg_ctx_set_option(void *ctx, const char *name, int value)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !name) return CG_ERR_ARG;
+    if (!std::strcmp(name, "timing")) C->timing = value != 0;
+    else if (!std::strcmp(name, "lds_budget")) C->lds_budget = value;
+    else if (!std::strcmp(name, "tile_grid")) C->tile_grid = value;
+    else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
+    else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
+    else return CG_ERR_ARG;
+    return CG_OK;
+}
+
+// stage times summed over calls: out[dir*6 + stage] (ms); returns calls
+int cg_ctx_stage_times(void *ctx, double *out, long long *calls)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !out) return CG_ERR_ARG;
+    for (int d = 0; d < 2; d++)
+        for (int i = 0; i < ST_COUNT; i++) out[d * ST_COUNT + i] = C->stage_ms[d][i];
+    if (calls) {
+        calls[0] = C->stage_calls[0];
+        calls[1] = C->stage_calls[1];
+    }
+    return CG_OK;
+}
+
+int cg_ctx_reset_times(void *ctx)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C) return CG_ERR_ARG;
+    std::memset(C->stage_ms, 0, sizeof(C->stage_ms));
+    C->stage_calls[0] = C->stage_calls[1] = 0;
+    return CG_OK;
+}
+
+// last plan: out[0..15] = nf1 nf2 nf3 kw beta es_c pix_x pix_y i_center
+// i_half_width o_center o_half_width gamma h tx ty
+int cg_ctx_last_plan(void *ctx, double *out)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !out) return CG_ERR_ARG;
+    const WPlan &P = C->last_plan;
+    double v[16] = {(double)P.nf1, (double)P.nf2, (double)P.nf3, (double)P.conv.kw, P.conv.beta,
+                    P.conv.es_c, P.pix_x, P.pix_y, P.i_center, P.i_half_width, P.o_center,
+                    P.o_half_width, P.gamma, P.h, (double)C->last_tiling.tx,
+                    (double)C->last_tiling.ty};
+    std::memcpy(out, v, sizeof(v));
+    return CG_OK;
+}
+
+// prec: 0 = complex64 vis/image with FP32 gridding and FFT, 1 = complex128;
+// uvw_prec / wgt_prec: 0 float, 1 double.  d_wgt may be NULL.
+int cg_ms2dirty_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
+                    double freq, const void *d_uvw, int uvw_prec, const void *d_vis,
+                    const void *d_wgt, int wgt_prec, void *d_dirty, double epsilon, double sigma,
+                    int sign)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C) return CG_ERR_ARG;
+    bool single = prec == 0;
+    if (prec == 0 && uvw_prec == 1 && wgt_prec == 0)
+        return ms2dirty_dev<float, double, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                  (const double *)d_uvw, (const float2 *)d_vis,
+                                                  (const float *)d_wgt, (float2 *)d_dirty, epsilon,
+                                                  sigma, sign, single);
+    if (prec == 0 && uvw_prec == 0 && wgt_prec == 0)
+        return ms2dirty_dev<float, float, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                 (const float *)d_uvw, (const float2 *)d_vis,
+                                                 (const float *)d_wgt, (float2 *)d_dirty, epsilon,
+                                                 sigma, sign, single);
+    if (prec == 1 && uvw_prec == 1 && wgt_prec == 1)
+        return ms2dirty_dev<double, double, double>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                    (const double *)d_uvw, (const double2 *)d_vis,
+                                                    (const double *)d_wgt, (double2 *)d_dirty,
+                                                    epsilon, sigma, sign, single);
+    return CG_ERR_ARG;
+}
+
+int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
+                    double freq, const void *d_uvw, int uvw_prec, void *d_vis, const void *d_wgt,
+                    int wgt_prec, const void *d_dirty, double epsilon, double sigma, int sign)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C) return CG_ERR_ARG;
+    bool single = prec == 0;
+    if (prec == 0 && uvw_prec == 1 && wgt_prec == 0)
+        return dirty2ms_dev<float, double, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                  (const double *)d_uvw, (float2 *)d_vis,
+                                                  (const float *)d_wgt, (const float2 *)d_dirty,
+                                                  epsilon, sigma, sign, single);
+    if (prec == 0 && uvw_prec == 0 && wgt_prec == 0)
+        return dirty2ms_dev<float, float, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                 (const float *)d_uvw, (float2 *)d_vis,
+                                                 (const float *)d_wgt, (const float2 *)d_dirty,
+                                                 epsilon, sigma, sign, single);
+    if (prec == 1 && uvw_prec == 1 && wgt_prec == 1)
+        return dirty2ms_dev<double, double, double>(*C, nrow, nxdirty, nydirty, fov, freq,
+                                                    (const double *)d_uvw, (double2 *)d_vis,
+                                                    (const double *)d_wgt, (const double2 *)d_dirty,
+                                                    epsilon, sigma, sign, single);
+    return CG_ERR_ARG;
+}
+
+int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x, const void *d_y,
+                   void *d_c, int N1, int N2, void *d_fk, double epsilon, double sigma)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || (type != 1 && type != 2)) return CG_ERR_ARG;
+    if (prec == 0)
+        return nufft2d_dev<float, float>(*C, type, M, (const float *)d_x, (const float *)d_y,
+                                         (float2 *)d_c, N1, N2, (float2 *)d_fk, epsilon, sigma,
+                                         true);
+    if (prec == 1)
+        return nufft2d_dev<double, double>(*C, type, M, (const double *)d_x, (const double *)d_y,
+                                           (double2 *)d_c, N1, N2, (double2 *)d_fk, epsilon, sigma,
+                                           false);
+    return CG_ERR_ARG;
+}
+
+// ---- host-only planner query (no device calls): out[16] = nf1 nf2 nf3 kw
+// beta es_c pix_x pix_y i_center i_half_width o_center o_half_width gamma h
+// f_over_c ier, for a given range of the scaled w (w * 2 pi f/c * sign)
+int cg_plan_query(int nrow, int nxdirty, int nydirty, double fov, double freq, double wmin,
+                  double wmax, double epsilon, double sigma, int sign, int single, double *out)
+{
+    if (!out) return CG_ERR_ARG;
+    WPlan P;
+    int ier = make_wplan(P, nrow, nxdirty, nydirty, fov, freq, epsilon, sigma, sign, single != 0,
+                         wmin, wmax);
+    double v[16] = {(double)P.nf1, (double)P.nf2, (double)P.nf3, (double)P.conv.kw, P.conv.beta,
+                    P.conv.es_c, P.pix_x, P.pix_y, P.i_center, P.i_half_width, P.o_center,
+                    P.o_half_width, P.gamma, P.h, P.f_over_c, (double)ier};
+    std::memcpy(out, v, sizeof(v));
+    return ier;
+}
+
+// ---- test hooks: device pointers into the context workspace (valid until
+// the next call on the context)
+int cg_ctx_grid(void *ctx, void **ptr, long long *cells)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !ptr || !cells) return CG_ERR_ARG;
+    *ptr = C->grid.p;
+    *cells = (long long)C->last_plan.nf1 * C->last_plan.nf2 * C->last_plan.nf3;
+    return CG_OK;
+}
+
+// sorted keys (uint32), permutation (int32) and tile starts (int32) of the
+// last ms2dirty/dirty2ms call; n[0] = rows, n[1] = tiles
+int cg_ctx_sorted(void *ctx, void **keys, void **perm, void **tstart, long long *n)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !keys || !perm || !tstart || !n) return CG_ERR_ARG;
+    *keys = C->keys.p;
+    *perm = C->vals.p;
+    *tstart = C->tstart.p;
+    n[0] = C->last_m;
+    n[1] = C->last_tiling.ntiles();
+    return CG_OK;
+}
+
+// in-place 2-D FFT of batch planes [batch][n2][n1] on the context stream
+int cg_fft2d_dev(void *ctx, int prec, void *d_data, int batch, int n2, int n1, int dir)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !d_data || batch < 0 || n1 <= 0 || n2 <= 0 || (dir != 1 && dir != -1))
+        return CG_ERR_ARG;
+    if (prec == 0) {
+        if (!fft_supported<float>(n1) || !fft_supported<float>(n2)) return CG_ERR_SIZE;
+        CG_TRY(twiddles<float>(*C, n1, n2));
+        return launch_fft2d<float>((float2 *)d_data, batch, n2, n1, dir, as<float2>(C->tw1),
+                                   as<float2>(C->tw2), C->stream);
+    }
+    if (prec == 1) {
+        if (!fft_supported<double>(n1) || !fft_supported<double>(n2)) return CG_ERR_SIZE;
+        CG_TRY(twiddles<double>(*C, n1, n2));
+        return launch_fft2d<double>((double2 *)d_data, batch, n2, n1, dir, as<double2>(C->tw1),
+                                    as<double2>(C->tw2), C->stream);
+    }
+    return CG_ERR_ARG;
+}
+
+// stable sort of (key, value) pairs by the low key_bits bits, in place
+int cg_sort_pairs_dev(void *ctx, void *d_keys, void *d_vals, int m, int key_bits)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || m < 0 || key_bits < 1 || key_bits > 32) return CG_ERR_ARG;
+    if (m == 0) return CG_OK;
+    CG_TRY(ensure(C->keys2, sizeof(uint32_t) * m));
+    CG_TRY(ensure(C->vals2, sizeof(int32_t) * m));
+    const int64_t hs = radix_hist_size(m);
+    CG_TRY(ensure(C->hist, sizeof(int32_t) * hs));
+    CG_TRY(ensure(C->scan, sizeof(int32_t) * scan_tmp_size(hs)));
+    return launch_radix_sort((uint32_t *)d_keys, (int32_t *)d_vals, as<uint32_t>(C->keys2),
+                             as<int32_t>(C->vals2), m, key_bits, as<int32_t>(C->hist),
+                             as<int32_t>(C->scan), C->stream);
+}
+
+// device-to-device copy on a stream (test hook)
+int cg_memcpy_dev(void *dst, const void *src, void *stream, size_t bytes)
+{
+    if (!bytes) return CG_OK;
+    CG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return CG_OK;
+}
+
+int cg_version(void) { return 100; }
+
+}  // extern "C"

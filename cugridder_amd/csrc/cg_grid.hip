@@ -1,0 +1,310 @@
+// ES-kernel gridding (ms2dirty) and degridding (dirty2ms) on LDS tiles.
+//
+// Gridding computes the reference's periodic 3-D spreading sum
+//   F[p][b2][b1] = sum_j c_j psi(b1-g1_j) psi(b2-g2_j) psi(p-g3_j)
+// (src/fourier/conv.cu:522-627 conv_3d_outputdriven_t1, equal to the
+// input-driven conv.cu:166-244), psi(d) = exp(beta sqrt(1 - c d^2)) for
+// |d| < kw/2 (conv.cu:32-36, unnormalised as in the reference).
+//
+// MI355X design: one workgroup owns a tx*ty tile of grid cells for a chunk of
+// w-planes and accumulates it in LDS with ds_add atomics; it reads the points
+// of the bins whose footprints reach the tile (points are binned by the tile
+// of their footprint start, so at most sbx*sby bins feed a tile) and writes
+// the finished tile once with coalesced stores.  No HBM atomics and no grid
+// memset: every cell is written exactly once.
+//
+// Degridding is the adjoint interpolation (src/fourier/interp.cu:132-210):
+// a workgroup stages its tile plus a (kw-1) halo of every plane in LDS and
+// each thread interpolates its points from LDS; the result is multiplied by
+// the per-point factor (weight and w-phase, ra_exec.cu:55-93) and scattered
+// back to the caller's row order.
+#include "cg_internal.hpp"
+
+namespace cg {
+
+__device__ __forceinline__ uint32_t geo_bits(float w) { return __float_as_uint(w); }
+__device__ __forceinline__ uint32_t geo_bits(double w) { return (uint32_t)__double_as_longlong(w); }
+
+template <typename T>
+__device__ __forceinline__ T es_val(T d, T beta, T c, T hw);
+template <>
+__device__ __forceinline__ float es_val<float>(float d, float beta, float c, float hw)
+{
+    return fabsf(d) < hw ? expf(beta * sqrtf(1.0f - c * d * d)) : 0.0f;
+}
+template <>
+__device__ __forceinline__ double es_val<double>(double d, double beta, double c, double hw)
+{
+    return fabs(d) < hw ? exp(beta * sqrt(1.0 - c * d * d)) : 0.0;
+}
+
+int grid_lds_bytes(const GridArgs &g, int elem) { return g.pc * g.tx * g.ty * elem; }
+int degrid_lds_bytes(const GridArgs &g, int kw, int elem)
+{
+    return g.pc * (g.tx + kw - 1) * (g.ty + kw - 1) * elem;
+}
+
+// ------------------------------------------------------------ gridding
+
+template <typename T, int KW>
+__global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
+                                              const cx_t<T> *__restrict__ val,
+                                              const int32_t *__restrict__ tile_start,
+                                              cx_t<T> *__restrict__ fw, GridArgs g, int is2d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *acc = reinterpret_cast<T *>(smem_raw);
+    const int chunk = blockIdx.x % g.nchunk;
+    const int tile = blockIdx.x / g.nchunk;
+    const int bx = tile % g.ntx, by = tile / g.ntx;
+    const int p0 = chunk * g.pc;
+    const int np = min(g.pc, g.nf3 - p0);
+    const int cells = np * g.ty * g.tx;
+    for (int k = threadIdx.x; k < 2 * cells; k += blockDim.x) acc[k] = (T)0;
+    __syncthreads();
+
+    const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
+    const int nzt = is2d ? 1 : KW;
+    for (int sy = 0; sy < g.sby; sy++) {
+        int sby_ = by - sy;
+        if (sby_ < 0) sby_ += g.nty;
+        int oy = (sby_ - by) * g.ty;
+        if (oy < 0) oy += g.nf2;
+        for (int sx = 0; sx < g.sbx; sx++) {
+            int sbx_ = bx - sx;
+            if (sbx_ < 0) sbx_ += g.ntx;
+            int ox = (sbx_ - bx) * g.tx;
+            if (ox < 0) ox += g.nf1;
+            const int s = sby_ * g.ntx + sbx_;
+            const int beg = tile_start[s], end = tile_start[s + 1];
+            for (int i = beg + (int)threadIdx.x; i < end; i += blockDim.x) {
+                const geo_t<T> G = geo[i];
+                const uint32_t pk = geo_bits(G.w);
+                int rx = ox + (int)(pk & 0xffu);
+                if (rx >= g.nf1) rx -= g.nf1;
+                int ry = oy + (int)((pk >> 8) & 0xffu);
+                if (ry >= g.nf2) ry -= g.nf2;
+                if (rx >= g.tx && rx + KW - 1 < g.nf1) continue;
+                if (ry >= g.ty && ry + KW - 1 < g.nf2) continue;
+                int rz = (int)(pk >> 16) - p0;
+                if (rz < 0) rz += g.nf3;
+                if (!is2d && rz >= np && rz + KW - 1 < g.nf3) continue;
+                const cx_t<T> v = val[i];
+                T kx[KW], ky[KW], kz[KW];
+#pragma unroll
+                for (int t = 0; t < KW; t++) {
+                    kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
+                    ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
+                    kz[t] = is2d ? (T)(t == 0) : es_val<T>(G.z + (T)t, beta, ec, hw);
+                }
+                if (is2d) rz = 0;
+                for (int tz = 0; tz < nzt; tz++) {
+                    int zz = rz + tz;
+                    if (zz >= g.nf3) zz -= g.nf3;
+                    if (zz >= np) continue;
+                    const T vzr = v.x * kz[tz], vzi = v.y * kz[tz];
+#pragma unroll
+                    for (int t2 = 0; t2 < KW; t2++) {
+                        int yy = ry + t2;
+                        if (yy >= g.nf2) yy -= g.nf2;
+                        if (yy >= g.ty) continue;
+                        const T vyr = vzr * ky[t2], vyi = vzi * ky[t2];
+                        T *row = acc + 2 * ((zz * g.ty + yy) * g.tx);
+#pragma unroll
+                        for (int t1 = 0; t1 < KW; t1++) {
+                            int xx = rx + t1;
+                            if (xx >= g.nf1) xx -= g.nf1;
+                            if (xx >= g.tx) continue;
+                            atomicAdd(row + 2 * xx, vyr * kx[t1]);
+                            atomicAdd(row + 2 * xx + 1, vyi * kx[t1]);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int plane_cells = g.ty * g.tx;
+    for (int k = threadIdx.x; k < cells; k += blockDim.x) {
+        int lz = k / plane_cells;
+        int rem = k - lz * plane_cells;
+        int y = rem / g.tx, x = rem - y * g.tx;
+        size_t gi = ((size_t)(p0 + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
+        fw[gi] = mkc(acc[2 * k], acc[2 * k + 1]);
+    }
+}
+
+template <typename T, int KW>
+static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, cx_t<T> *fw,
+                   const GridArgs &g, hipStream_t s)
+{
+    int is2d = g.nf3 == 1 ? 1 : 0;
+    size_t lds = (size_t)grid_lds_bytes(g, (int)sizeof(cx_t<T>));
+    unsigned nblk = (unsigned)((int64_t)g.ntx * g.nty * g.nchunk);
+    if (lds > 65536)
+        CG_HIP(hipFuncSetAttribute((const void *)k_grid<T, KW>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_grid<T, KW>), dim3(nblk), dim3(256), lds, s, geo, val, ts, fw, g, is2d);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+template <typename T>
+int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, cx_t<T> *fw,
+                const GridArgs &g, int kw, hipStream_t s)
+{
+    switch (kw) {
+#define CG_CASE(K) \
+    case K: return grid_kw<T, K>(geo, val, ts, fw, g, s);
+        CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8) CG_CASE(9)
+        CG_CASE(10) CG_CASE(11) CG_CASE(12) CG_CASE(13) CG_CASE(14) CG_CASE(15) CG_CASE(16)
+#undef CG_CASE
+    default: return CG_ERR_ARG;
+    }
+}
+
+// ------------------------------------------------------------ degridding
+
+template <typename T, int KW, int PPT>
+__global__ __launch_bounds__(256) void k_degrid(const geo_t<T> *__restrict__ geo,
+                                                const cx_t<T> *__restrict__ fac,
+                                                const int32_t *__restrict__ perm,
+                                                const int32_t *__restrict__ tile_start,
+                                                const cx_t<T> *__restrict__ fw,
+                                                cx_t<T> *__restrict__ out, GridArgs g, int is2d)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx_t<T> *lds = reinterpret_cast<cx_t<T> *>(smem_raw);
+    const int tile = blockIdx.x;
+    const int beg = tile_start[tile], end = tile_start[tile + 1];
+    if (beg == end) return;
+    const int bx = tile % g.ntx, by = tile / g.ntx;
+    const int hx = g.tx + KW - 1, hy = g.ty + KW - 1;
+    const int x0 = bx * g.tx, y0 = by * g.ty;
+    const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
+    const bool single_stage = g.nchunk == 1;
+    const int nzt = is2d ? 1 : KW;
+
+    auto stage = [&](int p0, int np) {
+        const int pcells = hy * hx;
+        const int n = np * pcells;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            int lz = k / pcells;
+            int rem = k - lz * pcells;
+            int y = rem / hx, x = rem - y * hx;
+            int gx = x0 + x;
+            if (gx >= g.nf1) gx -= g.nf1;
+            int gy = y0 + y;
+            if (gy >= g.nf2) gy -= g.nf2;
+            lds[k] = fw[((size_t)(p0 + lz) * g.nf2 + gy) * g.nf1 + gx];
+        }
+    };
+
+    if (single_stage) {
+        stage(0, g.nf3);
+        __syncthreads();
+    }
+    for (int base = beg; base < end; base += (int)blockDim.x * PPT) {
+        T ar[PPT], ai[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; k++) ar[k] = ai[k] = (T)0;
+        for (int c = 0; c < g.nchunk; c++) {
+            const int p0 = c * g.pc;
+            const int np = min(g.pc, g.nf3 - p0);
+            if (!single_stage) {
+                __syncthreads();
+                stage(p0, np);
+                __syncthreads();
+            }
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                const int i = base + (int)threadIdx.x + k * (int)blockDim.x;
+                if (i >= end) continue;
+                const geo_t<T> G = geo[i];
+                const uint32_t pk = geo_bits(G.w);
+                const int lx = min((int)(pk & 0xffu), g.tx - 1);
+                const int ly = min((int)((pk >> 8) & 0xffu), g.ty - 1);
+                int rz = (int)(pk >> 16) - p0;
+                if (rz < 0) rz += g.nf3;
+                if (is2d) rz = 0;
+                else if (rz >= np && rz + KW - 1 < g.nf3) continue;
+                T kx[KW], ky[KW];
+#pragma unroll
+                for (int t = 0; t < KW; t++) {
+                    kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
+                    ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
+                }
+                for (int tz = 0; tz < nzt; tz++) {
+                    int zz = rz + tz;
+                    if (zz >= g.nf3) zz -= g.nf3;
+                    if (zz >= np) continue;
+                    const T kzv = is2d ? (T)1 : es_val<T>(G.z + (T)tz, beta, ec, hw);
+                    const cx_t<T> *pl = lds + (size_t)zz * hy * hx + (size_t)ly * hx + lx;
+                    T sr = 0, si = 0;
+#pragma unroll
+                    for (int t2 = 0; t2 < KW; t2++) {
+                        T rr = 0, ri = 0;
+#pragma unroll
+                        for (int t1 = 0; t1 < KW; t1++) {
+                            const cx_t<T> f = pl[t2 * hx + t1];
+                            rr += f.x * kx[t1];
+                            ri += f.y * kx[t1];
+                        }
+                        sr += rr * ky[t2];
+                        si += ri * ky[t2];
+                    }
+                    ar[k] += sr * kzv;
+                    ai[k] += si * kzv;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const int i = base + (int)threadIdx.x + k * (int)blockDim.x;
+            if (i >= end) continue;
+            const cx_t<T> f = fac[i];
+            const int j = perm[i];
+            if (j >= 0 && j < g.m)  // a valid permutation always passes
+                out[j] = mkc(ar[k] * f.x - ai[k] * f.y, ar[k] * f.y + ai[k] * f.x);
+        }
+    }
+}
+
+template <typename T, int KW>
+static int degrid_kw(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
+                     const int32_t *ts, const cx_t<T> *fw, cx_t<T> *out, const GridArgs &g,
+                     hipStream_t s)
+{
+    int is2d = g.nf3 == 1 ? 1 : 0;
+    size_t lds = (size_t)degrid_lds_bytes(g, KW, (int)sizeof(cx_t<T>));
+    unsigned nblk = (unsigned)((int64_t)g.ntx * g.nty);
+    if (lds > 65536)
+        CG_HIP(hipFuncSetAttribute((const void *)k_degrid<T, KW, 4>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(256), lds, s, geo, fac, perm, ts, fw,
+                       out, g, is2d);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+template <typename T, typename TV>
+int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
+                  const int32_t *ts, const cx_t<T> *fw, TV *out, const GridArgs &g, int kw,
+                  hipStream_t s)
+{
+    switch (kw) {
+#define CG_CASE(K) \
+    case K: return degrid_kw<T, K>(geo, fac, perm, ts, fw, out, g, s);
+        CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8) CG_CASE(9)
+        CG_CASE(10) CG_CASE(11) CG_CASE(12) CG_CASE(13) CG_CASE(14) CG_CASE(15) CG_CASE(16)
+#undef CG_CASE
+    default: return CG_ERR_ARG;
+    }
+}
+
+template int launch_grid<float>(const float4 *, const float2 *, const int32_t *, float2 *, const GridArgs &, int, hipStream_t);
+template int launch_grid<double>(const double4 *, const double2 *, const int32_t *, double2 *, const GridArgs &, int, hipStream_t);
+template int launch_degrid<float, float2>(const float4 *, const float2 *, const int32_t *, const int32_t *, const float2 *, float2 *, const GridArgs &, int, hipStream_t);
+template int launch_degrid<double, double2>(const double4 *, const double2 *, const int32_t *, const int32_t *, const double2 *, double2 *, const GridArgs &, int, hipStream_t);
+
+}  // namespace cg

@@ -1,0 +1,130 @@
+// Launchers shared between the HIP translation units of libcurafft.so.
+#pragma once
+#include "cg_common.hpp"
+#include "cg_plan.hpp"
+
+namespace cg {
+
+// Geometry needed by the coordinate kernels (all FP64: north-star step (i)).
+struct CoordArgs {
+    double scale_uv;   // 2 pi f/c
+    double scale_w;    // 2 pi f/c * sign
+    double pix_x, pix_y;
+    double i_center, gamma, o_center;
+    int nf1, nf2, nf3, kw;
+    int tx, ty, ntx, nty;
+    int pirange;       // 1: coordinates already in [-pi,pi) radians (2-D NUFFT)
+    int has_w;         // 0: 2-D (no w axis)
+};
+
+CoordArgs make_coord_args(const WPlan &p, const Tiling &t);
+
+// ---- prep (cg_prep.hip)
+// min/max of the scaled w (FP64) over nrow rows of uvw [nrow][3]; partial
+// buffer needs 2*1024 doubles.  Result copied to host (synchronises).
+template <typename TU>
+int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
+                   double &wmin, double &wmax, hipStream_t s);
+// keys[j] = tile of the footprint start of row j; vals[j] = j
+template <typename TU>
+int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
+                int32_t *vals, hipStream_t s);
+// 2-D NUFFT variant: separate coordinate arrays in [-pi,pi)
+template <typename TU>
+int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a,
+                   uint32_t *keys, int32_t *vals, hipStream_t s);
+// tile_start[t] = first sorted position with key >= t, t in [0, ntiles]
+int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles,
+                       int32_t *tile_start, hipStream_t s);
+
+// Sorted point records: geo = {d1, d2, d3, packed(lx | ly<<8 | z0<<16)} and a
+// complex value (ms2dirty: vis*wgt*e^{i o_c w}; dirty2ms: wgt*e^{-i o_c w}).
+template <typename T> struct Geo;
+template <> struct Geo<float> { using type = float4; };
+template <> struct Geo<double> { using type = double4; };
+template <typename T> using geo_t = typename Geo<T>::type;
+
+template <typename T, typename TU, typename TV, typename TW>
+int launch_records(const TU *uvw, const TV *vis, const TW *wgt, const int32_t *perm,
+                   int64_t nrow, const CoordArgs &a, int adjoint, geo_t<T> *geo,
+                   cx_t<T> *val, hipStream_t s);
+template <typename T, typename TU, typename TV>
+int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm,
+                      int64_t m, const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val,
+                      hipStream_t s);
+
+// ---- sort (cg_sort.hip): stable LSD radix sort of (key, val) pairs by the
+// low `key_bits` bits.  tmp_keys/tmp_vals have m entries; hist needs
+// radix_hist_size(m) ints.  Result ends in keys/vals.
+int64_t radix_hist_size(int64_t m);
+int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys,
+                      int32_t *tmp_vals, int64_t m, int key_bits, int32_t *hist,
+                      int32_t *scan_tmp, hipStream_t s);
+int64_t scan_tmp_size(int64_t n);
+// exclusive scan of n ints in place
+int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *scan_tmp, hipStream_t s);
+
+// ---- gridding / degridding (cg_grid.hip)
+struct GridArgs {
+    int m;  // number of points (rows)
+    int nf1, nf2, nf3;
+    int tx, ty, ntx, nty, sbx, sby, pc, nchunk;
+    double beta, es_c;
+};
+template <typename T>
+int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
+                cx_t<T> *fw, const GridArgs &g, int kw, hipStream_t s);
+template <typename T, typename TV>
+int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
+                  const int32_t *tile_start, const cx_t<T> *fw, TV *out,
+                  const GridArgs &g, int kw, hipStream_t s);
+int grid_lds_bytes(const GridArgs &g, int elem);
+int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
+
+// ---- FFT (cg_fft.hip): in-place unnormalised 2-D FFT of `batch` planes
+// [batch][n2][n1], exponent sign dir (+1 / -1).  tw1/tw2 are device tables
+// of e^{+2 pi i t/n} (length n1 / n2) in precision T.
+template <typename T>
+int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T> *tw1,
+                 const cx_t<T> *tw2, hipStream_t s);
+template <typename T>
+bool fft_supported(int n);
+template <typename T>
+int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s);
+
+// ---- W-stacking per-pixel stages (cg_wstack.hip), FP64 phases & factors
+// Gauss-Legendre quadrature of the ES kernel's Fourier transform
+struct Quad {
+    int nq;
+    double xq[32], gq[32];
+};
+Quad make_quad(const ConvOpts &o);
+// phi[k] = 2 sum_i g_i cos(pi k (x_i - N + 1)/(N - 1)), N = nf/2 + 1
+// (src/fourier/deconv.cu:12-26), k in [0, nf/2]
+int launch_fseries_grid(const Quad &q, int nf, double *phi, hipStream_t s);
+
+struct WArgs {
+    int N1, N2, nf1, nf2, nf3;
+    double pix_x, pix_y, o_center, i_center, gamma, h;
+    Quad qd;
+};
+WArgs make_wargs(const WPlan &p);
+// z' and psihat(z') on the (N2/2+1) x (N1/2+1) quarter image
+int launch_quarter_table(const WArgs &w, double *zq, double *phi3, hipStream_t s);
+// ms2dirty epilogue: w-DFT over planes + deconvolution + w-correction + 1/n
+template <typename T, typename TV>
+int launch_ms2dirty_epilogue(const cx_t<T> *fw, const double *zq, const double *phi3,
+                             const double *phi1, const double *phi2, TV *dirty,
+                             const WArgs &w, hipStream_t s);
+// dirty2ms prologue: 1/n, w-correction, deconvolution and w-IDFT onto planes
+// (writes every grid cell, zero outside the image footprint)
+template <typename T, typename TV>
+int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *phi3,
+                             const double *phi1, const double *phi2, cx_t<T> *fw,
+                             const WArgs &w, hipStream_t s);
+// 2-D NUFFT deconvolution (type 1: grid -> modes; type 2: modes -> grid)
+template <typename T, typename TV>
+int launch_deconv2d(cx_t<T> *fw, TV *fk, const double *phi1, const double *phi2,
+                    int N1, int N2, int nf1, int nf2, int type, hipStream_t s);
+
+}  // namespace cg

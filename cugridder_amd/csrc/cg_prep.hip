@@ -1,0 +1,350 @@
+// Coordinate preparation: w range, tile keys and sorted point records.
+//
+// All coordinate arithmetic is FP64 (north-star step (i)): the reference's
+// get_effective_coordinate (src/astro/precomp.cu:48-80), rescaling_real
+// (src/utils.cu:532-556), pre_stage_2 (src/fourier/cuft.cu:82-103) and
+// SHIFT_RESCALE (include/conv.h:24) are folded into one pass that emits, per
+// visibility, the integer footprint start on the grid plus the FP32/FP64
+// offset of the first tap.  The reference's per-cell histogram/scan/gather
+// sort (src/utils.cu:105-132,457-486) is replaced by tile keys + a stable
+// radix sort (cg_sort.hip).
+#include "cg_internal.hpp"
+
+namespace cg {
+
+CoordArgs make_coord_args(const WPlan &p, const Tiling &t)
+{
+    CoordArgs a{};
+    a.scale_uv = 2.0 * kPi * p.f_over_c;
+    a.scale_w = 2.0 * kPi * p.f_over_c * p.sign;
+    a.pix_x = p.pix_x;
+    a.pix_y = p.pix_y;
+    a.i_center = p.i_center;
+    a.gamma = p.gamma;
+    a.o_center = p.o_center;
+    a.nf1 = p.nf1;
+    a.nf2 = p.nf2;
+    a.nf3 = p.nf3;
+    a.kw = p.conv.kw;
+    a.tx = t.tx;
+    a.ty = t.ty;
+    a.ntx = t.ntx;
+    a.nty = t.nty;
+    a.pirange = 0;
+    a.has_w = 1;
+    return a;
+}
+
+// include/conv.h:24 SHIFT_RESCALE (p = pirange)
+__device__ __forceinline__ double shift_rescale(double x, int N, int pirange)
+{
+    const double twopi = 6.28318530717958648;
+    double y = x;
+    if (!pirange) {
+        double t = x - floor(x / twopi) * twopi;
+        y = t - (t >= kPi ? twopi : 0.0);
+    }
+    return (y * 0.159154943091895336 + 0.5) * N;
+}
+
+struct Foot {
+    int x0, y0, z0;      // wrapped footprint starts
+    double d1, d2, d3;   // offset of the first tap: start - g
+};
+
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ __forceinline__ void footprint_1d(double g, int n, int kw, int &i0, double &d)
+{
+    double s = ceil(g - 0.5 * kw);
+    d = s - g;
+    i0 = wrapi((int)s, n);
+}
+
+// grid axis 1 (fast) <- v * pix_y, axis 2 <- u * pix_x (cugridder.cu:191)
+__device__ __forceinline__ Foot make_foot(double u, double v, double w, const CoordArgs &a)
+{
+    Foot f;
+    double xs = v * a.scale_uv * a.pix_y;
+    double ys = u * a.scale_uv * a.pix_x;
+    footprint_1d(shift_rescale(xs, a.nf1, 0), a.nf1, a.kw, f.x0, f.d1);
+    footprint_1d(shift_rescale(ys, a.nf2, 0), a.nf2, a.kw, f.y0, f.d2);
+    if (a.has_w) {
+        double wp = (w * a.scale_w - a.i_center) / a.gamma;
+        footprint_1d(shift_rescale(wp, a.nf3, 0), a.nf3, a.kw, f.z0, f.d3);
+    } else {
+        f.z0 = 0;
+        f.d3 = 0.0;
+    }
+    return f;
+}
+
+__device__ __forceinline__ uint32_t tile_key(const Foot &f, const CoordArgs &a)
+{
+    return (uint32_t)((f.y0 / a.ty) * a.ntx + (f.x0 / a.tx));
+}
+
+// ------------------------------------------------------------ w min/max
+
+template <typename TU>
+__global__ __launch_bounds__(256) void k_wminmax(const TU *uvw, int64_t nrow, double scale_w,
+                                                 double *part)
+{
+    __shared__ double smin[256], smax[256];
+    double mn = INFINITY, mx = -INFINITY;
+    for (int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x; j < nrow; j += (int64_t)gridDim.x * 256) {
+        double w = (double)uvw[3 * j + 2] * scale_w;
+        mn = fmin(mn, w);
+        mx = fmax(mx, w);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = smin[0];
+        part[1024 + blockIdx.x] = smax[0];
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_minmax_final(double *part, int nb)
+{
+    __shared__ double smin[1024], smax[1024];
+    int t = threadIdx.x;
+    smin[t] = t < nb ? part[t] : INFINITY;
+    smax[t] = t < nb ? part[1024 + t] : -INFINITY;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            smin[t] = fmin(smin[t], smin[t + s]);
+            smax[t] = fmax(smax[t], smax[t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[2048] = smin[0];
+        part[2049] = smax[0];
+    }
+}
+
+template <typename TU>
+int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part, double &wmin,
+                   double &wmax, hipStream_t s)
+{
+    if (nrow == 0) {
+        wmin = wmax = 0.0;
+        return CG_OK;
+    }
+    int nb = (int)std::min<int64_t>(1024, ceil_div(nrow, 256));
+    hipLaunchKernelGGL(k_wminmax<TU>, dim3(nb), dim3(256), 0, s, uvw, nrow, scale_w, d_part);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1024), 0, s, d_part, nb);
+    CG_HIP(hipGetLastError());
+    double h[2];
+    CG_HIP(hipMemcpyAsync(h, d_part + 2048, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    CG_HIP(hipStreamSynchronize(s));
+    wmin = h[0];
+    wmax = h[1];
+    return CG_OK;
+}
+
+// ------------------------------------------------------------ keys
+
+template <typename TU>
+__global__ __launch_bounds__(256) void k_keys(const TU *uvw, int64_t nrow, CoordArgs a,
+                                              uint32_t *keys, int32_t *vals)
+{
+    int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= nrow) return;
+    CoordArgs b = a;
+    b.has_w = 0;  // the key only depends on the (u, v) footprint
+    Foot f = make_foot((double)uvw[3 * j], (double)uvw[3 * j + 1], 0.0, b);
+    keys[j] = tile_key(f, a);
+    vals[j] = (int32_t)j;
+}
+
+template <typename TU>
+int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys, int32_t *vals,
+                hipStream_t s)
+{
+    if (nrow == 0) return CG_OK;
+    hipLaunchKernelGGL(k_keys<TU>, dim3((unsigned)ceil_div(nrow, 256)), dim3(256), 0, s, uvw, nrow,
+                       a, keys, vals);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// 2-D NUFFT: x -> axis 1 (fast), y -> axis 2, coordinates in [-pi,pi)
+__device__ __forceinline__ Foot make_foot_xy(double x, double y, const CoordArgs &a)
+{
+    Foot f;
+    footprint_1d(shift_rescale(x, a.nf1, a.pirange), a.nf1, a.kw, f.x0, f.d1);
+    footprint_1d(shift_rescale(y, a.nf2, a.pirange), a.nf2, a.kw, f.y0, f.d2);
+    f.z0 = 0;
+    f.d3 = 0.0;
+    return f;
+}
+
+template <typename TU>
+__global__ __launch_bounds__(256) void k_keys_xy(const TU *x, const TU *y, int64_t m, CoordArgs a,
+                                                 uint32_t *keys, int32_t *vals)
+{
+    int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= m) return;
+    Foot f = make_foot_xy((double)x[j], (double)y[j], a);
+    keys[j] = tile_key(f, a);
+    vals[j] = (int32_t)j;
+}
+
+template <typename TU>
+int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a, uint32_t *keys,
+                   int32_t *vals, hipStream_t s)
+{
+    if (m == 0) return CG_OK;
+    hipLaunchKernelGGL(k_keys_xy<TU>, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, x, y, m, a,
+                       keys, vals);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// ------------------------------------------------------------ tile bounds
+
+__global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t *keys, int64_t m,
+                                                     int64_t ntiles, int32_t *start)
+{
+    int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (t > ntiles) return;
+    int64_t lo = 0, hi = m;  // first index with keys[i] >= t
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)keys[mid] < t) lo = mid + 1;
+        else hi = mid;
+    }
+    start[t] = (int32_t)lo;
+}
+
+int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int32_t *tile_start,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div(ntiles + 1, 256)), dim3(256), 0, s,
+                       sorted_keys, m, ntiles, tile_start);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// ------------------------------------------------------------ records
+
+template <typename T> __device__ __forceinline__ geo_t<T> make_geo(double d1, double d2, double d3, uint32_t pk);
+template <> __device__ __forceinline__ float4 make_geo<float>(double d1, double d2, double d3, uint32_t pk)
+{
+    return make_float4((float)d1, (float)d2, (float)d3, __uint_as_float(pk));
+}
+template <> __device__ __forceinline__ double4 make_geo<double>(double d1, double d2, double d3, uint32_t pk)
+{
+    return make_double4(d1, d2, d3, __longlong_as_double((long long)pk));
+}
+
+__device__ __forceinline__ double2 ld_c(const float2 *p, int64_t j)
+{
+    float2 v = p[j];
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_c(const double2 *p, int64_t j) { return p[j]; }
+
+template <typename T, typename TU, typename TV, typename TW>
+__global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, const TW *wgt,
+                                                 const int32_t *perm, int64_t nrow, CoordArgs a,
+                                                 int adjoint, geo_t<T> *geo, cx_t<T> *val)
+{
+    int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= nrow) return;
+    int64_t j = perm[i];
+    if (j < 0 || j >= nrow) j = 0;  // a valid permutation never takes this
+    double u = (double)uvw[3 * j], v = (double)uvw[3 * j + 1], w = (double)uvw[3 * j + 2];
+    Foot f = make_foot(u, v, w, a);
+    uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8) | ((uint32_t)f.z0 << 16);
+    geo[i] = make_geo<T>(f.d1, f.d2, f.d3, pk);
+    double wt = wgt ? (double)wgt[j] : 1.0;
+    double ws = w * a.scale_w;
+    double2 c;
+    if (!adjoint) {
+        // ms2dirty: vis * wgt (ra_exec.cu:103-111) * e^{i o_c w} (cuft.cu:46-80)
+        double2 x = ld_c(vis, j);
+        double s, co;
+        sincos(a.o_center * ws, &s, &co);
+        c = make_double2(wt * (x.x * co - x.y * s), wt * (x.x * s + x.y * co));
+    } else {
+        // dirty2ms: factor wgt * e^{-i (w' gamma + i_c) o_c}  (ra_exec.cu:55-93)
+        double wp = (ws - a.i_center) / a.gamma;
+        double s, co;
+        sincos(-(wp * a.gamma + a.i_center) * a.o_center, &s, &co);
+        c = make_double2(wt * co, wt * s);
+    }
+    val[i] = mkc((T)c.x, (T)c.y);
+}
+
+template <typename T, typename TU, typename TV, typename TW>
+int launch_records(const TU *uvw, const TV *vis, const TW *wgt, const int32_t *perm, int64_t nrow,
+                   const CoordArgs &a, int adjoint, geo_t<T> *geo, cx_t<T> *val, hipStream_t s)
+{
+    if (nrow == 0) return CG_OK;
+    hipLaunchKernelGGL((k_records<T, TU, TV, TW>), dim3((unsigned)ceil_div(nrow, 256)), dim3(256), 0,
+                       s, uvw, vis, wgt, perm, nrow, a, adjoint, geo, val);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+template <typename T, typename TU, typename TV>
+__global__ __launch_bounds__(256) void k_records_xy(const TU *x, const TU *y, const TV *c,
+                                                    const int32_t *perm, int64_t m, CoordArgs a,
+                                                    geo_t<T> *geo, cx_t<T> *val)
+{
+    int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= m) return;
+    int64_t j = perm[i];
+    if (j < 0 || j >= m) j = 0;  // a valid permutation never takes this
+    Foot f = make_foot_xy((double)x[j], (double)y[j], a);
+    uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8);
+    geo[i] = make_geo<T>(f.d1, f.d2, 0.0, pk);
+    if (c) {
+        double2 v = ld_c(c, j);
+        val[i] = mkc((T)v.x, (T)v.y);
+    } else {
+        val[i] = mkc((T)1, (T)0);
+    }
+}
+
+template <typename T, typename TU, typename TV>
+int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm, int64_t m,
+                      const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val, hipStream_t s)
+{
+    if (m == 0) return CG_OK;
+    hipLaunchKernelGGL((k_records_xy<T, TU, TV>), dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s,
+                       x, y, c, perm, m, a, geo, val);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// explicit instantiations
+template int launch_wminmax<float>(const float *, int64_t, double, double *, double &, double &, hipStream_t);
+template int launch_wminmax<double>(const double *, int64_t, double, double *, double &, double &, hipStream_t);
+template int launch_keys<float>(const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
+template int launch_keys<double>(const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
+template int launch_keys_xy<float>(const float *, const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
+template int launch_keys_xy<double>(const double *, const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
+template int launch_records<float, float, float2, float>(const float *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, hipStream_t);
+template int launch_records<float, double, float2, float>(const double *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, hipStream_t);
+template int launch_records<double, double, double2, double>(const double *, const double2 *, const double *, const int32_t *, int64_t, const CoordArgs &, int, double4 *, double2 *, hipStream_t);
+template int launch_records_xy<float, float, float2>(const float *, const float *, const float2 *, const int32_t *, int64_t, const CoordArgs &, float4 *, float2 *, hipStream_t);
+template int launch_records_xy<double, double, double2>(const double *, const double *, const double2 *, const int32_t *, int64_t, const CoordArgs &, double4 *, double2 *, hipStream_t);
+
+}  // namespace cg

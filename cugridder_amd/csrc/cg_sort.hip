@@ -1,0 +1,230 @@
+// Deterministic stable LSD radix sort of (tile key, row index) pairs and an
+// exclusive int scan.
+//
+// Replaces the reference's per-cell counting sort (histogram_3d_sparse with
+// an atomicAdd in-bin rank, src/utils.cu:105-132; thrust exclusive_scan,
+// src/common_utils.cu:156-170; mapping_based_gather_3d, src/utils.cu:457-486).
+// That sort's in-bin order depends on atomic timing; here every pass ranks
+// items inside a 64-wide wavefront with ballots (a match on each digit bit)
+// and across wavefronts with an LDS prefix, so the permutation equals a host
+// std::stable_sort by key, bit for bit.
+#include "cg_internal.hpp"
+
+namespace cg {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortThreads * kSortRounds;  // 4096 items / block
+
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// ------------------------------------------------------------ scan
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// exclusive scan of per-thread values across the block; returns this
+// thread's exclusive prefix and writes the block total to *total
+__device__ __forceinline__ int block_excl_scan(int v, int *s_wave, int *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    int incl = wave_incl_scan(v);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        int w = lane < nw ? s_wave[lane] : 0;
+        int wi = wave_incl_scan(w);
+        if (lane < nw) s_wave[lane] = wi - w;
+        if (lane == nw - 1) s_wave[64] = wi;
+    }
+    __syncthreads();
+    int r = s_wave[wave] + incl - v;
+    *total = s_wave[64];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int32_t *d, int64_t n,
+                                                              int32_t *bsum)
+{
+    __shared__ int s_wave[65];
+    int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) v += d[base + k];
+    int tot;
+    block_excl_scan(v, s_wave, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// exclusive scan of one tile per block, adding offs[block] (offs may be null)
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(int32_t *d, int64_t n,
+                                                            const int32_t *offs)
+{
+    __shared__ int s_wave[65];
+    int64_t base = blockIdx.x * (int64_t)kScanTile + threadIdx.x * kScanItems;
+    int x[kScanItems];
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        x[k] = base + k < n ? d[base + k] : 0;
+        v += x[k];
+    }
+    int tot;
+    int pre = block_excl_scan(v, s_wave, &tot);
+    if (offs) pre += offs[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) d[base + k] = pre;
+        pre += x[k];
+    }
+}
+
+int64_t scan_tmp_size(int64_t n)
+{
+    int64_t s = 0;
+    while (n > kScanTile) {
+        n = ceil_div(n, kScanTile);
+        s += n;
+    }
+    return s + 1;
+}
+
+int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *tmp, hipStream_t s)
+{
+    if (n <= 0) return CG_OK;
+    if (n <= kScanTile) {
+        hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(kScanThreads), 0, s, data, n,
+                           (const int32_t *)nullptr);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    }
+    int64_t nb = ceil_div(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n, tmp);
+    CG_HIP(hipGetLastError());
+    int r = launch_exclusive_scan(tmp, nb, tmp + nb, s);
+    if (r) return r;
+    hipLaunchKernelGGL(k_scan_tile, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
+                       (const int32_t *)tmp);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// ------------------------------------------------------------ radix sort
+
+int64_t radix_hist_size(int64_t m) { return (int64_t)kRadix * std::max<int64_t>(1, ceil_div(m, kSortTile)); }
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *keys, int64_t m,
+                                                             int shift, int32_t *hist, int nblocks)
+{
+    __shared__ int cnt[kRadix];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = blockIdx.x * (int64_t)kSortTile;
+#pragma unroll 4
+    for (int k = 0; k < kSortRounds; k++) {
+        int64_t i = base + k * kSortThreads + threadIdx.x;
+        if (i < m) atomicAdd(&cnt[(keys[i] >> shift) & (kRadix - 1)], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, int64_t m, int shift,
+    const int32_t *offs, int nblocks)
+{
+    constexpr int NW = kSortThreads / 64;
+    __shared__ int s_base[kRadix];
+    __shared__ int s_wcnt[NW][kRadix];
+    __shared__ int s_rtot[kRadix];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    s_base[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+    const uint64_t lt = __lanemask_lt();
+    int64_t base = blockIdx.x * (int64_t)kSortTile;
+    for (int r = 0; r < kSortRounds; r++) {
+        int64_t i = base + r * kSortThreads + tid;
+        bool valid = i < m;
+        uint32_t key = valid ? kin[i] : 0u;
+        int32_t val = valid ? vin[i] : 0;
+        int d = (int)((key >> shift) & (kRadix - 1));
+#pragma unroll
+        for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t mask = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; b++) {
+            int bit = (d >> b) & 1;
+            uint64_t bb = __ballot(bit);
+            mask &= bit ? bb : ~bb;
+        }
+        int rank = __popcll(mask & lt);
+        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(mask);
+        __syncthreads();
+        {
+            int acc = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                int t = s_wcnt[w][tid];
+                s_wcnt[w][tid] = acc;
+                acc += t;
+            }
+            s_rtot[tid] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            int pos = s_base[d] + s_wcnt[wave][d] + rank;
+            if (pos >= 0 && pos < m) {  // always true for a consistent histogram
+                kout[pos] = key;
+                vout[pos] = val;
+            }
+        }
+        __syncthreads();
+        s_base[tid] += s_rtot[tid];
+        __syncthreads();
+    }
+}
+
+int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys, int32_t *tmp_vals,
+                      int64_t m, int key_bits, int32_t *hist, int32_t *scan_tmp, hipStream_t s)
+{
+    if (m <= 1) return CG_OK;
+    int nblocks = (int)ceil_div(m, kSortTile);
+    int passes = (key_bits + kRadixBits - 1) / kRadixBits;
+    uint32_t *ka = keys, *kb = tmp_keys;
+    int32_t *va = vals, *vb = tmp_vals;
+    for (int p = 0; p < passes; p++) {
+        int shift = p * kRadixBits;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(kSortThreads), 0, s, ka, m, shift,
+                           hist, nblocks);
+        CG_HIP(hipGetLastError());
+        int r = launch_exclusive_scan(hist, (int64_t)kRadix * nblocks, scan_tmp, s);
+        if (r) return r;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, ka, va, kb, vb,
+                           m, shift, hist, nblocks);
+        CG_HIP(hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    if (ka != keys) {
+        CG_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, s));
+        CG_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * m, hipMemcpyDeviceToDevice, s));
+    }
+    return CG_OK;
+}
+
+}  // namespace cg

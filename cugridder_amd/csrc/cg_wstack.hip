@@ -1,0 +1,282 @@
+// Per-pixel stages of improved W-stacking, with FP64 kept for north-star
+// steps (ii) n-1, (iii) the w-phase and (iv) the correction factors.
+//
+// ms2dirty epilogue fuses, per image pixel, the reference's
+//   w_term_dft      (src/fourier/cuft.cu:597-645): sum_p F_p e^{i z'(p-P/2)}
+//   deconv_2d type1 (src/fourier/deconv.cu:250-294): / (phi1 phi2)
+//   w_term_deconv   (src/fourier/deconv.cu:401-432): * e^{i(n-1-o_c) i_c} / phi3
+//   div_n_lm        (src/astro/ra_exec.cu:23-37):   / n
+// and the dirty2ms prologue fuses their adjoints (div_n_lm, w_term_deconv
+// with flag -1, deconv_2d type 2, w_term_idft cuft.cu:647-689) and writes
+// every cell of every plane (zero outside the image footprint), so the grid
+// needs no separate memset.
+#include "cg_internal.hpp"
+
+namespace cg {
+
+WArgs make_wargs(const WPlan &p)
+{
+    WArgs w{};
+    w.N1 = p.N1;
+    w.N2 = p.N2;
+    w.nf1 = p.nf1;
+    w.nf2 = p.nf2;
+    w.nf3 = p.nf3;
+    w.pix_x = p.pix_x;
+    w.pix_y = p.pix_y;
+    w.o_center = p.o_center;
+    w.i_center = p.i_center;
+    w.gamma = p.gamma;
+    w.h = p.h;
+    w.qd = make_quad(p.conv);
+    return w;
+}
+
+Quad make_quad(const ConvOpts &o)
+{
+    Quad q{};
+    q.nq = kernel_quadrature(o, q.xq, q.gq);
+    return q;
+}
+
+__global__ __launch_bounds__(256) void k_fseries_grid(Quad qd, int nf, double *phi)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int N = nf / 2 + 1;
+    if (k >= N) return;
+    double s = 0.0;
+    for (int i = 0; i < qd.nq; i++) s += qd.gq[i] * cos((qd.xq[i] - N + 1) / (double)(N - 1) * kPi * k);
+    phi[k] = 2.0 * s;
+}
+
+int launch_fseries_grid(const Quad &q, int nf, double *phi, hipStream_t s)
+{
+    const int N = nf / 2 + 1;
+    hipLaunchKernelGGL(k_fseries_grid, dim3((N + 255) / 256), dim3(256), 0, s, q, nf, phi);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// k_generation (src/astro/precomp.cu:215-225) + pre_stage_2 on z
+// (cuft.cu:102-105) + fourier_series_appro at z' (deconv.cu:49-65)
+__global__ __launch_bounds__(256) void k_quarter(WArgs w, double *zq, double *phi3)
+{
+    const int W = w.N1 / 2 + 1, H = w.N2 / 2 + 1;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= W * H) return;
+    const int r = q / W, c = q - r * W;
+    const double l = r * w.pix_x, m = c * w.pix_y;
+    const double t = 1.0 - l * l - m * m;
+    // n-1, written without cancellation for t >= 0
+    const double z = t < 0 ? sqrt(-t) - 1.0 : -(l * l + m * m) / (sqrt(t) + 1.0);
+    const double zp = (z - w.o_center) * w.gamma * w.h;
+    double s = 0.0;
+    for (int i = 0; i < w.qd.nq; i++) s += w.qd.gq[i] * 2.0 * cos(w.qd.xq[i] * zp);
+    zq[q] = zp;
+    phi3[q] = s;
+}
+
+int launch_quarter_table(const WArgs &w, double *zq, double *phi3, hipStream_t s)
+{
+    int nq = (w.N1 / 2 + 1) * (w.N2 / 2 + 1);
+    hipLaunchKernelGGL(k_quarter, dim3((nq + 255) / 256), dim3(256), 0, s, w, zq, phi3);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+template <typename TV> __device__ __forceinline__ TV from_d2(double2 v);
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 v) { return v; }
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// n-1 and n at image offsets (l, m); n is sqrt(t) (NaN beyond the horizon,
+// as div_n_lm); the w-correction phase uses sqrt(|t|)-1 as w_term_deconv.
+__device__ __forceinline__ void pixel_n(double l, double m, double &nm1_phase, double &n)
+{
+    const double r2 = l * l + m * m;
+    const double t = 1.0 - r2;
+    n = sqrt(t);
+    nm1_phase = t < 0 ? sqrt(-t) - 1.0 : -r2 / (n + 1.0);
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void k_ms2dirty_epi(const cx_t<T> *__restrict__ fw,
+                                                      const double *__restrict__ zq,
+                                                      const double *__restrict__ phi3,
+                                                      const double *__restrict__ phi1,
+                                                      const double *__restrict__ phi2,
+                                                      TV *__restrict__ dirty, WArgs w)
+{
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (idx >= (int64_t)w.N1 * w.N2) return;
+    const int row = (int)(idx / w.N1), col = (int)(idx - (int64_t)row * w.N1);
+    const int dc = col - w.N1 / 2, dr = row - w.N2 / 2;
+    const int w1 = dc >= 0 ? dc : w.nf1 + dc;
+    const int w2 = dr >= 0 ? dr : w.nf2 + dr;
+    const int q = abs(dc) + abs(dr) * (w.N1 / 2 + 1);
+    const size_t plane = (size_t)w.nf1 * w.nf2;
+    const cx_t<T> *src = fw + (size_t)w2 * w.nf1 + w1;
+    const double zp = zq[q];
+    // e^{i z' (p - P/2)}, FP64 recurrence
+    double sv, cv;
+    sincos(zp, &sv, &cv);
+    const double2 step = make_double2(cv, sv);
+    sincos(-zp * (w.nf3 / 2), &sv, &cv);
+    double2 ph = make_double2(cv, sv);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = 0; p < w.nf3; p++) {
+        const double2 f = to_d2(src[plane * p]);
+        acc.x += f.x * ph.x - f.y * ph.y;
+        acc.y += f.x * ph.y + f.y * ph.x;
+        ph = zmul(ph, step);
+    }
+    const double l = dr * w.pix_x, m = dc * w.pix_y;
+    double nm1, n;
+    pixel_n(l, m, nm1, n);
+    sincos((nm1 - w.o_center) * w.i_center, &sv, &cv);
+    const double scale = 1.0 / (phi1[abs(dc)] * phi2[abs(dr)] * phi3[q] * n);
+    acc = zmul(acc, make_double2(cv, sv));
+    dirty[idx] = from_d2<TV>(make_double2(acc.x * scale, acc.y * scale));
+}
+
+template <typename T, typename TV>
+int launch_ms2dirty_epilogue(const cx_t<T> *fw, const double *zq, const double *phi3,
+                             const double *phi1, const double *phi2, TV *dirty, const WArgs &w,
+                             hipStream_t s)
+{
+    int64_t n = (int64_t)w.N1 * w.N2;
+    hipLaunchKernelGGL((k_ms2dirty_epi<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                       fw, zq, phi3, phi1, phi2, dirty, w);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// grid cell -> image index along one axis (inverse of w = (k - N/2) mod nf),
+// -1 when the cell lies outside the image footprint
+__device__ __forceinline__ int cell_to_img(int wv, int N, int nf)
+{
+    if (wv < N - N / 2) return wv + N / 2;
+    if (wv >= nf - N / 2) return wv - nf + N / 2;
+    return -1;
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dirty,
+                                                      const double *__restrict__ zq,
+                                                      const double *__restrict__ phi3,
+                                                      const double *__restrict__ phi1,
+                                                      const double *__restrict__ phi2,
+                                                      cx_t<T> *__restrict__ fw, WArgs w)
+{
+    const int64_t cell = blockIdx.x * (int64_t)256 + threadIdx.x;
+    const size_t plane = (size_t)w.nf1 * w.nf2;
+    if (cell >= (int64_t)plane) return;
+    const int w2 = (int)(cell / w.nf1), w1 = (int)(cell - (int64_t)w2 * w.nf1);
+    const int col = cell_to_img(w1, w.N1, w.nf1);
+    const int row = cell_to_img(w2, w.N2, w.nf2);
+    cx_t<T> *dst = fw + cell;
+    if (col < 0 || row < 0) {
+        for (int p = 0; p < w.nf3; p++) dst[plane * p] = mkc((T)0, (T)0);
+        return;
+    }
+    const int dc = col - w.N1 / 2, dr = row - w.N2 / 2;
+    const int q = abs(dc) + abs(dr) * (w.N1 / 2 + 1);
+    const double l = dr * w.pix_x, m = dc * w.pix_y;
+    double nm1, n;
+    pixel_n(l, m, nm1, n);
+    double2 f = to_d2(dirty[(size_t)row * w.N1 + col]);
+    double sv, cv;
+    sincos(-(nm1 - w.o_center) * w.i_center, &sv, &cv);
+    const double scale = 1.0 / (n * phi3[q] * phi1[abs(dc)] * phi2[abs(dr)]);
+    f = zmul(f, make_double2(cv * scale, sv * scale));
+    const double zp = zq[q];
+    sincos(-zp, &sv, &cv);
+    const double2 step = make_double2(cv, sv);
+    sincos(zp * (w.nf3 / 2), &sv, &cv);
+    double2 ph = make_double2(cv, sv);
+    for (int p = 0; p < w.nf3; p++) {
+        const double2 v = zmul(f, ph);
+        dst[plane * p] = mkc((T)v.x, (T)v.y);
+        ph = zmul(ph, step);
+    }
+}
+
+template <typename T, typename TV>
+int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *phi3,
+                             const double *phi1, const double *phi2, cx_t<T> *fw, const WArgs &w,
+                             hipStream_t s)
+{
+    int64_t n = (int64_t)w.nf1 * w.nf2;
+    hipLaunchKernelGGL((k_dirty2ms_pro<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                       dirty, zq, phi3, phi1, phi2, fw, w);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+// 2-D NUFFT deconvolution, deconv_2d (src/fourier/deconv.cu:250-294), CMCL
+// mode order.  Type 2 writes every grid cell (zero outside the modes).
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void k_deconv2d_t1(const cx_t<T> *fw, TV *fk, const double *phi1,
+                                                     const double *phi2, int N1, int N2, int nf1,
+                                                     int nf2)
+{
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (idx >= (int64_t)N1 * N2) return;
+    const int k2 = (int)(idx / N1), k1 = (int)(idx - (int64_t)k2 * N1);
+    const int d1 = k1 - N1 / 2, d2 = k2 - N2 / 2;
+    const int w1 = d1 >= 0 ? d1 : nf1 + d1, w2 = d2 >= 0 ? d2 : nf2 + d2;
+    const double2 f = to_d2(fw[(size_t)w2 * nf1 + w1]);
+    const double s = 1.0 / (phi1[abs(d1)] * phi2[abs(d2)]);
+    fk[idx] = from_d2<TV>(make_double2(f.x * s, f.y * s));
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void k_deconv2d_t2(cx_t<T> *fw, const TV *fk, const double *phi1,
+                                                     const double *phi2, int N1, int N2, int nf1,
+                                                     int nf2)
+{
+    const int64_t cell = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (cell >= (int64_t)nf1 * nf2) return;
+    const int w2 = (int)(cell / nf1), w1 = (int)(cell - (int64_t)w2 * nf1);
+    const int k1 = cell_to_img(w1, N1, nf1), k2 = cell_to_img(w2, N2, nf2);
+    if (k1 < 0 || k2 < 0) {
+        fw[cell] = mkc((T)0, (T)0);
+        return;
+    }
+    const int d1 = k1 - N1 / 2, d2 = k2 - N2 / 2;
+    const double2 f = to_d2(fk[(size_t)k2 * N1 + k1]);
+    const double s = 1.0 / (phi1[abs(d1)] * phi2[abs(d2)]);
+    fw[cell] = mkc((T)(f.x * s), (T)(f.y * s));
+}
+
+template <typename T, typename TV>
+int launch_deconv2d(cx_t<T> *fw, TV *fk, const double *phi1, const double *phi2, int N1, int N2,
+                    int nf1, int nf2, int type, hipStream_t s)
+{
+    if (type == 1) {
+        int64_t n = (int64_t)N1 * N2;
+        hipLaunchKernelGGL((k_deconv2d_t1<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                           fw, fk, phi1, phi2, N1, N2, nf1, nf2);
+    } else {
+        int64_t n = (int64_t)nf1 * nf2;
+        hipLaunchKernelGGL((k_deconv2d_t2<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
+                           fw, fk, phi1, phi2, N1, N2, nf1, nf2);
+    }
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+template int launch_ms2dirty_epilogue<float, float2>(const float2 *, const double *, const double *, const double *, const double *, float2 *, const WArgs &, hipStream_t);
+template int launch_ms2dirty_epilogue<double, double2>(const double2 *, const double *, const double *, const double *, const double *, double2 *, const WArgs &, hipStream_t);
+template int launch_dirty2ms_prologue<float, float2>(const float2 *, const double *, const double *, const double *, const double *, float2 *, const WArgs &, hipStream_t);
+template int launch_dirty2ms_prologue<double, double2>(const double2 *, const double *, const double *, const double *, const double *, double2 *, const WArgs &, hipStream_t);
+template int launch_deconv2d<float, float2>(float2 *, float2 *, const double *, const double *, int, int, int, int, int, hipStream_t);
+template int launch_deconv2d<double, double2>(double2 *, double2 *, const double *, const double *, int, int, int, int, int, hipStream_t);
+
+}  // namespace cg

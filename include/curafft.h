@@ -1,0 +1,164 @@
+/*
+ * curafft.h -- C-ABI of libcurafft.so, the MI355X (gfx950) W-stacking imager.
+ *
+ * The first eight functions are drop-in replacements for the reference's
+ * exported symbols (RapidsAtHKUST/cuGridder include/cugridder.h:20-46,
+ * defined at src/astro/cugridder.cu:540-571) with the same names, argument
+ * order and meaning.  Complex arrays are passed as interleaved (re, im)
+ * scalars, which is the memory layout of std::complex<PCS> in the reference
+ * and of numpy complex128/complex64 used by its ctypes binding
+ * (python/curagridder/cursl.py:29-72).
+ *
+ * Conventions (reference): uvw is host [nrow][3] (u, v, w in metres); vis is
+ * [nrow]; wgt is [nrow] (NULL in the _1 variants); dirty is [nxdirty][nydirty]
+ * row-major, dirty[i*ny + j] <-> l = (i - nx/2)*fov/nx, m = (j - ny/2)*fov/ny;
+ * fov in degrees; freq in Hz; sign multiplies w (the NIFTY-style binding
+ * passes -1).  ms2dirty computes
+ *   dirty(l,m) = 1/n sum_j wgt_j vis_j exp(2 pi i f/c (u l + v m + sign w (n-1)))
+ * and dirty2ms its adjoint, to relative error ~epsilon.
+ *
+ * Return value: 0 ok, 1 epsilon was clamped to the precision floor (as the
+ * reference's setup_conv_opts), 2 invalid argument, 3 unsupported size,
+ * 4 HIP runtime error, 5 device allocation failed.  The library never exits
+ * the process.
+ */
+#ifndef CURAFFT_H
+#define CURAFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference entry points, double precision (complex128) ------------- */
+
+/* replaces ms2dirty_2, src/astro/cugridder.cu:541-547 (MS2DIRTY_2) */
+int ms2dirty_2(int nrow, int nxdirty, int nydirty, double fov, double freq,
+               double *uvw, double *vis, double *wgt, double *dirty,
+               double epsilon, double sigma, int sign);
+/* replaces ms2dirty_1, src/astro/cugridder.cu:549-555 (no weights) */
+int ms2dirty_1(int nrow, int nxdirty, int nydirty, double fov, double freq,
+               double *uvw, double *vis, double *dirty, double epsilon,
+               double sigma, int sign);
+/* replaces dirty2ms_1, src/astro/cugridder.cu:557-563 */
+int dirty2ms_1(int nrow, int nxdirty, int nydirty, double fov, double freq,
+               double *uvw, double *vis, double *dirty, double epsilon,
+               double sigma, int sign);
+/* replaces dirty2ms_2, src/astro/cugridder.cu:565-571 */
+int dirty2ms_2(int nrow, int nxdirty, int nydirty, double fov, double freq,
+               double *uvw, double *vis, double *wgt, double *dirty,
+               double epsilon, double sigma, int sign);
+
+/* ---- reference entry points, single precision (complex64, FP32 uvw) ---- */
+
+/* replaces ms2dirtyf_2 (the -DSINGLE build of cugridder.cu:541-547) */
+int ms2dirtyf_2(int nrow, int nxdirty, int nydirty, float fov, float freq,
+                float *uvw, float *vis, float *wgt, float *dirty,
+                float epsilon, float sigma, int sign);
+/* replaces ms2dirtyf_1 (-DSINGLE cugridder.cu:549-555) */
+int ms2dirtyf_1(int nrow, int nxdirty, int nydirty, float fov, float freq,
+                float *uvw, float *vis, float *dirty, float epsilon,
+                float sigma, int sign);
+/* replaces dirty2msf_1 (-DSINGLE cugridder.cu:557-563) */
+int dirty2msf_1(int nrow, int nxdirty, int nydirty, float fov, float freq,
+                float *uvw, float *vis, float *dirty, float epsilon,
+                float sigma, int sign);
+/* replaces dirty2msf_2 (-DSINGLE cugridder.cu:565-571) */
+int dirty2msf_2(int nrow, int nxdirty, int nydirty, float fov, float freq,
+                float *uvw, float *vis, float *wgt, float *dirty,
+                float epsilon, float sigma, int sign);
+
+/* ---- additions with no reference counterpart ---------------------------- */
+
+/* Mixed precision: FP64 uvw (coordinate scaling in FP64), complex64 vis and
+ * image, FP32 weights (NULL allowed).  Same semantics as ms2dirty_2 /
+ * dirty2ms_2; the float ABI above forces FP32 uvw. */
+int ms2dirty_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                   const double *uvw, const float *vis, const float *wgt,
+                   float *dirty, double epsilon, double sigma, int sign);
+int dirty2ms_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                   const double *uvw, float *vis, const float *wgt,
+                   const float *dirty, double epsilon, double sigma, int sign);
+
+/* 2-D NUFFT (the reference reaches these only through its test drivers,
+ * test/nufft_2d_1_test.cu:126-258 and test/nufft_2d_2_test.cu:140-315).
+ * Points x, y in [-pi, pi); modes fk[N2][N1], k1 = j - N1/2 (fast), k2 =
+ * i - N2/2.  type 1: fk = sum_j c_j e^{+i(k1 x_j + k2 y_j)};
+ * type 2: c_j = sum_k fk e^{-i(k1 x_j + k2 y_j)}. */
+int nufft2d1(int M, const double *x, const double *y, const double *c, int N1,
+             int N2, double *fk, double epsilon, double sigma);
+int nufft2d2(int M, const double *x, const double *y, double *c, int N1,
+             int N2, const double *fk, double epsilon, double sigma);
+int nufft2d1f(int M, const float *x, const float *y, const float *c, int N1,
+              int N2, float *fk, float epsilon, float sigma);
+int nufft2d2f(int M, const float *x, const float *y, float *c, int N1, int N2,
+              const float *fk, float epsilon, float sigma);
+
+/* Contexts: a reusable device workspace bound to a HIP stream (NULL = the
+ * null stream).  Device-pointer calls below run the same pipeline as the
+ * host entry points on inputs already in HBM. */
+int cg_ctx_create(void **ctx, void *stream);
+int cg_ctx_destroy(void *ctx);
+/* options: "timing" (0/1, per-stage HIP events), "lds_budget" (bytes),
+ * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
+ * "stop_after" (test hook, see below) */
+int cg_ctx_set_option(void *ctx, const char *name, int value);
+/* per-stage times summed over calls, out[12] = {ms2dirty[6], dirty2ms[6]}
+ * in ms, stages: plan, sort, records, grid/degrid, fft, image; calls[2] */
+int cg_ctx_stage_times(void *ctx, double *out, long long *calls);
+int cg_ctx_reset_times(void *ctx);
+/* parameters of the last call: nf1 nf2 nf3 kw beta es_c pix_x pix_y
+ * i_center i_half_width o_center o_half_width gamma h tile_x tile_y */
+int cg_ctx_last_plan(void *ctx, double *out);
+
+/* prec 0: complex64 vis/image, FP32 gridding and FFT; prec 1: complex128.
+ * uvw_prec, wgt_prec: 0 float, 1 double (prec 1 requires both 1).  Supported
+ * combinations: (0,1,0) mixed, (0,0,0) float, (1,1,1) double. */
+int cg_ms2dirty_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty,
+                    double fov, double freq, const void *d_uvw, int uvw_prec,
+                    const void *d_vis, const void *d_wgt, int wgt_prec,
+                    void *d_dirty, double epsilon, double sigma, int sign);
+int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty,
+                    double fov, double freq, const void *d_uvw, int uvw_prec,
+                    void *d_vis, const void *d_wgt, int wgt_prec,
+                    const void *d_dirty, double epsilon, double sigma, int sign);
+/* type 1 reads d_c and writes d_fk; type 2 reads d_fk and writes d_c */
+int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x,
+                   const void *d_y, void *d_c, int N1, int N2, void *d_fk,
+                   double epsilon, double sigma);
+
+/* Host-only planner (no device calls): out[16] = nf1 nf2 nf3 kw beta es_c
+ * pix_x pix_y i_center i_half_width o_center o_half_width gamma h f_over_c
+ * ier, given the range [wmin, wmax] of the scaled w = w * 2 pi f/c * sign. */
+int cg_plan_query(int nrow, int nxdirty, int nydirty, double fov, double freq,
+                  double wmin, double wmax, double epsilon, double sigma,
+                  int sign, int single, double *out);
+
+/* Test hooks.  Device pointers into the context workspace from the last call
+ * (valid until the next call): the w-plane grid and its cell count; the
+ * sorted tile keys (uint32), row permutation (int32), tile starts (int32)
+ * with n[0] = rows, n[1] = tiles.  Option "stop_after" (1: after gridding or
+ * the dirty2ms prologue, 2: after the plane FFTs) stops a call early. */
+int cg_ctx_grid(void *ctx, void **ptr, long long *cells);
+int cg_ctx_sorted(void *ctx, void **keys, void **perm, void **tstart,
+                  long long *n);
+/* In-place unnormalised 2-D FFT of batch planes [batch][n2][n1], exponent
+ * sign dir (+1/-1); prec 0 complex64, 1 complex128. */
+int cg_fft2d_dev(void *ctx, int prec, void *d_data, int batch, int n2, int n1,
+                 int dir);
+/* Stable sort of m (uint32 key, int32 value) pairs by the low key_bits bits,
+ * in place on the context stream. */
+int cg_sort_pairs_dev(void *ctx, void *d_keys, void *d_vals, int m,
+                      int key_bits);
+
+/* Device-to-device copy of bytes on a HIP stream (test hook). */
+int cg_memcpy_dev(void *dst, const void *src, void *stream, size_t bytes);
+
+int cg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CURAFFT_H */
