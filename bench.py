@@ -1,0 +1,301 @@
+"""Throughput of improved W-stacking ms2dirty + dirty2ms on one MI355X.
+
+One step = ms2dirty of the workload's visibilities followed by dirty2ms of a
+fixed image, on inputs already resident in HBM, through libcurafft.so's
+device-pointer entry points (the same kernels as the reference-compatible
+host entry points).  Prints one JSON line (see --help for the workloads).
+
+The default workload is the full W-stacking configuration (8192^2 image,
+fov 3 deg, 1.4 GHz, 2^27 visibilities from 256-antenna Earth-rotation tracks,
+eps 1e-6).  `--workload plain` is the one-channel 1024^2 case (131072 rows,
+eps 1e-5) and `--workload nufft2d` the 2-D type-1 + type-2 NUFFT (2^25
+points, 4096^2 modes, eps 1e-5).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C_LIGHT = 299792458.0
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mvis/s ms2dirty+dirty2ms at eps 1e-5, plain vs batched/learned; HBM GB/s"
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- workloads
+
+def make_plain(torch, dev):
+    """131072 rows, u,v ~ N(0, 1500 m), w ~ U(-300, 300) m, freq 1e8, fov 2
+    deg, 1024^2, vis complex64 N(0,1), wgt U(0.5, 1.5) float32, eps 1e-5."""
+    rng = np.random.default_rng(1)
+    M = 131072
+    uvw = np.empty((M, 3))
+    uvw[:, 0] = rng.normal(0, 1500, M)
+    uvw[:, 1] = rng.normal(0, 1500, M)
+    uvw[:, 2] = rng.uniform(-300, 300, M)
+    vis = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    wgt = rng.uniform(0.5, 1.5, M).astype(np.float32)
+    img = rng.standard_normal((1024, 1024)).astype(np.complex64)
+    t = lambda a: torch.tensor(a, device=dev)  # noqa: E731
+    return dict(name="plain one-channel 131072 rows 1024^2 fov 2deg 1e8 Hz eps 1e-5",
+                uvw=t(uvw), vis=t(vis), wgt=t(wgt), img=t(img), nx=1024, ny=1024, fov=2.0,
+                freq=1e8, eps=1e-5, M=M)
+
+
+def make_wstack_full(torch, dev, nrow=1 << 27, npix=8192, seed=3):
+    """256 antennas uniform in a 4 km disk (latitude 45 deg), Earth-rotation
+    tracks over 8 h at 10 s steps toward dec 45 deg, both baseline
+    orientations (65280 x 2880 = 188 M samples), 2^27 drawn without
+    replacement; positions scaled so max |w| = 5000 wavelengths at 1.4 GHz
+    (the raw 8 km baselines reach ~37000).  vis complex64 N(0,1), weights 1."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    f64 = torch.float64
+    nant = 256
+    r = 4000.0 * torch.sqrt(torch.rand(nant, generator=g, device=dev, dtype=f64))
+    th = 2 * np.pi * torch.rand(nant, generator=g, device=dev, dtype=f64)
+    E, N = r * torch.cos(th), r * torch.sin(th)
+    lat, dec = np.deg2rad(45.0), np.deg2rad(45.0)
+    X, Y, Z = -np.sin(lat) * N, E, np.cos(lat) * N
+    ii, jj = torch.triu_indices(nant, nant, 1, device=dev)
+    bx = torch.cat([X[jj] - X[ii], X[ii] - X[jj]])
+    by = torch.cat([Y[jj] - Y[ii], Y[ii] - Y[jj]])
+    bz = torch.cat([Z[jj] - Z[ii], Z[ii] - Z[jj]])
+    H = torch.linspace(-np.pi / 3, np.pi / 3, 2880, device=dev, dtype=f64)
+    sH, cH = torch.sin(H)[None, :], torch.cos(H)[None, :]
+    bx, by, bz = bx[:, None], by[:, None], bz[:, None]
+    u = (sH * bx + cH * by).reshape(-1)
+    v = (-np.sin(dec) * cH * bx + np.sin(dec) * sH * by + np.cos(dec) * bz).reshape(-1)
+    w = (np.cos(dec) * cH * bx - np.cos(dec) * sH * by + np.sin(dec) * bz).reshape(-1)
+    del bx, by, bz
+    sel = torch.randperm(u.numel(), generator=g, device=dev)[:nrow]
+    uvw = torch.stack([u[sel], v[sel], w[sel]], dim=1).contiguous()
+    del u, v, w, sel
+    freq = 1.4e9
+    lam = C_LIGHT / freq
+    uvw *= 5000.0 * lam / uvw[:, 2].abs().max()
+    vis = torch.complex(torch.randn(nrow, generator=g, device=dev),
+                        torch.randn(nrow, generator=g, device=dev))
+    img = torch.complex(torch.randn(npix, npix, generator=g, device=dev),
+                        torch.zeros(npix, npix, device=dev))
+    return dict(name="full W-stacking %d^2 fov 3deg 1.4 GHz %d rows eps 1e-6" % (npix, nrow),
+                uvw=uvw, vis=vis, wgt=None, img=img, nx=npix, ny=npix, fov=3.0, freq=freq,
+                eps=1e-6, M=nrow)
+
+
+def make_nufft2d(torch, dev, M=1 << 25, N=4096, seed=2):
+    """2^25 points in [-pi,pi)^2: 80% from 16 Gaussians (means uniform,
+    std 0.3, wrapped) + 20% uniform; strengths complex64 N(0,1); 4096^2."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    nb = int(0.8 * M)
+    means = (torch.rand(16, 2, generator=g, device=dev) * 2 - 1) * np.pi
+    comp = torch.randint(0, 16, (nb,), generator=g, device=dev)
+    pts = means[comp] + 0.3 * torch.randn(nb, 2, generator=g, device=dev)
+    unif = (torch.rand(M - nb, 2, generator=g, device=dev) * 2 - 1) * np.pi
+    pts = torch.cat([pts, unif])
+    pts = torch.remainder(pts + np.pi, 2 * np.pi) - np.pi
+    c = torch.complex(torch.randn(M, generator=g, device=dev), torch.randn(M, generator=g, device=dev))
+    fk = torch.complex(torch.randn(N, N, generator=g, device=dev), torch.randn(N, N, generator=g, device=dev))
+    return dict(name="2-D NUFFT type 1 + type 2, 2^25 clustered points, 4096^2 modes, eps 1e-5",
+                x=pts[:, 0].contiguous(), y=pts[:, 1].contiguous(), c=c, fk=fk, N=N, eps=1e-5, M=M)
+
+
+# ---------------------------------------------------------------- roofline
+
+def kernel_bytes(stage, direction, W, plan):
+    """Algorithmic (compulsory) HBM bytes of one launch of the stage's kernel,
+    complex64 grid: records are 24 B per row (FP32 offsets + packed cell +
+    complex value), the grid C = nf1*nf2*nf3 cells of 8 B."""
+    M = W["M"]
+    C = plan["nf1"] * plan["nf2"] * plan["nf3"]
+    npix = W["nx"] * W["ny"]
+    if stage == "grid" and direction == "ms2dirty":
+        return 24 * M + 8 * C                     # read records, write grid once
+    if stage == "grid" and direction == "dirty2ms":
+        return 8 * C + 24 * M + 4 * M + 8 * M      # read grid + records + perm, write vis
+    if stage in ("fft_rows", "fft_cols"):
+        return 16 * C                              # read + write every cell
+    if stage == "image" and direction == "ms2dirty":
+        return 8 * plan["nf3"] * npix + 8 * npix   # read P values per pixel, write image
+    if stage == "image" and direction == "dirty2ms":
+        return 8 * npix + 8 * C                    # read image, write every cell
+    return None
+
+
+def run_cpu_baseline(W, workload):
+    """FP64 OpenMP restatement of the reference algorithm (the oracle) on the
+    host cores, on a bounded sample of the workload."""
+    from oracle import oracle as O
+    threads = O.num_threads()
+    if workload == "plain":
+        uvw = W["uvw"].cpu().numpy()
+        vis = W["vis"].cpu().numpy().astype(np.complex128)
+        wgt = W["wgt"].cpu().numpy().astype(np.float64)
+        img = W["img"].cpu().numpy().astype(np.complex128)
+        sample = "the whole workload (131072 rows, 1024^2)"
+        nx, ny, fov, eps = W["nx"], W["ny"], W["fov"], W["eps"]
+    elif workload == "wstack":
+        # 2^20 rows of the same tracks, 1024^2 pixels of the same size
+        # (fov 3/8 deg), same eps: the full 16384^2 x 22-plane FP64 grid does
+        # not fit a host
+        M = 1 << 20
+        uvw = W["uvw"][:M].cpu().numpy()
+        vis = W["vis"][:M].cpu().numpy().astype(np.complex128)
+        wgt = None
+        nx = ny = 1024
+        fov, eps = W["fov"] / 8.0, W["eps"]
+        img = W["img"][:nx, :ny].cpu().numpy().astype(np.complex128)
+        sample = "2^20 rows of the same tracks, 1024^2 pixels of the same size (fov 0.375 deg), eps 1e-6"
+    else:
+        return None
+    t0 = time.perf_counter()
+    O.ms2dirty(uvw, W["freq"], vis, wgt, nx, ny, fov, eps)
+    t1 = time.perf_counter()
+    O.dirty2ms(uvw, W["freq"], img, wgt, fov, eps)
+    t2 = time.perf_counter()
+    mvis = uvw.shape[0] / (t2 - t0) / 1e6
+    log("cpu baseline: ms2dirty %.2f s, dirty2ms %.2f s on %d threads" % (t1 - t0, t2 - t1, threads))
+    return {"value": round(mvis, 4), "unit": "Mvis/s", "cores": threads, "kind": "port",
+            "sample": sample + "; FP64 C/OpenMP restatement of the reference algorithm"}
+
+
+def traffic_from_profiles(workload, kernel):
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel)
+    except Exception:
+        return None
+
+
+# ---------------------------------------------------------------- main
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d"), default="wstack")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--nrow", type=int, default=1 << 27, help="rows of the wstack workload")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box of this build has one MI355X and the "
+              "multi-GPU path is a design note (w-plane slab sharding), not code; run --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cugridder_amd import Context
+    dev = "cuda:0"
+    log("generating workload %s" % args.workload)
+    if args.workload == "plain":
+        W = make_plain(torch, dev)
+    elif args.workload == "wstack":
+        W = make_wstack_full(torch, dev, nrow=args.nrow)
+    else:
+        W = make_nufft2d(torch, dev)
+    torch.cuda.synchronize()
+    ctx = Context(timing=True)
+
+    if args.workload == "nufft2d":
+        out_fk = torch.empty(W["N"], W["N"], dtype=torch.complex64, device=dev)
+        out_c = torch.empty(W["M"], dtype=torch.complex64, device=dev)
+
+        def step():
+            ctx.nufft2d1(W["x"], W["y"], W["c"], W["N"], W["N"], W["eps"], out=out_fk)
+            ctx.nufft2d2(W["x"], W["y"], W["fk"], W["eps"], out=out_c)
+    else:
+        out_d = torch.empty(W["nx"], W["ny"], dtype=torch.complex64, device=dev)
+        out_v = torch.empty(W["M"], dtype=torch.complex64, device=dev)
+
+        def step():
+            ctx.ms2dirty(W["uvw"], W["vis"], W["wgt"], W["nx"], W["ny"], W["fov"], W["freq"],
+                         W["eps"], out=out_d)
+            ctx.dirty2ms(W["uvw"], W["img"], W["wgt"], W["fov"], W["freq"], W["eps"], out=out_v)
+
+    for i in range(args.warmup):
+        step()
+        log("warmup step %d done" % i)
+    torch.cuda.synchronize()
+    ctx.reset_times()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        step()
+        log("timed step %d done" % i)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev0.elapsed_time(ev1) / args.steps
+    value = W["M"] / (ms_per_step * 1e-3) / 1e6
+    st = ctx.stage_times()
+    plan = ctx.last_plan()
+    calls = max(st["calls"][0], 1)
+    stage_ms = {d: {k: v / calls for k, v in st[d].items()} for d in ("ms2dirty", "dirty2ms")}
+    log("stage ms per call: %s" % json.dumps({d: {k: round(v, 3) for k, v in s.items()}
+                                              for d, s in stage_ms.items()}))
+
+    roof = None
+    if args.workload != "nufft2d":
+        best = None
+        for d in ("ms2dirty", "dirty2ms"):
+            for k in ("grid", "fft_rows", "fft_cols", "image"):
+                if best is None or stage_ms[d][k] > best[2]:
+                    best = (d, k, stage_ms[d][k])
+        d, k, t_ms = best
+        nbytes = kernel_bytes(k, d, W, plan)
+        achieved = nbytes / (t_ms * 1e-3) / 1e9
+        kname = {"grid": "k_grid" if d == "ms2dirty" else "k_degrid", "fft_rows": "k_fft_rows",
+                 "fft_cols": "k_fft_cols",
+                 "image": "k_ms2dirty_epi" if d == "ms2dirty" else "k_dirty2ms_pro"}[k]
+        roof = {"kernel": "%s (%s)" % (kname, d), "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(args.workload, kname),
+                "algorithmic_bytes": nbytes, "avg_ms": round(t_ms, 4)}
+    else:
+        t_grid = stage_ms["ms2dirty"]["grid"]
+        nbytes = 16 * W["M"] + 8 * (2 * W["N"]) ** 2
+        achieved = nbytes / (t_grid * 1e-3) / 1e9
+        roof = {"kernel": "k_grid (type 1)", "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "algorithmic_bytes": nbytes, "avg_ms": round(t_grid, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = run_cpu_baseline(W, args.workload)
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %r" % (e,))
+
+    unit = "Mpts/s" if args.workload == "nufft2d" else "Mvis/s"
+    cfg = {"workload": W["name"], "nrow": W["M"], "eps": W["eps"],
+           "precision": "complex64 gridding/FFT, FP64 coordinates, phases and corrections"}
+    if args.workload != "nufft2d":
+        cfg.update({"image": "%dx%d" % (W["nx"], W["ny"]), "fov_deg": W["fov"], "freq_hz": W["freq"],
+                    "nf": [plan["nf1"], plan["nf2"], plan["nf3"]], "kw": plan["kw"]})
+    out = {"metric": METRIC, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+           "data": "synthetic", "config": cfg, "stage_ms": {d: {k: round(v, 3) for k, v in s.items()}
+                                                            for d, s in stage_ms.items()},
+           "roofline": roof, "cpu_baseline": cpu, "wall_s": round(wall, 2)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

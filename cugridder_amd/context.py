@@ -10,7 +10,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-STAGES = ("plan", "sort", "records", "grid", "fft", "image")
+STAGES = ("plan", "sort", "records", "grid", "fft_rows", "fft_cols", "image")
 
 
 def _ptr(t):
@@ -115,11 +115,12 @@ class Context:
     # ------------------------------------------------------------ introspection
     def stage_times(self):
         """{'ms2dirty': {stage: ms}, 'dirty2ms': {...}, 'calls': (n1, n2)} summed."""
-        out = np.zeros(12)
+        n = len(STAGES)
+        out = np.zeros(2 * n)
         calls = (ctypes.c_longlong * 2)()
         check(lib().cg_ctx_stage_times(self._h, out, calls), "stage_times")
-        return {"ms2dirty": dict(zip(STAGES, out[:6].tolist())),
-                "dirty2ms": dict(zip(STAGES, out[6:].tolist())),
+        return {"ms2dirty": dict(zip(STAGES, out[:n].tolist())),
+                "dirty2ms": dict(zip(STAGES, out[n:].tolist())),
                 "calls": (calls[0], calls[1])}
 
     def reset_times(self):

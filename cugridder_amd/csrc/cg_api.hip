@@ -33,12 +33,12 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_IMAGE, ST_COUNT };
+enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_FFT2, ST_IMAGE, ST_COUNT };
 
 struct Ctx {
     hipStream_t stream = nullptr;
     int lds_budget = 65536;
-    int tile_grid = 16;
+    int tile_grid = 8;
     int tile_degrid = 32;
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
@@ -226,7 +226,8 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     int ier = plan_call<T, TU>(C, P, nrow, nx, ny, fov, freq, uvw, eps, sigma, sign, single);
     if (ier >= 2) return ier;
     const int eb = (int)sizeof(cx_t<T>);
-    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, eb, C.lds_budget, C.tile_grid, 0);
+    // gridding accumulates in FP64 (16 bytes per cell) whatever the precision
+    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, C.lds_budget, C.tile_grid, 0);
     C.last_plan = P;
     C.last_tiling = t;
     C.last_m = nrow;
@@ -250,10 +251,13 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
     if (C.stop_after == 1) return P.ier;
-    CG_TRY(clk.mark(ST_FFT));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, as<cx_t<T>>(C.tw1),
-                           as<cx_t<T>>(C.tw2), C.stream));
+    CG_TRY(clk.mark(ST_FFT));
+    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, true,
+                              as<cx_t<T>>(C.tw1), C.stream));
+    CG_TRY(clk.mark(ST_FFT2));
+    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, false,
+                              as<cx_t<T>>(C.tw2), C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_IMAGE));
     const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
@@ -316,10 +320,13 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                                                   as<double>(C.phi1), as<double>(C.phi2),
                                                   as<cx_t<T>>(C.grid), w, C.stream)));
     if (C.stop_after == 1) return P.ier;
-    CG_TRY(clk.mark(ST_FFT));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, as<cx_t<T>>(C.tw1),
-                           as<cx_t<T>>(C.tw2), C.stream));
+    CG_TRY(clk.mark(ST_FFT));
+    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, true,
+                              as<cx_t<T>>(C.tw1), C.stream));
+    CG_TRY(clk.mark(ST_FFT2));
+    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, false,
+                              as<cx_t<T>>(C.tw2), C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_GRID));
     GridArgs g = grid_args(P, t, M);
@@ -336,7 +343,8 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
         CG_HIP(hipEventElapsedTime(&ms[ST_SORT], C.ev[ST_SORT], C.ev[ST_RECORDS]));
         CG_HIP(hipEventElapsedTime(&ms[ST_RECORDS], C.ev[ST_RECORDS], C.ev[ST_IMAGE]));
         CG_HIP(hipEventElapsedTime(&ms[ST_IMAGE], C.ev[ST_IMAGE], C.ev[ST_FFT]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_GRID]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_FFT2]));
+        CG_HIP(hipEventElapsedTime(&ms[ST_FFT2], C.ev[ST_FFT2], C.ev[ST_GRID]));
         CG_HIP(hipEventElapsedTime(&ms[ST_GRID], C.ev[ST_GRID], C.ev[ST_COUNT]));
         for (int i = 0; i < ST_COUNT; i++) C.stage_ms[1][i] += ms[i];
         C.stage_calls[1]++;
@@ -367,7 +375,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = conv.kw;
-    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_grid, 0)
+    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.lds_budget, C.tile_grid, 0)
                          : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
     C.last_plan = P;
     C.last_tiling = t;
@@ -420,8 +428,11 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                               as<cx_t<T>>(C.grid), g, kw, C.stream));
         CG_TRY(clk.mark(ST_FFT));
-        CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, as<cx_t<T>>(C.tw1),
-                               as<cx_t<T>>(C.tw2), C.stream));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, true,
+                                  as<cx_t<T>>(C.tw1), C.stream));
+        CG_TRY(clk.mark(ST_FFT2));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, false,
+                                  as<cx_t<T>>(C.tw2), C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 1, C.stream)));
@@ -430,8 +441,11 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 2, C.stream)));
         CG_TRY(clk.mark(ST_FFT));
-        CG_TRY(launch_fft2d<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, as<cx_t<T>>(C.tw1),
-                               as<cx_t<T>>(C.tw2), C.stream));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, true,
+                                  as<cx_t<T>>(C.tw1), C.stream));
+        CG_TRY(clk.mark(ST_FFT2));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, false,
+                                  as<cx_t<T>>(C.tw2), C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
@@ -691,7 +705,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     return CG_OK;
 }
 
-// stage times summed over calls: out[dir*6 + stage] (ms); returns calls
+// stage times summed over calls: out[dir*7 + stage] (ms); returns calls
 int cg_ctx_stage_times(void *ctx, double *out, long long *calls)
 {
     Ctx *C = (Ctx *)ctx;

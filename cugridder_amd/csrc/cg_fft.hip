@@ -289,6 +289,15 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
 }
 
 template <typename T>
+int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
+                    const cx_t<T> *tw, hipStream_t s)
+{
+    if (batch <= 0 || (!rows && n2 <= 1)) return CG_OK;
+    return dir > 0 ? fft_axis<T, 1>(data, batch, n2, n1, rows, tw, s)
+                   : fft_axis<T, -1>(data, batch, n2, n1, rows, tw, s);
+}
+
+template <typename T>
 __global__ void k_twiddles(cx_t<T> *tw, int n)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,6 +317,8 @@ int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s)
 
 template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, hipStream_t);
 template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, hipStream_t);
+template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, hipStream_t);
+template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, hipStream_t);
 template bool fft_supported<float>(int);
 template bool fft_supported<double>(int);
 template int launch_twiddles<float>(float2 *, int, hipStream_t);

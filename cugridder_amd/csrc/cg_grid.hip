@@ -7,7 +7,9 @@
 // |d| < kw/2 (conv.cu:32-36, unnormalised as in the reference).
 //
 // MI355X design: one workgroup owns a tx*ty tile of grid cells for a chunk of
-// w-planes and accumulates it in LDS with ds_add atomics; it reads the points
+// w-planes and accumulates it in LDS with FP64 ds_add atomics (on gfx950 the
+// FP32 LDS atomic add runs ~20x slower than the FP64 one, and FP64 also makes
+// the accumulation order-insensitive at FP32 output precision); it reads the points
 // of the bins whose footprints reach the tile (points are binned by the tile
 // of their footprint start, so at most sbx*sby bins feed a tile) and writes
 // the finished tile once with coalesced stores.  No HBM atomics and no grid
@@ -38,7 +40,8 @@ __device__ __forceinline__ double es_val<double>(double d, double beta, double c
     return fabs(d) < hw ? exp(beta * sqrt(1.0 - c * d * d)) : 0.0;
 }
 
-int grid_lds_bytes(const GridArgs &g, int elem) { return g.pc * g.tx * g.ty * elem; }
+// gridding accumulates in FP64 whatever the grid precision: 16 bytes per cell
+int grid_lds_bytes(const GridArgs &g, int) { return g.pc * g.tx * g.ty * 16; }
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem)
 {
     return g.pc * (g.tx + kw - 1) * (g.ty + kw - 1) * elem;
@@ -53,14 +56,14 @@ __global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
                                               cx_t<T> *__restrict__ fw, GridArgs g, int is2d)
 {
     extern __shared__ unsigned char smem_raw[];
-    T *acc = reinterpret_cast<T *>(smem_raw);
+    double *acc = reinterpret_cast<double *>(smem_raw);
     const int chunk = blockIdx.x % g.nchunk;
     const int tile = blockIdx.x / g.nchunk;
     const int bx = tile % g.ntx, by = tile / g.ntx;
     const int p0 = chunk * g.pc;
     const int np = min(g.pc, g.nf3 - p0);
     const int cells = np * g.ty * g.tx;
-    for (int k = threadIdx.x; k < 2 * cells; k += blockDim.x) acc[k] = (T)0;
+    for (int k = threadIdx.x; k < 2 * cells; k += blockDim.x) acc[k] = 0.0;
     __syncthreads();
 
     const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
@@ -109,14 +112,14 @@ __global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
                         if (yy >= g.nf2) yy -= g.nf2;
                         if (yy >= g.ty) continue;
                         const T vyr = vzr * ky[t2], vyi = vzi * ky[t2];
-                        T *row = acc + 2 * ((zz * g.ty + yy) * g.tx);
+                        double *row = acc + 2 * ((zz * g.ty + yy) * g.tx);
 #pragma unroll
                         for (int t1 = 0; t1 < KW; t1++) {
                             int xx = rx + t1;
                             if (xx >= g.nf1) xx -= g.nf1;
                             if (xx >= g.tx) continue;
-                            atomicAdd(row + 2 * xx, vyr * kx[t1]);
-                            atomicAdd(row + 2 * xx + 1, vyi * kx[t1]);
+                            atomicAdd(row + 2 * xx, (double)(vyr * kx[t1]));
+                            atomicAdd(row + 2 * xx + 1, (double)(vyi * kx[t1]));
                         }
                     }
                 }
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
         int rem = k - lz * plane_cells;
         int y = rem / g.tx, x = rem - y * g.tx;
         size_t gi = ((size_t)(p0 + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
-        fw[gi] = mkc(acc[2 * k], acc[2 * k + 1]);
+        fw[gi] = mkc((T)acc[2 * k], (T)acc[2 * k + 1]);
     }
 }
 

@@ -87,6 +87,10 @@ int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
 template <typename T>
 int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T> *tw1,
                  const cx_t<T> *tw2, hipStream_t s);
+// one pass of launch_fft2d: rows (fast axis) or columns
+template <typename T>
+int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
+                    const cx_t<T> *tw, hipStream_t s);
 template <typename T>
 bool fft_supported(int n);
 template <typename T>

@@ -10,6 +10,10 @@
 // radix sort (cg_sort.hip).
 #include "cg_internal.hpp"
 
+// Coordinates are computed with separately rounded IEEE operations (no FMA
+// contraction), so tile keys equal a host FP64 evaluation bit for bit.
+#pragma clang fp contract(off)
+
 namespace cg {
 
 CoordArgs make_coord_args(const WPlan &p, const Tiling &t)

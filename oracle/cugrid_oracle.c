@@ -657,6 +657,81 @@ int orc_dirty2ms(int nrow, int nxdirty, int nydirty, double fov, double freq,
     return ier;
 }
 
+/* Stage exports for kernel-level checks: the ms2dirty grid right after
+ * spreading (before the plane FFTs) and the dirty2ms grid right after the
+ * w-IDFT (before the FFTs), exactly as inside orc_ms2dirty / orc_dirty2ms. */
+int orc_ms2dirty_grid(int nrow, int nxdirty, int nydirty, double fov,
+                      double freq, const double *uvw, const double *vis,
+                      const double *wgt, double eps, double sigma, int sign,
+                      int single, double *fwout)
+{
+    orc_plan P;
+    int ier = orc_make_plan(nrow, nxdirty, nydirty, fov, freq, uvw, eps, sigma,
+                            sign, single, &P);
+    if (ier == 2) return ier;
+    int M = nrow;
+    double *x = malloc(sizeof(double) * (M + 1));
+    double *y = malloc(sizeof(double) * (M + 1));
+    double *z = malloc(sizeof(double) * (M + 1));
+    double *wt = malloc(sizeof(double) * (M + 1));
+    cplx *c = malloc(sizeof(cplx) * (M + 1));
+    orc_coords(&P, uvw, x, y, z, wt);
+    const cplx *vv = (const cplx *)vis;
+    for (int j = 0; j < M; j++) {
+        c[j] = vv[j];
+        if (wgt) c[j] *= wgt[j];
+        if (P.o_center != 0) c[j] *= cexp(I * (P.o_center * wt[j]));
+    }
+    memset(fwout, 0, sizeof(cplx) * (size_t)P.nf1 * P.nf2 * P.nf3);
+    orc_spread3d(M, x, y, z, (const double *)c, P.nf1, P.nf2, P.nf3, P.kw, P.beta,
+                 P.es_c, 0, fwout);
+    free(x); free(y); free(z); free(wt); free(c);
+    return ier;
+}
+
+int orc_dirty2ms_planes(int nrow, int nxdirty, int nydirty, double fov,
+                        double freq, const double *uvw, const double *dirty,
+                        double eps, double sigma, int sign, int single,
+                        double *fwout)
+{
+    orc_plan P;
+    int ier = orc_make_plan(nrow, nxdirty, nydirty, fov, freq, uvw, eps, sigma,
+                            sign, single, &P);
+    if (ier == 2) return ier;
+    int N1 = P.N1, N2 = P.N2, nf1 = P.nf1, nf2 = P.nf2, nf3 = P.nf3;
+    int W = N1 / 2 + 1, H = N2 / 2 + 1;
+    double *zq = malloc(sizeof(double) * W * H);
+    orc_k_generation(&P, zq);
+    for (int q = 0; q < W * H; q++) zq[q] = (zq[q] - P.o_center) * P.gamma * P.h;
+    double *phi1 = malloc(sizeof(double) * (nf1 / 2 + 1));
+    double *phi2 = malloc(sizeof(double) * (nf2 / 2 + 1));
+    double *phi3 = malloc(sizeof(double) * W * H);
+    orc_fseries_grid(P.kw, P.beta, P.es_c, nf1, phi1);
+    orc_fseries_grid(P.kw, P.beta, P.es_c, nf2, phi2);
+    orc_fseries_at(P.kw, P.beta, P.es_c, zq, W * H, phi3);
+    size_t plane = (size_t)nf1 * nf2;
+    cplx *fw = (cplx *)fwout;
+    memset(fw, 0, sizeof(cplx) * plane * nf3);
+    const cplx *in = (const cplx *)dirty;
+    for (int idx = 0; idx < N1 * N2; idx++) {
+        int row = idx / N1, col = idx % N1;
+        double l = (row - N2 / 2) * P.pix_x, m = (col - N1 / 2) * P.pix_y;
+        double tmp = 1.0 - l * l - m * m;
+        cplx f = in[idx] / sqrt(tmp);
+        double nm1 = tmp < 0 ? sqrt(-tmp) - 1 : sqrt(tmp) - 1;
+        int iz = abs(col - N1 / 2) + abs(row - N2 / 2) * W;
+        f = f * cexp(I * (-(nm1 - P.o_center) * P.i_center)) / phi3[iz];
+        f /= phi1[abs(col - N1 / 2)] * phi2[abs(row - N2 / 2)];
+        int w1 = col >= N1 / 2 ? col - N1 / 2 : nf1 + col - N1 / 2;
+        int w2 = row >= N2 / 2 ? row - N2 / 2 : nf2 + row - N2 / 2;
+        size_t ifw = (size_t)w1 + (size_t)w2 * nf1;
+        for (int p = 0; p < nf3; p++)
+            fw[ifw + plane * p] = f * cexp(I * (-zq[iz] * (p - nf3 / 2)));
+    }
+    free(zq); free(phi1); free(phi2); free(phi3);
+    return ier;
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 direct DFT oracle (python/curagridder/test_wst.py:15-32 formula with  */
 /* the sign convention of the C-ABI; w_s_degridder_test.cu:249-263 adjoint). */

@@ -80,6 +80,12 @@ def lib():
                                       _dp, c_double, c_double, c_int, c_int, _dp]
         L.orc_plan_export.restype = c_int
         L.orc_num_threads.restype = c_int
+        L.orc_ms2dirty_grid.argtypes = [c_int, c_int, c_int, c_double, c_double, _dp, _cp,
+                                        ctypes.c_void_p, c_double, c_double, c_int, c_int, _cp]
+        L.orc_ms2dirty_grid.restype = c_int
+        L.orc_dirty2ms_planes.argtypes = [c_int, c_int, c_int, c_double, c_double, _dp, _cp,
+                                          c_double, c_double, c_int, c_int, _cp]
+        L.orc_dirty2ms_planes.restype = c_int
         _LIB = L
     return _LIB
 
@@ -221,6 +227,31 @@ def dirty2ms(uvw, freq, dirty, wgt, fov, eps, sigma=2.0, sign=-1,
     if ier >= 2:
         raise ValueError("oracle dirty2ms: bad parameters (ier=%d)" % ier)
     return vis
+
+
+def ms2dirty_grid(uvw, freq, vis, wgt, nxdirty, nydirty, fov, eps, sigma=2.0, sign=-1,
+                  single=False):
+    """ms2dirty's w-plane grid after spreading, before the FFTs [nf3, nf2, nf1]."""
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    p = plan(uvw.shape[0], nxdirty, nydirty, fov, freq, uvw, eps, sigma, sign, single)
+    vis = np.ascontiguousarray(vis, dtype=np.complex128).ravel()
+    w, wp = _wptr(wgt)
+    fw = np.zeros((p["nf3"], p["nf2"], p["nf1"]), dtype=np.complex128)
+    lib().orc_ms2dirty_grid(uvw.shape[0], nxdirty, nydirty, fov, freq, uvw, vis, wp, eps, sigma,
+                            sign, int(single), fw)
+    return fw
+
+
+def dirty2ms_planes(uvw, freq, dirty, fov, eps, sigma=2.0, sign=-1, single=False):
+    """dirty2ms's w-plane grid after the w-IDFT, before the FFTs."""
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    dirty = np.ascontiguousarray(dirty, dtype=np.complex128)
+    nx, ny = dirty.shape
+    p = plan(uvw.shape[0], nx, ny, fov, freq, uvw, eps, sigma, sign, single)
+    fw = np.zeros((p["nf3"], p["nf2"], p["nf1"]), dtype=np.complex128)
+    lib().orc_dirty2ms_planes(uvw.shape[0], nx, ny, fov, freq, uvw, dirty, eps, sigma, sign,
+                              int(single), fw)
+    return fw
 
 
 def dft_ms2dirty(uvw, freq, vis, wgt, nxdirty, nydirty, pix_x, pix_y, sign=-1,

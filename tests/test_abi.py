@@ -1,0 +1,73 @@
+"""CPU checks of the drop-in boundary: libcurafft.so loads, exports every
+function include/curafft.h declares, and its host-only planner agrees with
+the oracle's restatement of the reference planner."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cugridder_amd
+from oracle import oracle as O
+from tests.conftest import ROOT, wst_case
+
+HEADER = os.path.join(ROOT, "include", "curafft.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*int\s+(\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_reference_symbols():
+    names = declared_functions()
+    for ref in ("ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2", "ms2dirtyf_1",
+                "ms2dirtyf_2", "dirty2msf_1", "dirty2msf_2"):
+        assert ref in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(cugridder_amd.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_declares_argtypes():
+    L = cugridder_amd.lib()
+    assert L.cg_version() == 100
+    for n in declared_functions():
+        assert getattr(L, n).restype is ctypes.c_int
+
+
+@pytest.mark.parametrize("nrow,nx,ny,fov,eps,freq", [
+    (100, 32, 64, 10.0, 1e-10, 1e9), (27, 16, 250, 20.0, 5e-7, 1e9), (2, 64, 128, 1.0, 5e-3, 1e9),
+    (1000, 1024, 1024, 2.0, 1e-5, 1e8), (500, 8192, 8192, 3.0, 1e-6, 1.4e9), (10, 250, 64, 1.0, 2e-1, 1e9)])
+@pytest.mark.parametrize("single", [False, True])
+def test_planner_matches_oracle(nrow, nx, ny, fov, eps, freq, single):
+    k = wst_case(nrow, nx, ny, fov, f0=freq)
+    uvw = k["uvw"]
+    ref = O.plan(nrow, nx, ny, fov, freq, uvw, eps, sign=-1, single=single)
+    w = uvw[:, 2] * freq / 299792458.0 * -1 * 2 * np.pi
+    out = np.zeros(16)
+    ier = cugridder_amd.lib().cg_plan_query(nrow, nx, ny, fov, freq, w.min(), w.max(), eps, 2.0, -1,
+                                           int(single), out)
+    assert ier == ref["ier"]
+    keys = ("nf1", "nf2", "nf3", "kw", "beta", "es_c", "pix_x", "pix_y", "i_center",
+            "i_half_width", "o_center", "o_half_width", "gamma", "h", "f_over_c")
+    for i, key in enumerate(keys):
+        # i_center = (wmax + wmin)/2 cancels; both sides scale w in a
+        # different multiplication order
+        tol_abs = 1e-13 * ref["i_half_width"] if key == "i_center" else 1e-300
+        assert out[i] == pytest.approx(ref[key], rel=1e-12, abs=tol_abs), key
+
+
+def test_planner_rejects_bad_arguments():
+    out = np.zeros(16)
+    L = cugridder_amd.lib()
+    assert L.cg_plan_query(10, 0, 64, 1.0, 1e9, 0.0, 1.0, 1e-5, 2.0, -1, 0, out) == 2
+    assert L.cg_plan_query(10, 64, 64, -1.0, 1e9, 0.0, 1.0, 1e-5, 2.0, -1, 0, out) == 2
+    assert L.cg_plan_query(10, 64, 64, 1.0, 1e9, 0.0, 1.0, 1e-5, 1.0, -1, 0, out) == 2
+    # eps below the precision floor is clamped with ier = 1 (setup_conv_opts)
+    assert L.cg_plan_query(10, 64, 64, 1.0, 1e9, 0.0, 1.0, 1e-9, 2.0, -1, 1, out) == 1

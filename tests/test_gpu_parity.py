@@ -1,0 +1,170 @@
+"""End-to-end parity on the GPU through the C-ABI (host entry points used by
+the reference's ctypes binding, and the device-pointer entry points).
+
+Criteria: against the FP64 restatement of the reference algorithm (same
+inputs, same kw/beta/nf/nf3), relative L2 <= eps (FP64) or <= max(eps, 2e-6)
+(FP32 arithmetic); against the exact direct sum, the method's own accuracy
+(5 eps for ms2dirty, 100 eps for dirty2ms on few rows; see test_oracle.py);
+adjointness as the reference asserts it (rtol 1e-12 FP64, 5e-5 FP32,
+python/curagridder/test_wst.py:84-89)."""
+import glob
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import as_float_abi, rel_l2, wst_case
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _floor(single):
+    return 2e-6 if single else 1e-13
+
+
+# reference parameter grid, python/curagridder/test_wst.py:36-43 (nx/ny paired)
+SWEEP = list(itertools.product((2, 27, 100), ((32, 128), (64, 250), (16, 64)), (1, 10, 20),
+                               (2e-1, 5e-3, 5e-5, 5e-7, 5e-12), (False, True), (False, True)))
+
+
+@pytest.mark.parametrize("nrow,shape,fov,eps,single,use_wgt", SWEEP)
+def test_wst_sweep(gpu, nrow, shape, fov, eps, single, use_wgt):
+    if single and eps < 5e-6:
+        pytest.skip("single precision below 5e-6 (as the reference's test_wst.py:52-53)")
+    import cugridder_amd as cg
+    nx, ny = shape
+    k = wst_case(nrow, nx, ny, fov, use_wgt=use_wgt)
+    ms, dirty, wgt = k["ms"], k["dirty"], k["wgt"]
+    if single:
+        ms = ms.astype("c8")
+        dirty = dirty.astype("f4")
+        wgt = wgt.astype("f4") if wgt is not None else None
+    d2 = cg.ms2dirty(k["uvw"], [k["f0"]], ms, wgt, nx, ny, k["xpix"], k["ypix"], 0, 0, eps,
+                     True).astype("f8")
+    m2 = cg.dirty2ms(k["uvw"], [k["f0"]], dirty, wgt, k["xpix"], k["ypix"], 0, 0, eps,
+                     True).astype("c16").ravel()
+    tol = 5e-5 if single else 1e-12
+    np.testing.assert_allclose(np.vdot(ms, m2).real, np.vdot(d2, dirty), rtol=tol)
+    # the oracle gets exactly the inputs the entry point received
+    w64 = None if wgt is None else wgt.astype("f8")
+    if single:
+        uvw_r, fov_r, f0_r, eps_r = (as_float_abi(k["uvw"]), as_float_abi(k["xpix"] * nx * 180 / np.pi),
+                                     as_float_abi(k["f0"]), as_float_abi(eps))
+    else:
+        uvw_r, fov_r, f0_r, eps_r = k["uvw"], k["xpix"] * nx * 180 / np.pi, k["f0"], eps
+    ref_d = O.ms2dirty(uvw_r, f0_r, ms.astype("c16"), w64, nx, ny, fov_r, eps_r,
+                       single=single).real
+    ref_v = O.dirty2ms(uvw_r, f0_r, dirty.astype("c16"), w64, fov_r, eps_r, single=single)
+    assert rel_l2(d2, ref_d) <= max(eps, _floor(single))
+    assert rel_l2(m2, ref_v) <= max(eps, _floor(single))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "wstack_*.npz"))))
+@pytest.mark.parametrize("prec", ["double", "single", "mixed"])
+def test_golden_wstack(gpu, path, prec):
+    import cugridder_amd as cg
+    from cugridder_amd import curagridder as cga
+    g = np.load(path)
+    nx, ny = g["dirty"].shape
+    f0, fov = float(g["freq"]), float(g["fov"])
+    eps = 1e-9 if prec == "double" else 1e-5
+    L = cg.lib()
+    if prec == "double":
+        d = np.zeros((nx, ny), np.complex128)
+        assert L.ms2dirty_2(len(g["vis"]), nx, ny, fov, f0, g["uvw"], g["vis"], g["wgt"], d, eps,
+                            2.0, -1) == 0
+        v = np.zeros(len(g["vis"]), np.complex128)
+        assert L.dirty2ms_2(len(g["vis"]), nx, ny, fov, f0, g["uvw"], v, g["wgt"], g["dirty"], eps,
+                            2.0, -1) == 0
+    elif prec == "single":
+        d = np.zeros((nx, ny), np.complex64)
+        assert L.ms2dirtyf_2(len(g["vis"]), nx, ny, fov, f0, g["uvw"].astype("f4"),
+                             g["vis"].astype("c8"), g["wgt"].astype("f4"), d, eps, 2.0, -1) == 0
+        v = np.zeros(len(g["vis"]), np.complex64)
+        assert L.dirty2msf_2(len(g["vis"]), nx, ny, fov, f0, g["uvw"].astype("f4"), v,
+                             g["wgt"].astype("f4"), g["dirty"].astype("c8"), eps, 2.0, -1) == 0
+    else:
+        d = cga.vis2dirty_mixed(g["uvw"], [f0], g["vis"], g["wgt"], nx, ny, fov, eps)
+        v = cga.dirty2vis_mixed(g["uvw"], [f0], g["dirty"], g["wgt"], fov, eps)
+    single = prec != "double"
+    assert rel_l2(d, g["dft_dirty"]) < 5 * eps + (3e-6 if single else 0)
+    assert rel_l2(v, g["dft_vis"]) < 100 * eps + (3e-6 if single else 0)
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_golden_nufft2d(gpu, prec):
+    from cugridder_amd import curagridder as cga
+    g = np.load(os.path.join(GOLDEN, "nufft2d_500_48x32.npz"))
+    N2, N1 = g["fk"].shape
+    eps = 1e-9 if prec == "double" else 1e-5
+    cdt = np.complex128 if prec == "double" else np.complex64
+    fdt = np.float64 if prec == "double" else np.float32
+    f = cga.nufft2d1(g["x"].astype(fdt), g["y"].astype(fdt), g["c"].astype(cdt), N1, N2, eps)
+    c = cga.nufft2d2(g["x"].astype(fdt), g["y"].astype(fdt), g["fk"].astype(cdt), eps)
+    floor = 0 if prec == "double" else 3e-6
+    assert rel_l2(f, g["dft_type1"]) < 10 * eps + floor
+    assert rel_l2(c, g["dft_type2"]) < 10 * eps + floor
+    if prec == "double":
+        ref1 = O.nufft2d1(g["x"], g["y"], g["c"], N1, N2, eps)
+    else:
+        ref1 = O.nufft2d1(as_float_abi(g["x"]), as_float_abi(g["y"]), as_float_abi(g["c"]), N1, N2,
+                          as_float_abi(eps), single=True)
+    assert rel_l2(f, ref1) <= max(eps, _floor(prec != "double"))
+
+
+def test_cfg1_plain_full_size(gpu):
+    """The plain one-channel workload (131072 rows, 1024^2, fov 2 deg,
+    freq 1e8, eps 1e-5, mixed precision) against the FP64 restatement on the
+    whole image and the direct sum on sampled pixels/rows."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(1)
+    M = 131072
+    uvw = np.empty((M, 3))
+    uvw[:, 0] = rng.normal(0, 1500, M)
+    uvw[:, 1] = rng.normal(0, 1500, M)
+    uvw[:, 2] = rng.uniform(-300, 300, M)
+    vis = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    wgt = rng.uniform(0.5, 1.5, M).astype(np.float32)
+    ctx = Context()
+    tu, tv, tw = (torch.tensor(a, device="cuda") for a in (uvw, vis, wgt))
+    d = ctx.ms2dirty(tu, tv, tw, 1024, 1024, 2.0, 1e8, 1e-5).cpu().numpy()
+    img = (rng.standard_normal((1024, 1024))).astype(np.complex64)
+    v = ctx.dirty2ms(tu, torch.tensor(img, device="cuda"), tw, 2.0, 1e8, 1e-5).cpu().numpy()
+    ref = O.ms2dirty(uvw, 1e8, vis.astype("c16"), wgt.astype("f8"), 1024, 1024, 2.0, 1e-5)
+    assert rel_l2(d, ref) <= 1e-5
+    refv = O.dirty2ms(uvw, 1e8, img.astype("c16"), wgt.astype("f8"), 2.0, 1e-5)
+    assert rel_l2(v, refv) <= 1e-5
+    pix = np.random.default_rng(5).choice(1024 * 1024, 1000, replace=False).astype(np.int64)
+    px = 2.0 * np.pi / 180 / 1024
+    t = O.dft_ms2dirty(uvw, 1e8, vis.astype("c16"), wgt.astype("f8"), 1024, 1024, px, px,
+                       pixels=pix)
+    assert rel_l2(d.ravel()[pix], t) < 3e-5
+    a1 = np.vdot(vis.astype("c16"), v.astype("c16")).real
+    a2 = np.vdot(d.astype("c16"), img.astype("c16")).real
+    assert abs(a1 - a2) <= 1e-5 * abs(a2)
+
+
+def test_error_codes_do_not_abort(gpu):
+    import cugridder_amd as cg
+    L = cg.lib()
+    uvw = np.zeros((4, 3))
+    vis = np.ones(4, np.complex128)
+    d = np.zeros((16, 16), np.complex128)
+    assert L.ms2dirty_1(4, 0, 16, 1.0, 1e9, uvw, vis, d, 1e-5, 2.0, -1) == 2
+    assert L.ms2dirty_1(4, 16, 16, 1.0, 1e9, uvw, vis, d, 1e-5, 1.0, -1) == 2
+    # eps below the FP64 floor: clamped, ier = 1, still computed
+    assert L.ms2dirty_1(4, 16, 16, 1.0, 1e9, uvw, vis, d, 1e-20, 2.0, -1) == 1
+    assert np.isfinite(d).all()
+
+
+def test_single_row_and_tiny_image(gpu):
+    import cugridder_amd as cg
+    k = wst_case(1, 16, 16, 5.0)
+    d = cg.ms2dirty(k["uvw"], [k["f0"]], k["ms"], None, 16, 16, k["xpix"], k["ypix"], 0, 0, 1e-8,
+                    True)
+    ref = O.ms2dirty(k["uvw"], k["f0"], k["ms"], None, 16, 16, 5.0, 1e-8).real
+    assert rel_l2(d, ref) <= 1e-8
