@@ -106,10 +106,12 @@ def lib():
     L.cg_fft2d_dev.argtypes = [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int]
     L.cg_sort_pairs_dev.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int]
     L.cg_memcpy_dev.argtypes = [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]
+    L.cg_fft_col_split.argtypes = [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     for name in ("cg_ctx_create", "cg_ctx_destroy", "cg_ctx_set_option", "cg_ctx_stage_times",
                  "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
                  "cg_nufft2d_dev", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
-                 "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_version"):
+                 "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split",
+                 "cg_version"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

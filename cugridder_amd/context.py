@@ -137,6 +137,17 @@ class Context:
         return d
 
     # ------------------------------------------------------------ test hooks
+    @staticmethod
+    def spectrum_rows(n2):
+        """Column-axis frequency held by each grid row after the forward
+        column FFT (identity unless the axis is split, see cg_fft_col_split)."""
+        na, nb = ctypes.c_int(), ctypes.c_int()
+        split = lib().cg_fft_col_split(int(n2), ctypes.byref(na), ctypes.byref(nb))
+        r = np.arange(n2)
+        if split != 1:
+            return r
+        return r // nb.value + na.value * (r % nb.value)
+
     def grid_view(self, dtype):
         """The workspace w-plane grid of the last call as a torch tensor view
         [nf3, nf2, nf1] (copied)."""

@@ -1,20 +1,45 @@
 // Hand-written batched 2-D FFT of the w-planes (replaces the reference's
 // cuFFT C2C/Z2Z plans, src/fourier/cuft.cu:20-44 and :192-212).
 //
-// Unnormalised transform y[k] = sum_j x[j] exp(dir 2 pi i j k / n), applied
-// along the fast axis (rows) then the slow axis (columns) of each plane.
-// Each workgroup loads whole transforms into LDS (rows: a run of contiguous
-// rows; columns: a block of adjacent columns, so every global access is a
-// contiguous segment), runs a mixed-radix Stockham autosort (radix 8, 4, 2,
-// 3, 5 in any order) in place -- every pass reads its butterflies into
-// registers, barriers, then writes -- and stores the transforms back.
-// Twiddles come from an FP64-computed table e^{+2 pi i t/n}.
+// Unnormalised transform y[k] = sum_j x[j] exp(dir 2 pi i j k / n) along the
+// fast axis (rows, length n1) and the slow axis (columns, length n2) of each
+// [n2][n1] plane.  Every transform runs as a mixed-radix Stockham autosort
+// (radix 8, 4, 2, 3, 5) in LDS: each pass reads its butterflies into
+// registers, barriers, then writes them back in place.
+//
+// Rows: persistent workgroups walk groups of whole rows (one 16384-point
+// fp32 row is 132 KB of padded LDS).  CG_FFT_PREFETCH=1 selects a variant
+// that prefetches the next group into registers while the current one is
+// transformed (the barriers between passes fence LDS only, so the prefetch
+// stays in flight); at 16 elements per thread it spills and measured slower,
+// so it is off by default.
+//
+// Columns: a short column (n2 <= 128, or a length with no usable split) is
+// transformed whole by a workgroup holding a block of adjacent columns.  A
+// long column uses the in-place four-step split n2 = na * nb on blocks of 64
+// (fp32) / 32 (fp64) adjacent columns, i.e. 512-byte row segments: step A
+// transforms the na-point sub-columns of stride nb and applies the twiddles
+// w^(b ka); step B transforms the nb-point contiguous sub-columns.  Without a
+// transposition the spectrum of a split column lands in "split order":
+// frequency k = ka + na kb sits at row ka nb + kb.  The forward (dir > 0)
+// column pass produces that order and the inverse (dir < 0) consumes it; the
+// per-pixel kernels that read or write the spectrum index through it, and
+// launch_fft2d restores natural order for callers that need it.
+//
+// Pruning: only the fast-axis indices k < live.a or k >= live.b carry image
+// pixels.  The column pass transforms only those columns.  A forward row pass
+// stores only those outputs; an inverse row pass reads only those inputs
+// (the others are zero by construction).
+//
+// Twiddles come from FP64-computed tables e^{+2 pi i t/n}; the sub-
+// transforms of a split read the length-n2 table with a stride.
+#include <cstdlib>
+
 #include "cg_internal.hpp"
 
 namespace cg {
 
-// max elements per thread: 16 complex64 or 8 complex128 (128 VGPRs at 1024
-// threads), so one workgroup holds up to 16384 / 8192 elements = 128 KB LDS
+// max elements per thread: 16 complex64 or 8 complex128
 template <typename T>
 constexpr int fft_ept() { return sizeof(T) == 4 ? 16 : 8; }
 
@@ -32,6 +57,15 @@ template <typename T, int DIR>
 __device__ __forceinline__ cx_t<T> cmul_i(cx_t<T> a)
 {
     return DIR > 0 ? mkc(-a.y, a.x) : mkc(a.y, -a.x);
+}
+
+// workgroup barrier that orders LDS only: outstanding global loads (the
+// row prefetch) are not waited for
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <typename T, int R, int DIR>
@@ -97,10 +131,25 @@ __device__ __forceinline__ void dft_small(cx_t<T> *a)
 struct FftShape {
     int n;        // transform length
     int nb;       // transforms per workgroup
-    int lt, le;   // LDS strides: element (b, e) at b*lt + e*le
+    int lt, le;   // LDS position of element e of transform b: b*lt + e*le
+    int bfast;    // 1: consecutive threads take consecutive transforms (nb = 2^nbsh)
+    int nbsh;
+    int pad;      // > 0: one padding slot after every 2^pad LDS elements
+    int tws;      // twiddle table stride (table length n * tws)
     int nrad;
     int rad[16];
 };
+
+// fast-axis indices that carry image pixels: k < a or k >= b
+struct Live {
+    int a, b;
+    __host__ __device__ bool operator()(int k) const { return k < a || k >= b; }
+    __host__ __device__ int count(int n) const { return a + (n - b); }
+    // k-th live index
+    __host__ __device__ int at(int k) const { return k < a ? k : k - a + b; }
+};
+
+__device__ __forceinline__ int lds_pos(const FftShape &S, int x) { return S.pad ? x + (x >> S.pad) : x; }
 
 // one Stockham pass of radix R over all nb transforms held in LDS
 template <typename T, int R, int DIR>
@@ -110,37 +159,51 @@ __device__ __forceinline__ void stockham_pass(cx_t<T> *buf, const FftShape &S, i
     constexpr int K = (fft_ept<T>() + R - 1) / R;
     const int Tn = S.n / R;
     const int nbf = S.nb * Tn;
-    const int twstep = S.n / (p * R);
+    const int twstep = S.n / (p * R) * S.tws;
     cx_t<T> u[K][R];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int q = threadIdx.x + k * blockDim.x;
         if (q < nbf) {
-            const int b = q / Tn, i = q - b * Tn;
+            int b, i;
+            if (S.bfast) {
+                b = q & (S.nb - 1);
+                i = q >> S.nbsh;
+            } else {
+                b = q / Tn;
+                i = q - b * Tn;
+            }
 #pragma unroll
-            for (int r = 0; r < R; r++) u[k][r] = buf[b * S.lt + (i + r * Tn) * S.le];
+            for (int r = 0; r < R; r++) u[k][r] = buf[lds_pos(S, b * S.lt + (i + r * Tn) * S.le)];
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int q = threadIdx.x + k * blockDim.x;
         if (q < nbf) {
-            const int b = q / Tn, i = q - b * Tn;
+            int b, i;
+            if (S.bfast) {
+                b = q & (S.nb - 1);
+                i = q >> S.nbsh;
+            } else {
+                b = q / Tn;
+                i = q - b * Tn;
+            }
             const int kk = i % p;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                cx_t<T> w = tw[r * kk * twstep];
+                cx_t<T> w = tw[(unsigned)(r * kk * twstep)];
                 if (DIR < 0) w.y = -w.y;
                 u[k][r] = cmul<T>(u[k][r], w);
             }
             dft_small<T, R, DIR>(u[k]);
             const int j = (i - kk) * R + kk;
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[b * S.lt + (j + r * p) * S.le] = u[k][r];
+            for (int r = 0; r < R; r++) buf[lds_pos(S, b * S.lt + (j + r * p) * S.le)] = u[k][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 template <typename T, int DIR>
@@ -161,48 +224,147 @@ __device__ void lds_fft(cx_t<T> *buf, const FftShape &S, const cx_t<T> *tw)
     }
 }
 
-// rows: transforms along the contiguous axis; blockIdx.x = row group,
-// blockIdx.y = plane
-template <typename T, int DIR>
-__global__ __launch_bounds__(1024) void k_fft_rows(cx_t<T> *data, int n2, int n1, FftShape S,
-                                                   const cx_t<T> *__restrict__ tw)
+// Rows.  Group g holds rows [g nb, g nb + nb) of the nrows rows of all planes
+// (rows are contiguous across planes).  DIR > 0 stores only live outputs;
+// DIR < 0 reads only live inputs.
+template <typename T, int DIR, bool PF>
+__global__ __launch_bounds__(1024) void k_fft_rows(cx_t<T> *data, int64_t nrows, int n1,
+                                                   FftShape S, const cx_t<T> *__restrict__ tw,
+                                                   Live live)
 {
+    constexpr int EPT = fft_ept<T>();
     extern __shared__ unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
-    cx_t<T> *pl = data + (size_t)blockIdx.y * n2 * n1;
-    const int row0 = blockIdx.x * S.nb;
-    const int nrow = min(S.nb, n2 - row0);
     const int E = S.nb * n1;
-    for (int k = threadIdx.x; k < E; k += blockDim.x) {
-        const int b = k / n1;
-        buf[k] = b < nrow ? pl[(size_t)row0 * n1 + k] : mkc((T)0, (T)0);
+    const int64_t ngroups = (nrows + S.nb - 1) / S.nb;
+    cx_t<T> pre[EPT];
+    // elements of group grp that exist (the last group may be short)
+    auto count = [&](int64_t grp) -> int {
+        return (int)min((int64_t)E, (nrows - grp * S.nb) * n1);
+    };
+    auto load = [&](int64_t grp) {
+        const cx_t<T> *src = data + grp * (int64_t)E;
+        const int lim = count(grp);
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * blockDim.x;
+            const int e = S.nb == 1 ? k : k % n1;
+            pre[u] = (k < lim && (DIR > 0 || live(e))) ? src[k] : mkc((T)0, (T)0);
+        }
+    };
+    int64_t g = blockIdx.x;
+    if (PF && g < ngroups) load(g);
+    for (; g < ngroups; g += gridDim.x) {
+        if (!PF) load(g);
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * blockDim.x;
+            if (k < E) buf[lds_pos(S, k)] = pre[u];
+        }
+        lds_barrier();
+        if (PF && g + gridDim.x < ngroups) load(g + gridDim.x);
+        lds_fft<T, DIR>(buf, S, tw);
+        cx_t<T> *dst = data + g * (int64_t)E;
+        const int lim = count(g);
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * blockDim.x;
+            const int e = S.nb == 1 ? k : k % n1;
+            if (k < lim && (DIR < 0 || live(e))) dst[k] = buf[lds_pos(S, k)];
+        }
+        lds_barrier();
     }
-    __syncthreads();
-    lds_fft<T, DIR>(buf, S, tw);
-    for (int k = threadIdx.x; k < nrow * n1; k += blockDim.x) pl[(size_t)row0 * n1 + k] = buf[k];
 }
 
-// columns: blockIdx.x = column group of nb adjacent columns
+// Columns, whole transform per workgroup: blockIdx.x = block of S.nb live
+// columns, blockIdx.y = plane.  Natural order in and out.
 template <typename T, int DIR>
 __global__ __launch_bounds__(1024) void k_fft_cols(cx_t<T> *data, int n2, int n1, FftShape S,
-                                                   const cx_t<T> *__restrict__ tw)
+                                                   const cx_t<T> *__restrict__ tw, Live live)
 {
     extern __shared__ unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     cx_t<T> *pl = data + (size_t)blockIdx.y * n2 * n1;
+    const int nlive = live.count(n1);
     const int c0 = blockIdx.x * S.nb;
-    const int ncol = min(S.nb, n1 - c0);
     const int E = S.nb * n2;
     for (int k = threadIdx.x; k < E; k += blockDim.x) {
-        const int e = k / S.nb, b = k - e * S.nb;
-        buf[k] = b < ncol ? pl[(size_t)e * n1 + c0 + b] : mkc((T)0, (T)0);
+        const int e = k >> S.nbsh, b = k & (S.nb - 1);
+        const int c = c0 + b;
+        buf[k] = c < nlive ? pl[(size_t)e * n1 + live.at(c)] : mkc((T)0, (T)0);
     }
-    __syncthreads();
+    lds_barrier();
     lds_fft<T, DIR>(buf, S, tw);
     for (int k = threadIdx.x; k < E; k += blockDim.x) {
-        const int e = k / S.nb, b = k - e * S.nb;
-        if (b < ncol) pl[(size_t)e * n1 + c0 + b] = buf[k];
+        const int e = k >> S.nbsh, b = k & (S.nb - 1);
+        const int c = c0 + b;
+        if (c < nlive) pl[(size_t)e * n1 + live.at(c)] = buf[k];
     }
+}
+
+// One step of the in-place split of long columns.  Sub-transform j
+// (blockIdx.y) of length L = S.n reads and writes rows j*sj + e*se, e < L;
+// twmode 1 multiplies output e by w^(j e) after the transform, twmode 2
+// multiplies input e by w^(j e) before it, w = e^{dir 2 pi i/n2} (tw2 is the
+// length-n2 table).  blockIdx.x = block of S.nb live columns, blockIdx.z =
+// plane.
+template <typename T, int DIR>
+__global__ __launch_bounds__(1024) void k_fft_cols_step(cx_t<T> *data, int n2, int n1, FftShape S,
+                                                        int se, int sj, int twmode,
+                                                        const cx_t<T> *__restrict__ tw2, Live live)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
+    const int j = blockIdx.y;
+    cx_t<T> *pl = data + (size_t)blockIdx.z * n2 * n1;
+    const int nlive = live.count(n1);
+    const int c0 = blockIdx.x * S.nb;
+    const int E = S.nb * S.n;
+    const size_t row0 = (size_t)j * sj;
+    for (int k = threadIdx.x; k < E; k += blockDim.x) {
+        const int e = k >> S.nbsh, b = k & (S.nb - 1);
+        const int c = c0 + b;
+        cx_t<T> v = c < nlive ? pl[(row0 + (size_t)e * se) * n1 + live.at(c)] : mkc((T)0, (T)0);
+        if (twmode == 2) {
+            cx_t<T> w = tw2[j * e];
+            if (DIR < 0) w.y = -w.y;
+            v = cmul<T>(v, w);
+        }
+        buf[k] = v;
+    }
+    lds_barrier();
+    lds_fft<T, DIR>(buf, S, tw2);
+    for (int k = threadIdx.x; k < E; k += blockDim.x) {
+        const int e = k >> S.nbsh, b = k & (S.nb - 1);
+        const int c = c0 + b;
+        if (c >= nlive) continue;
+        cx_t<T> v = buf[k];
+        if (twmode == 1) {
+            cx_t<T> w = tw2[j * e];
+            if (DIR < 0) w.y = -w.y;
+            v = cmul<T>(v, w);
+        }
+        pl[(row0 + (size_t)e * se) * n1 + live.at(c)] = v;
+    }
+}
+
+// split order <-> natural order of the column axis, out of place:
+// to_natural: dst[k] = src[pos(k)], else dst[pos(k)] = src[k]
+template <typename T>
+__global__ __launch_bounds__(256) void k_col_permute(const cx_t<T> *__restrict__ src,
+                                                     cx_t<T> *__restrict__ dst, int64_t total,
+                                                     int n2, int n1, int na, int nb, int to_natural)
+{
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (idx >= total) return;
+    const int64_t rowg = idx / n1;
+    const int c = (int)(idx - rowg * n1);
+    const int64_t plane = rowg / n2;
+    const int k = (int)(rowg - plane * n2);
+    const int pos = (k % na) * nb + k / na;
+    const int64_t base = plane * n2;
+    if (to_natural) dst[idx] = src[(base + pos) * n1 + c];
+    else dst[(base + pos) * n1 + c] = src[idx];
 }
 
 static int factor_radices(int n, int *rad)
@@ -216,85 +378,217 @@ static int factor_radices(int n, int *rad)
     return n == 1 ? nr : -1;
 }
 
+static int round64(int x) { return (x + 63) / 64 * 64; }
+static int ilog2(int x)
+{
+    int l = 0;
+    while ((1 << (l + 1)) <= x) l++;
+    return l;
+}
+
+// columns up to this length are transformed whole; longer ones are split
+constexpr int kColWhole = 128;
+constexpr int kColSubMax = 256;
+constexpr int kLdsMax = 163840;
+
+bool fft_col_split(int n, int &na, int &nb)
+{
+    na = nb = 0;
+    if (n <= kColWhole) return false;
+    int best = 0;
+    for (int a = 2; a <= kColSubMax; a++) {
+        if (n % a) continue;
+        const int b = n / a;
+        if (b > kColSubMax) continue;
+        if (best == 0 || std::abs(a - b) < std::abs(best - n / best)) best = a;
+    }
+    if (!best) return false;
+    na = best;
+    nb = n / best;
+    return true;
+}
+
+// tuning switches read from the environment
+static int fft_knob(const char *name, int dflt)
+{
+    const char *e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+}
+
+// rows: row padded in LDS by one slot per 32 elements (bank spread of the
+// strided Stockham writes)
+template <typename T>
+static int row_lds_bytes(int n1, int nb)
+{
+    const int E = nb * n1;
+    return (E + (E >> 5)) * (int)sizeof(cx_t<T>);
+}
+
 template <typename T>
 bool fft_supported(int n)
 {
     int rad[32];
     if (n < 1 || factor_radices(n, rad) < 0) return false;
-    return (int64_t)n * (int64_t)sizeof(cx_t<T>) <= 131072 && n <= 1024 * fft_ept<T>();
+    if (n > 1024 * fft_ept<T>() || row_lds_bytes<T>(n, 1) > kLdsMax) return false;
+    return true;
 }
 
-static int round64(int x) { return (x + 63) / 64 * 64; }
-
 template <typename T, int DIR>
-static int fft_axis(cx_t<T> *data, int batch, int n2, int n1, bool rows, const cx_t<T> *tw,
+static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw, Live live,
                     hipStream_t s)
 {
-    const int n = rows ? n1 : n2;
-    const int other = rows ? n2 : n1;
     FftShape S{};
-    S.n = n;
-    S.nrad = factor_radices(n, S.rad);
+    S.n = n1;
+    S.nrad = factor_radices(n1, S.rad);
     if (S.nrad < 0) return CG_ERR_SIZE;
-    const int elem = (int)sizeof(cx_t<T>);
-    // elements per workgroup: up to 4096 (32 KB fp32) for short transforms,
-    // a whole transform when it is longer
-    int target = std::max(n, 4096);
-    int nb = std::max(1, target / n);
-    nb = std::min(nb, other);
-    while (nb > 1 && (int64_t)nb * n * elem > 65536) nb--;
-    const int E = nb * n;
-    if (E > 1024 * fft_ept<T>() || (int64_t)E * elem > 131072) return CG_ERR_SIZE;
-    int threads = round64((E + fft_ept<T>() - 1) / fft_ept<T>());
-    threads = std::min(std::max(threads, 64), 1024);
+    int nb = std::max(1, 4096 / n1);
+    while (nb > 1 && row_lds_bytes<T>(n1, nb) > 65536) nb--;
+    const int E = nb * n1;
+    const size_t lds = (size_t)row_lds_bytes<T>(n1, nb);
+    if (E > 1024 * fft_ept<T>() || lds > (size_t)kLdsMax) return CG_ERR_SIZE;
+    const int threads = std::min(std::max(round64((E + fft_ept<T>() - 1) / fft_ept<T>()), 64), 1024);
     S.nb = nb;
-    if (rows) {
-        S.lt = n;
-        S.le = 1;
-    } else {
-        S.lt = 1;
-        S.le = nb;
-    }
-    size_t lds = (size_t)E * elem;
-    dim3 grid((unsigned)((other + nb - 1) / nb), (unsigned)batch);
-    if (rows) {
+    S.lt = n1;
+    S.le = 1;
+    S.pad = fft_knob("CG_FFT_PAD", 1) ? 5 : 0;
+    S.tws = 1;
+    const int64_t nrows = (int64_t)n2 * batch;
+    const int64_t ngroups = (nrows + nb - 1) / nb;
+    int per_cu = std::max<int>(1, (int)(kLdsMax / lds));
+    per_cu = std::min(per_cu, std::max(1, 2048 / threads));
+    const int64_t grid = std::min<int64_t>(ngroups, (int64_t)256 * per_cu);
+    if (fft_knob("CG_FFT_PREFETCH", 0)) {
         if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR>,
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR, true>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_fft_rows<T, DIR>), grid, dim3(threads), lds, s, data, n2, n1, S, tw);
+        hipLaunchKernelGGL((k_fft_rows<T, DIR, true>), dim3((unsigned)grid), dim3(threads), lds, s,
+                           data, nrows, n1, S, tw, live);
     } else {
         if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols<T, DIR>,
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_fft_cols<T, DIR>), grid, dim3(threads), lds, s, data, n2, n1, S, tw);
+        hipLaunchKernelGGL((k_fft_rows<T, DIR, false>), dim3((unsigned)grid), dim3(threads), lds, s,
+                           data, nrows, n1, S, tw, live);
     }
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
 
-template <typename T>
-int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T> *tw1,
-                 const cx_t<T> *tw2, hipStream_t s)
+template <typename T, int DIR>
+static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2, Live live,
+                    hipStream_t s)
 {
-    if (batch <= 0) return CG_OK;
+    const int elem = (int)sizeof(cx_t<T>);
+    const int nlive = live.count(n1);
+    if (nlive <= 0 || batch <= 0) return CG_OK;
+    int na, nb;
+    if (!fft_col_split(n2, na, nb)) {
+        FftShape S{};
+        S.n = n2;
+        S.nrad = factor_radices(n2, S.rad);
+        if (S.nrad < 0) return CG_ERR_SIZE;
+        // power-of-two block of adjacent columns, up to 64 KB of LDS
+        int cb = 1;
+        while (cb * 2 <= 64 && (int64_t)cb * 2 * n2 * elem <= 65536 && cb * 2 <= nlive) cb *= 2;
+        const int E = cb * n2;
+        const size_t lds = (size_t)E * elem;
+        if (E > 1024 * fft_ept<T>() || lds > (size_t)kLdsMax) return CG_ERR_SIZE;
+        const int threads = std::min(std::max(round64((E + fft_ept<T>() - 1) / fft_ept<T>()), 64), 1024);
+        S.nb = cb;
+        S.nbsh = ilog2(cb);
+        S.bfast = 1;
+        S.lt = 1;
+        S.le = cb;
+        S.tws = 1;
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols<T, DIR>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3((unsigned)((nlive + cb - 1) / cb), (unsigned)batch),
+                           dim3(threads), lds, s, data, n2, n1, S, tw2, live);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    }
+    auto step = [&](int L, int se, int sj, int nsub, int twmode) -> int {
+        FftShape S{};
+        S.n = L;
+        S.nrad = factor_radices(L, S.rad);
+        if (S.nrad < 0) return CG_ERR_SIZE;
+        // 512-byte row segments, at most 64 KB of LDS
+        int cb = 512 / elem;
+        while (cb > 1 && (int64_t)cb * L * elem > 65536) cb /= 2;
+        const int E = cb * L;
+        const int threads = std::min(std::max(round64((E + fft_ept<T>() - 1) / fft_ept<T>()), 64), 1024);
+        if ((int64_t)threads * fft_ept<T>() < E) return CG_ERR_SIZE;
+        S.nb = cb;
+        S.nbsh = ilog2(cb);
+        S.bfast = 1;
+        S.lt = 1;
+        S.le = cb;
+        S.tws = n2 / L;
+        dim3 grid((unsigned)((nlive + cb - 1) / cb), (unsigned)nsub, (unsigned)batch);
+        hipLaunchKernelGGL((k_fft_cols_step<T, DIR>), grid, dim3(threads), (size_t)E * elem, s,
+                           data, n2, n1, S, se, sj, twmode, tw2, live);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    };
     int r;
-    if (dir > 0) {
-        if ((r = fft_axis<T, 1>(data, batch, n2, n1, true, tw1, s))) return r;
-        if (n2 > 1 && (r = fft_axis<T, 1>(data, batch, n2, n1, false, tw2, s))) return r;
+    if (DIR > 0) {
+        // x[a nb + b] at row a nb + b -> Y_b[ka] w^(b ka) at row ka nb + b
+        if ((r = step(na, nb, 1, nb, 1))) return r;
+        // Y over b -> X[ka + na kb] at row ka nb + kb
+        if ((r = step(nb, 1, nb, na, 0))) return r;
     } else {
-        if ((r = fft_axis<T, -1>(data, batch, n2, n1, true, tw1, s))) return r;
-        if (n2 > 1 && (r = fft_axis<T, -1>(data, batch, n2, n1, false, tw2, s))) return r;
+        // X[ka + na kb] at row ka nb + kb -> Z_ka[b] at row ka nb + b
+        if ((r = step(nb, 1, nb, na, 0))) return r;
+        // Z_ka[b] w^(b ka) over ka -> x[a nb + b] at row a nb + b
+        if ((r = step(na, nb, 1, nb, 2))) return r;
     }
     return CG_OK;
 }
 
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
-                    const cx_t<T> *tw, hipStream_t s)
+                    const cx_t<T> *tw, int live_a, int live_b, hipStream_t s)
 {
-    if (batch <= 0 || (!rows && n2 <= 1)) return CG_OK;
-    return dir > 0 ? fft_axis<T, 1>(data, batch, n2, n1, rows, tw, s)
-                   : fft_axis<T, -1>(data, batch, n2, n1, rows, tw, s);
+    if (batch <= 0) return CG_OK;
+    Live live{live_a, live_b};
+    if (rows) return dir > 0 ? fft_rows<T, 1>(data, batch, n2, n1, tw, live, s)
+                             : fft_rows<T, -1>(data, batch, n2, n1, tw, live, s);
+    if (n2 <= 1) return CG_OK;
+    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, s)
+                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, s);
+}
+
+template <typename T>
+static int col_permute(cx_t<T> *data, cx_t<T> *tmp, int batch, int n2, int n1, int na, int nb,
+                       int to_natural, hipStream_t s)
+{
+    const int64_t total = (int64_t)batch * n2 * n1;
+    hipLaunchKernelGGL(k_col_permute<T>, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, data,
+                       tmp, total, n2, n1, na, nb, to_natural);
+    CG_HIP(hipGetLastError());
+    CG_HIP(hipMemcpyAsync(data, tmp, sizeof(cx_t<T>) * (size_t)total, hipMemcpyDeviceToDevice, s));
+    return CG_OK;
+}
+
+template <typename T>
+int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T> *tw1,
+                 const cx_t<T> *tw2, cx_t<T> *tmp, hipStream_t s)
+{
+    if (batch <= 0) return CG_OK;
+    int na = 0, nb = 0;
+    const bool split = n2 > 1 && fft_col_split(n2, na, nb);
+    if (split && !tmp) return CG_ERR_ARG;
+    int r;
+    if (dir > 0) {
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, s))) return r;
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, s))) return r;
+        if (split) return col_permute<T>(data, tmp, batch, n2, n1, na, nb, 1, s);
+        return CG_OK;
+    }
+    if (split && (r = col_permute<T>(data, tmp, batch, n2, n1, na, nb, 0, s))) return r;
+    if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, s))) return r;
+    return launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, s);
 }
 
 template <typename T>
@@ -315,10 +609,10 @@ int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s)
     return CG_OK;
 }
 
-template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, hipStream_t);
-template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, hipStream_t);
-template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, hipStream_t);
-template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, hipStream_t);
+template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, float2 *, hipStream_t);
+template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, double2 *, hipStream_t);
+template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, hipStream_t);
+template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, hipStream_t);
 template bool fft_supported<float>(int);
 template bool fft_supported<double>(int);
 template int launch_twiddles<float>(float2 *, int, hipStream_t);

@@ -82,15 +82,23 @@ int grid_lds_bytes(const GridArgs &g, int elem);
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
 
 // ---- FFT (cg_fft.hip): in-place unnormalised 2-D FFT of `batch` planes
-// [batch][n2][n1], exponent sign dir (+1 / -1).  tw1/tw2 are device tables
-// of e^{+2 pi i t/n} (length n1 / n2) in precision T.
+// [batch][n2][n1], exponent sign dir (+1 / -1), natural order in and out.
+// tw1/tw2 are device tables of e^{+2 pi i t/n} (length n1 / n2) in
+// precision T; tmp (batch*n2*n1 elements) is needed when n2 is split.
 template <typename T>
 int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T> *tw1,
-                 const cx_t<T> *tw2, hipStream_t s);
-// one pass of launch_fft2d: rows (fast axis) or columns
+                 const cx_t<T> *tw2, cx_t<T> *tmp, hipStream_t s);
+// One pass: rows (fast axis) or columns.  Only fast-axis indices k < live_a
+// or k >= live_b matter: the column pass transforms only those columns, a
+// dir > 0 row pass stores only those outputs and a dir < 0 row pass reads
+// only those inputs.  A dir > 0 column pass leaves a split column axis in
+// split order (frequency ka + na kb at row ka nb + kb, see fft_col_split);
+// a dir < 0 column pass expects that order.
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
-                    const cx_t<T> *tw, hipStream_t s);
+                    const cx_t<T> *tw, int live_a, int live_b, hipStream_t s);
+// n = na * nb when columns of length n are transformed by the split
+bool fft_col_split(int n, int &na, int &nb);
 template <typename T>
 bool fft_supported(int n);
 template <typename T>
@@ -109,6 +117,7 @@ int launch_fseries_grid(const Quad &q, int nf, double *phi, hipStream_t s);
 
 struct WArgs {
     int N1, N2, nf1, nf2, nf3;
+    int cna, cnb;  // split of the column axis (0: natural order)
     double pix_x, pix_y, o_center, i_center, gamma, h;
     Quad qd;
 };
@@ -121,7 +130,7 @@ int launch_ms2dirty_epilogue(const cx_t<T> *fw, const double *zq, const double *
                              const double *phi1, const double *phi2, TV *dirty,
                              const WArgs &w, hipStream_t s);
 // dirty2ms prologue: 1/n, w-correction, deconvolution and w-IDFT onto planes
-// (writes every grid cell, zero outside the image footprint)
+// (writes every cell of the live columns, zero outside the image footprint)
 template <typename T, typename TV>
 int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *phi3,
                              const double *phi1, const double *phi2, cx_t<T> *fw,

@@ -8,8 +8,10 @@
 //   div_n_lm        (src/astro/ra_exec.cu:23-37):   / n
 // and the dirty2ms prologue fuses their adjoints (div_n_lm, w_term_deconv
 // with flag -1, deconv_2d type 2, w_term_idft cuft.cu:647-689) and writes
-// every cell of every plane (zero outside the image footprint), so the grid
-// needs no separate memset.
+// every cell of the live columns of every plane (zero outside the image
+// footprint); the other columns are never read by the inverse FFT, so the
+// grid needs no memset.  Both index the spectrum's column axis in the order
+// the column FFT leaves it (split order for long columns).
 #include "cg_internal.hpp"
 
 namespace cg {
@@ -29,6 +31,7 @@ WArgs make_wargs(const WPlan &p)
     w.gamma = p.gamma;
     w.h = p.h;
     w.qd = make_quad(p.conv);
+    if (!(p.nf2 > 1 && fft_col_split(p.nf2, w.cna, w.cnb))) w.cna = w.cnb = 0;
     return w;
 }
 
@@ -90,6 +93,27 @@ template <typename TV> __device__ __forceinline__ TV from_d2(double2 v);
 template <> __device__ __forceinline__ float2 from_d2<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 template <> __device__ __forceinline__ double2 from_d2<double2>(double2 v) { return v; }
 
+// Spectrum row holding column-axis frequency k, and the inverse map: the
+// column FFT leaves a split column (na > 0) with frequency ka + na kb at row
+// ka nb + kb.
+__device__ __forceinline__ int spec_row(int k, int na, int nb) { return na ? (k % na) * nb + k / na : k; }
+__device__ __forceinline__ int spec_freq(int r, int na, int nb) { return na ? r / nb + na * (r % nb) : r; }
+
+// live column cc in [0, N) of the fast axis -> grid column w1 and image
+// column: the first N - N/2 live columns are w1 = cc (image col cc + N/2),
+// the rest sit at the top of the axis (image col cc - (N - N/2))
+__device__ __forceinline__ void live_col(int cc, int N, int nf, int &w1, int &col)
+{
+    const int A = N - N / 2;
+    if (cc < A) {
+        w1 = cc;
+        col = cc + N / 2;
+    } else {
+        w1 = cc - A + nf - N / 2;
+        col = cc - A;
+    }
+}
+
 __device__ __forceinline__ double2 zmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -121,7 +145,7 @@ __global__ __launch_bounds__(256) void k_ms2dirty_epi(const cx_t<T> *__restrict_
     const int w2 = dr >= 0 ? dr : w.nf2 + dr;
     const int q = abs(dc) + abs(dr) * (w.N1 / 2 + 1);
     const size_t plane = (size_t)w.nf1 * w.nf2;
-    const cx_t<T> *src = fw + (size_t)w2 * w.nf1 + w1;
+    const cx_t<T> *src = fw + (size_t)spec_row(w2, w.cna, w.cnb) * w.nf1 + w1;
     const double zp = zq[q];
     // e^{i z' (p - P/2)}, FP64 recurrence
     double sv, cv;
@@ -174,14 +198,15 @@ __global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dir
                                                       const double *__restrict__ phi2,
                                                       cx_t<T> *__restrict__ fw, WArgs w)
 {
-    const int64_t cell = blockIdx.x * (int64_t)256 + threadIdx.x;
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
     const size_t plane = (size_t)w.nf1 * w.nf2;
-    if (cell >= (int64_t)plane) return;
-    const int w2 = (int)(cell / w.nf1), w1 = (int)(cell - (int64_t)w2 * w.nf1);
-    const int col = cell_to_img(w1, w.N1, w.nf1);
-    const int row = cell_to_img(w2, w.N2, w.nf2);
-    cx_t<T> *dst = fw + cell;
-    if (col < 0 || row < 0) {
+    if (idx >= (int64_t)w.nf2 * w.N1) return;
+    const int r = (int)(idx / w.N1), cc = (int)(idx - (int64_t)r * w.N1);
+    int w1, col;
+    live_col(cc, w.N1, w.nf1, w1, col);
+    const int row = cell_to_img(spec_freq(r, w.cna, w.cnb), w.N2, w.nf2);
+    cx_t<T> *dst = fw + (size_t)r * w.nf1 + w1;
+    if (row < 0) {
         for (int p = 0; p < w.nf3; p++) dst[plane * p] = mkc((T)0, (T)0);
         return;
     }
@@ -212,7 +237,7 @@ int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *ph
                              const double *phi1, const double *phi2, cx_t<T> *fw, const WArgs &w,
                              hipStream_t s)
 {
-    int64_t n = (int64_t)w.nf1 * w.nf2;
+    int64_t n = (int64_t)w.nf2 * w.N1;
     hipLaunchKernelGGL((k_dirty2ms_pro<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
                        dirty, zq, phi3, phi1, phi2, fw, w);
     CG_HIP(hipGetLastError());
@@ -220,18 +245,19 @@ int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *ph
 }
 
 // 2-D NUFFT deconvolution, deconv_2d (src/fourier/deconv.cu:250-294), CMCL
-// mode order.  Type 2 writes every grid cell (zero outside the modes).
+// mode order.  Type 2 writes every cell of the live columns (zero outside
+// the modes); the inverse row FFT never reads the other columns.
 template <typename T, typename TV>
 __global__ __launch_bounds__(256) void k_deconv2d_t1(const cx_t<T> *fw, TV *fk, const double *phi1,
                                                      const double *phi2, int N1, int N2, int nf1,
-                                                     int nf2)
+                                                     int nf2, int na, int nb)
 {
     const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (idx >= (int64_t)N1 * N2) return;
     const int k2 = (int)(idx / N1), k1 = (int)(idx - (int64_t)k2 * N1);
     const int d1 = k1 - N1 / 2, d2 = k2 - N2 / 2;
     const int w1 = d1 >= 0 ? d1 : nf1 + d1, w2 = d2 >= 0 ? d2 : nf2 + d2;
-    const double2 f = to_d2(fw[(size_t)w2 * nf1 + w1]);
+    const double2 f = to_d2(fw[(size_t)spec_row(w2, na, nb) * nf1 + w1]);
     const double s = 1.0 / (phi1[abs(d1)] * phi2[abs(d2)]);
     fk[idx] = from_d2<TV>(make_double2(f.x * s, f.y * s));
 }
@@ -239,13 +265,16 @@ __global__ __launch_bounds__(256) void k_deconv2d_t1(const cx_t<T> *fw, TV *fk, 
 template <typename T, typename TV>
 __global__ __launch_bounds__(256) void k_deconv2d_t2(cx_t<T> *fw, const TV *fk, const double *phi1,
                                                      const double *phi2, int N1, int N2, int nf1,
-                                                     int nf2)
+                                                     int nf2, int na, int nb)
 {
-    const int64_t cell = blockIdx.x * (int64_t)256 + threadIdx.x;
-    if (cell >= (int64_t)nf1 * nf2) return;
-    const int w2 = (int)(cell / nf1), w1 = (int)(cell - (int64_t)w2 * nf1);
-    const int k1 = cell_to_img(w1, N1, nf1), k2 = cell_to_img(w2, N2, nf2);
-    if (k1 < 0 || k2 < 0) {
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (idx >= (int64_t)nf2 * N1) return;
+    const int r = (int)(idx / N1), cc = (int)(idx - (int64_t)r * N1);
+    int w1, k1;
+    live_col(cc, N1, nf1, w1, k1);
+    const int k2 = cell_to_img(spec_freq(r, na, nb), N2, nf2);
+    const size_t cell = (size_t)r * nf1 + w1;
+    if (k2 < 0) {
         fw[cell] = mkc((T)0, (T)0);
         return;
     }
@@ -259,14 +288,16 @@ template <typename T, typename TV>
 int launch_deconv2d(cx_t<T> *fw, TV *fk, const double *phi1, const double *phi2, int N1, int N2,
                     int nf1, int nf2, int type, hipStream_t s)
 {
+    int na = 0, nb = 0;
+    if (!(nf2 > 1 && fft_col_split(nf2, na, nb))) na = nb = 0;
     if (type == 1) {
         int64_t n = (int64_t)N1 * N2;
         hipLaunchKernelGGL((k_deconv2d_t1<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
-                           fw, fk, phi1, phi2, N1, N2, nf1, nf2);
+                           fw, fk, phi1, phi2, N1, N2, nf1, nf2, na, nb);
     } else {
-        int64_t n = (int64_t)nf1 * nf2;
+        int64_t n = (int64_t)nf2 * N1;
         hipLaunchKernelGGL((k_deconv2d_t2<T, TV>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s,
-                           fw, fk, phi1, phi2, N1, N2, nf1, nf2);
+                           fw, fk, phi1, phi2, N1, N2, nf1, nf2, na, nb);
     }
     CG_HIP(hipGetLastError());
     return CG_OK;

@@ -142,6 +142,12 @@ int cg_plan_query(int nrow, int nxdirty, int nydirty, double fov, double freq,
  * with n[0] = rows, n[1] = tiles.  Option "stop_after" (1: after gridding or
  * the dirty2ms prologue, 2: after the plane FFTs) stops a call early. */
 int cg_ctx_grid(void *ctx, void **ptr, long long *cells);
+/* Order of the spectrum's column axis in the grid after the forward column
+ * FFT: returns 1 and n = na * nb when frequency ka + na * kb is stored at
+ * row ka * nb + kb ("split order"), 0 when rows are in natural order.  The
+ * dirty2ms prologue writes the same order, and only the fast-axis columns
+ * holding image pixels. */
+int cg_fft_col_split(int n, int *na, int *nb);
 int cg_ctx_sorted(void *ctx, void **keys, void **perm, void **tstart,
                   long long *n);
 /* In-place unnormalised 2-D FFT of batch planes [batch][n2][n1], exponent

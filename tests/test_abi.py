@@ -36,9 +36,31 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_declares_argtypes():
     L = cugridder_amd.lib()
-    assert L.cg_version() == 100
+    assert L.cg_version() == 101
     for n in declared_functions():
         assert getattr(L, n).restype is ctypes.c_int
+
+
+@pytest.mark.parametrize("n", [1, 2, 96, 128, 160, 200, 320, 1250, 2000, 4096, 6250, 13122, 16384])
+def test_fft_col_split(n):
+    """Columns longer than 128 are split n = na * nb (both <= 256); the
+    spectrum row map is then a permutation with frequency ka + na kb at row
+    ka nb + kb."""
+    from cugridder_amd import Context
+    L = cugridder_amd.lib()
+    na, nb = ctypes.c_int(), ctypes.c_int()
+    r = L.cg_fft_col_split(n, ctypes.byref(na), ctypes.byref(nb))
+    rows = Context.spectrum_rows(n)
+    assert sorted(rows.tolist()) == list(range(n))
+    if n <= 128:
+        assert r == 0 and np.array_equal(rows, np.arange(n))
+        return
+    assert r == 1
+    a, b = na.value, nb.value
+    assert a * b == n and 2 <= a <= 256 and 2 <= b <= 256
+    for ka, kb in ((0, 0), (a - 1, 0), (0, b - 1), (a - 1, b - 1), (a // 2, b // 3)):
+        assert rows[ka * b + kb] == ka + a * kb
+    assert L.cg_fft_col_split(0, ctypes.byref(na), ctypes.byref(nb)) == 2
 
 
 @pytest.mark.parametrize("nrow,nx,ny,fov,eps,freq", [
