@@ -190,6 +190,9 @@ def main():
     ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d"), default="wstack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nrow", type=int, default=1 << 27, help="rows of the wstack workload")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 1 GiB device-to-device copy before the steps, so that a "
+                         "counter run can scale FETCH_SIZE/WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has one MI355X and the "
@@ -208,6 +211,13 @@ def main():
     else:
         W = make_nufft2d(torch, dev)
     torch.cuda.synchronize()
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 1 GiB read + 1 GiB written")
     ctx = Context(timing=True)
 
     if args.workload == "nufft2d":

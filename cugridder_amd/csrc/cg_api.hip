@@ -37,7 +37,8 @@ enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_FFT2, ST_IMAG
 
 struct Ctx {
     hipStream_t stream = nullptr;
-    int lds_budget = 65536;
+    int lds_budget = 65536;       // degridding tile + halo staging
+    int grid_lds_budget = 65536;  // gridding accumulators
     int tile_grid = 8;
     int tile_degrid = 32;
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
@@ -89,16 +90,11 @@ static X *as(DevBuf &b)
     return reinterpret_cast<X *>(b.p);
 }
 
-static int key_bits_for(int64_t ntiles)
-{
-    int b = 1;
-    while (((int64_t)1 << b) < ntiles) b++;
-    return b;
-}
 
 static void env_overrides(Ctx &C)
 {
     if (const char *e = std::getenv("CG_LDS_BUDGET")) C.lds_budget = std::atoi(e);
+    if (const char *e = std::getenv("CG_GRID_LDS")) C.grid_lds_budget = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_GRID")) C.tile_grid = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
 }
@@ -172,7 +168,7 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
     CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
     CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                             as<int32_t>(C.vals2), M, key_bits_for(ntiles), as<int32_t>(C.hist),
+                             as<int32_t>(C.vals2), M, sort_key_bits(ntiles), as<int32_t>(C.hist),
                              as<int32_t>(C.scan), C.stream));
     CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart), C.stream));
     return CG_OK;
@@ -227,7 +223,7 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     if (ier >= 2) return ier;
     const int eb = (int)sizeof(cx_t<T>);
     // gridding accumulates in FP64 (16 bytes per cell) whatever the precision
-    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, C.lds_budget, C.tile_grid, 0);
+    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, C.grid_lds_budget, C.tile_grid, 0);
     C.last_plan = P;
     C.last_tiling = t;
     C.last_m = nrow;
@@ -380,7 +376,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = conv.kw;
-    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.lds_budget, C.tile_grid, 0)
+    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.grid_lds_budget, C.tile_grid, 0)
                          : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
     C.last_plan = P;
     C.last_tiling = t;
@@ -389,10 +385,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     a.nf2 = P.nf2;
     a.nf3 = 1;
     a.kw = kw;
-    a.tx = t.tx;
-    a.ty = t.ty;
-    a.ntx = t.ntx;
-    a.nty = t.nty;
+    set_coord_tiling(a, t);
     a.pirange = 1;
     a.has_w = 0;
     const int64_t ntiles = t.ntiles();
@@ -408,9 +401,10 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY(ensure(C.scan, sizeof(int32_t) * scan_tmp_size(hs)));
         CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
         CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                                 as<int32_t>(C.vals2), M, key_bits_for(ntiles), as<int32_t>(C.hist),
-                                 as<int32_t>(C.scan), C.stream));
-        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart), C.stream));
+                                 as<int32_t>(C.vals2), M, sort_key_bits(ntiles),
+                                 as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream));
+        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart),
+                                  C.stream));
     } else {
         CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (ntiles + 1), C.stream));
     }
@@ -705,6 +699,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     if (!C || !name) return CG_ERR_ARG;
     if (!std::strcmp(name, "timing")) C->timing = value != 0;
     else if (!std::strcmp(name, "lds_budget")) C->lds_budget = value;
+    else if (!std::strcmp(name, "grid_lds_budget")) C->grid_lds_budget = value;
     else if (!std::strcmp(name, "tile_grid")) C->tile_grid = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;

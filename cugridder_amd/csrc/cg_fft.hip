@@ -233,7 +233,7 @@ __global__ __launch_bounds__(1024) void k_fft_rows(cx_t<T> *data, int64_t nrows,
                                                    Live live)
 {
     constexpr int EPT = fft_ept<T>();
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     const int E = S.nb * n1;
     const int64_t ngroups = (nrows + S.nb - 1) / S.nb;
@@ -282,7 +282,7 @@ template <typename T, int DIR>
 __global__ __launch_bounds__(1024) void k_fft_cols(cx_t<T> *data, int n2, int n1, FftShape S,
                                                    const cx_t<T> *__restrict__ tw, Live live)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     cx_t<T> *pl = data + (size_t)blockIdx.y * n2 * n1;
     const int nlive = live.count(n1);
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(1024) void k_fft_cols_step(cx_t<T> *data, int n2, i
                                                         int se, int sj, int twmode,
                                                         const cx_t<T> *__restrict__ tw2, Live live)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     const int j = blockIdx.y;
     cx_t<T> *pl = data + (size_t)blockIdx.z * n2 * n1;

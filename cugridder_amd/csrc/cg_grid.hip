@@ -7,7 +7,9 @@
 // |d| < kw/2 (conv.cu:32-36, unnormalised as in the reference).
 //
 // MI355X design: one workgroup owns a tx*ty tile of grid cells for a chunk of
-// w-planes and accumulates it in LDS with FP64 ds_add atomics (on gfx950 the
+// w-planes and accumulates it in LDS (real parts, then imaginary parts, so
+// the lanes' random cells spread over twice as many banks as interleaved
+// pairs would) with FP64 ds_add atomics (on gfx950 the
 // FP32 LDS atomic add runs ~20x slower than the FP64 one, and FP64 also makes
 // the accumulation order-insensitive at FP32 output precision); it reads the points
 // of the bins whose footprints reach the tile (points are binned by the tile
@@ -49,14 +51,43 @@ int degrid_lds_bytes(const GridArgs &g, int kw, int elem)
 
 // ------------------------------------------------------------ gridding
 
-template <typename T, int KW>
-__global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
-                                              const cx_t<T> *__restrict__ val,
-                                              const int32_t *__restrict__ tile_start,
-                                              cx_t<T> *__restrict__ fw, GridArgs g, int is2d)
+// Footprint of a point relative to a tile (or plane chunk) [0, t) of an axis
+// of length n, given its start r relative to the tile origin (in [0, n)):
+// first tap position s (negative when the footprint starts in an earlier
+// tile) and the taps [lo, hi) that land inside the tile.  With one tile
+// along the axis (t == n) the footprint wraps inside it.
+struct Span {
+    int s, lo, hi;
+};
+template <int KW>
+__device__ __forceinline__ Span tile_span(int r, int t, int n)
 {
-    extern __shared__ unsigned char smem_raw[];
+    Span sp;
+    if (t == n) {
+        sp.s = r;
+        sp.lo = 0;
+        sp.hi = KW;
+    } else {
+        sp.s = r < t ? r : r - n;
+        sp.lo = max(0, -sp.s);
+        sp.hi = min(KW, t - sp.s);
+    }
+    return sp;
+}
+
+// most source bins feeding one tile: (1 + ceil(15/2))^2 for tiles >= 2
+constexpr int kMaxBins = 81;
+
+template <typename T, int KW>
+__global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
+                                               const cx_t<T> *__restrict__ val,
+                                               const int32_t *__restrict__ tile_start,
+                                               cx_t<T> *__restrict__ fw, GridArgs g, int is2d)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     double *acc = reinterpret_cast<double *>(smem_raw);
+    __shared__ int s_beg[kMaxBins], s_cnt[kMaxBins], s_ox[kMaxBins], s_oy[kMaxBins];
+    __shared__ int s_total;
     const int chunk = blockIdx.x % g.nchunk;
     const int tile = blockIdx.x / g.nchunk;
     const int bx = tile % g.ntx, by = tile / g.ntx;
@@ -64,64 +95,83 @@ __global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
     const int np = min(g.pc, g.nf3 - p0);
     const int cells = np * g.ty * g.tx;
     for (int k = threadIdx.x; k < 2 * cells; k += blockDim.x) acc[k] = 0.0;
+    // source bins whose footprints can reach this tile: bin (bx - sx, by - sy)
+    const int nbins = g.sbx * g.sby;
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int b = 0; b < nbins; b++) {
+            const int sy = b / g.sbx, sx = b - sy * g.sbx;
+            int sby_ = by - sy, sbx_ = bx - sx;
+            if (sby_ < 0) sby_ += g.nty;
+            if (sbx_ < 0) sbx_ += g.ntx;
+            int oy = (sby_ - by) * g.ty, ox = (sbx_ - bx) * g.tx;
+            if (oy < 0) oy += g.nf2;
+            if (ox < 0) ox += g.nf1;
+            const int sidx = sby_ * g.ntx + sbx_;
+            s_beg[b] = tile_start[sidx];
+            s_cnt[b] = tile_start[sidx + 1] - s_beg[b];
+            s_ox[b] = ox;
+            s_oy[b] = oy;
+            tot += s_cnt[b];
+        }
+        s_total = tot;
+    }
     __syncthreads();
 
     const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
-    const int nzt = is2d ? 1 : KW;
-    for (int sy = 0; sy < g.sby; sy++) {
-        int sby_ = by - sy;
-        if (sby_ < 0) sby_ += g.nty;
-        int oy = (sby_ - by) * g.ty;
-        if (oy < 0) oy += g.nf2;
-        for (int sx = 0; sx < g.sbx; sx++) {
-            int sbx_ = bx - sx;
-            if (sbx_ < 0) sbx_ += g.ntx;
-            int ox = (sbx_ - bx) * g.tx;
-            if (ox < 0) ox += g.nf1;
-            const int s = sby_ * g.ntx + sbx_;
-            const int beg = tile_start[s], end = tile_start[s + 1];
-            for (int i = beg + (int)threadIdx.x; i < end; i += blockDim.x) {
-                const geo_t<T> G = geo[i];
-                const uint32_t pk = geo_bits(G.w);
-                int rx = ox + (int)(pk & 0xffu);
-                if (rx >= g.nf1) rx -= g.nf1;
-                int ry = oy + (int)((pk >> 8) & 0xffu);
-                if (ry >= g.nf2) ry -= g.nf2;
-                if (rx >= g.tx && rx + KW - 1 < g.nf1) continue;
-                if (ry >= g.ty && ry + KW - 1 < g.nf2) continue;
-                int rz = (int)(pk >> 16) - p0;
-                if (rz < 0) rz += g.nf3;
-                if (!is2d && rz >= np && rz + KW - 1 < g.nf3) continue;
-                const cx_t<T> v = val[i];
-                T kx[KW], ky[KW], kz[KW];
+    const int total = s_total;
+    // all points of all source bins as one list, so every thread has work;
+    // points are sorted by overlap class inside a bin, so the lanes of a
+    // wavefront mostly share their tap ranges and skip the same taps
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        int b = 0, j = i;
+        while (b < nbins - 1 && j >= s_cnt[b]) {
+            j -= s_cnt[b];
+            b++;
+        }
+        const int idx = s_beg[b] + j;
+        const geo_t<T> G = geo[idx];
+        const uint32_t pk = geo_bits(G.w);
+        int rx = s_ox[b] + (int)(pk & 0xffu);
+        if (rx >= g.nf1) rx -= g.nf1;
+        int ry = s_oy[b] + (int)((pk >> 8) & 0xffu);
+        if (ry >= g.nf2) ry -= g.nf2;
+        const Span X = tile_span<KW>(rx, g.tx, g.nf1);
+        const Span Y = tile_span<KW>(ry, g.ty, g.nf2);
+        if (X.lo >= X.hi || Y.lo >= Y.hi) continue;
+        Span Z{0, 0, 1};
+        if (!is2d) {
+            int rz = (int)(pk >> 16) - p0;
+            if (rz < 0) rz += g.nf3;
+            Z = tile_span<KW>(rz, np, g.nf3);
+            if (Z.lo >= Z.hi) continue;
+        }
+        const cx_t<T> v = val[idx];
+        T kx[KW], ky[KW];
 #pragma unroll
-                for (int t = 0; t < KW; t++) {
-                    kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
-                    ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
-                    kz[t] = is2d ? (T)(t == 0) : es_val<T>(G.z + (T)t, beta, ec, hw);
-                }
-                if (is2d) rz = 0;
-                for (int tz = 0; tz < nzt; tz++) {
-                    int zz = rz + tz;
-                    if (zz >= g.nf3) zz -= g.nf3;
-                    if (zz >= np) continue;
-                    const T vzr = v.x * kz[tz], vzi = v.y * kz[tz];
+        for (int t = 0; t < KW; t++) {
+            kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
+            ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
+        }
+        for (int tz = Z.lo; tz < Z.hi; tz++) {
+            int zz = Z.s + tz;
+            if (zz >= np) zz -= np;  // wraps only when one chunk holds every plane
+            const T kz = is2d ? (T)1 : es_val<T>(G.z + (T)tz, beta, ec, hw);
+            const T vzr = v.x * kz, vzi = v.y * kz;
 #pragma unroll
-                    for (int t2 = 0; t2 < KW; t2++) {
-                        int yy = ry + t2;
-                        if (yy >= g.nf2) yy -= g.nf2;
-                        if (yy >= g.ty) continue;
-                        const T vyr = vzr * ky[t2], vyi = vzi * ky[t2];
-                        double *row = acc + 2 * ((zz * g.ty + yy) * g.tx);
+            for (int t2 = 0; t2 < KW; t2++) {
+                if (t2 < Y.lo || t2 >= Y.hi) continue;
+                int yy = Y.s + t2;
+                if (yy >= g.nf2) yy -= g.nf2;  // wraps only with one tile along nf2
+                const T vyr = vzr * ky[t2], vyi = vzi * ky[t2];
+                double *row = acc + (zz * g.ty + yy) * g.tx;
 #pragma unroll
-                        for (int t1 = 0; t1 < KW; t1++) {
-                            int xx = rx + t1;
-                            if (xx >= g.nf1) xx -= g.nf1;
-                            if (xx >= g.tx) continue;
-                            atomicAdd(row + 2 * xx, (double)(vyr * kx[t1]));
-                            atomicAdd(row + 2 * xx + 1, (double)(vyi * kx[t1]));
-                        }
-                    }
+                for (int t1 = 0; t1 < KW; t1++) {
+                    if (t1 < X.lo || t1 >= X.hi) continue;
+                    int xx = X.s + t1;
+                    if (xx >= g.nf1) xx -= g.nf1;
+                    atomicAdd(row + xx, (double)(vyr * kx[t1]));
+                    atomicAdd(row + cells + xx, (double)(vyi * kx[t1]));
                 }
             }
         }
@@ -133,21 +183,33 @@ __global__ __launch_bounds__(256) void k_grid(const geo_t<T> *__restrict__ geo,
         int rem = k - lz * plane_cells;
         int y = rem / g.tx, x = rem - y * g.tx;
         size_t gi = ((size_t)(p0 + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
-        fw[gi] = mkc((T)acc[2 * k], (T)acc[2 * k + 1]);
+        fw[gi] = mkc((T)acc[k], (T)acc[cells + k]);
     }
+}
+
+// threads per gridding workgroup: enough waves per CU when the tile's LDS
+// leaves room for few workgroups
+static int grid_threads(size_t lds)
+{
+    const int per_cu = std::max<int>(1, (int)(163840 / std::max<size_t>(lds, 1)));
+    if (per_cu >= 4) return 256;
+    if (per_cu >= 2) return 512;
+    return 1024;
 }
 
 template <typename T, int KW>
 static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, cx_t<T> *fw,
                    const GridArgs &g, hipStream_t s)
 {
+    if (g.sbx * g.sby > kMaxBins) return CG_ERR_SIZE;
     int is2d = g.nf3 == 1 ? 1 : 0;
     size_t lds = (size_t)grid_lds_bytes(g, (int)sizeof(cx_t<T>));
     unsigned nblk = (unsigned)((int64_t)g.ntx * g.nty * g.nchunk);
     if (lds > 65536)
         CG_HIP(hipFuncSetAttribute((const void *)k_grid<T, KW>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_grid<T, KW>), dim3(nblk), dim3(256), lds, s, geo, val, ts, fw, g, is2d);
+    hipLaunchKernelGGL((k_grid<T, KW>), dim3(nblk), dim3(grid_threads(lds)), lds, s, geo, val, ts,
+                       fw, g, is2d);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
@@ -176,7 +238,7 @@ __global__ __launch_bounds__(256) void k_degrid(const geo_t<T> *__restrict__ geo
                                                 const cx_t<T> *__restrict__ fw,
                                                 cx_t<T> *__restrict__ out, GridArgs g, int is2d)
 {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *lds = reinterpret_cast<cx_t<T> *>(smem_raw);
     const int tile = blockIdx.x;
     const int beg = tile_start[tile], end = tile_start[tile + 1];

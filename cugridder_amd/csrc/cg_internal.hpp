@@ -18,6 +18,9 @@ struct CoordArgs {
 };
 
 CoordArgs make_coord_args(const WPlan &p, const Tiling &t);
+void set_coord_tiling(CoordArgs &a, const Tiling &t);
+// bits of the sort key (the tile index)
+int sort_key_bits(int64_t ntiles);
 
 // ---- prep (cg_prep.hip)
 // min/max of the scaled w (FP64) over nrow rows of uvw [nrow][3]; partial

@@ -30,13 +30,25 @@ CoordArgs make_coord_args(const WPlan &p, const Tiling &t)
     a.nf2 = p.nf2;
     a.nf3 = p.nf3;
     a.kw = p.conv.kw;
+    a.pirange = 0;
+    a.has_w = 1;
+    set_coord_tiling(a, t);
+    return a;
+}
+
+void set_coord_tiling(CoordArgs &a, const Tiling &t)
+{
     a.tx = t.tx;
     a.ty = t.ty;
     a.ntx = t.ntx;
     a.nty = t.nty;
-    a.pirange = 0;
-    a.has_w = 1;
-    return a;
+}
+
+int sort_key_bits(int64_t ntiles)
+{
+    int b = 1;
+    while (((int64_t)1 << b) < ntiles) b++;
+    return b;
 }
 
 // include/conv.h:24 SHIFT_RESCALE (p = pirange)
@@ -236,8 +248,8 @@ __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t *keys, int64
     start[t] = (int32_t)lo;
 }
 
-int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int32_t *tile_start,
-                       hipStream_t s)
+int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles,
+                       int32_t *tile_start, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div(ntiles + 1, 256)), dim3(256), 0, s,
                        sorted_keys, m, ntiles, tile_start);

@@ -100,7 +100,8 @@ int nufft2d2f(int M, const float *x, const float *y, float *c, int N1, int N2,
  * host entry points on inputs already in HBM. */
 int cg_ctx_create(void **ctx, void *stream);
 int cg_ctx_destroy(void *ctx);
-/* options: "timing" (0/1, per-stage HIP events), "lds_budget" (bytes),
+/* options: "timing" (0/1, per-stage HIP events), "lds_budget" (degridding
+ * staging bytes), "grid_lds_budget" (gridding accumulator bytes),
  * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
