@@ -166,7 +166,7 @@ class Context:
         """(keys uint32-as-int64, perm int32, tile_start int32) of the last call."""
         torch = self._torch
         k, pm, ts = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
-        n = (ctypes.c_longlong * 2)()
+        n = (ctypes.c_longlong * 3)()
         check(lib().cg_ctx_sorted(self._h, ctypes.byref(k), ctypes.byref(pm), ctypes.byref(ts), n),
               "ctx_sorted")
         m, nt = n[0], n[1]
@@ -177,6 +177,7 @@ class Context:
             _memcpy_d2d(keys.data_ptr(), k.value, 4 * m)
             _memcpy_d2d(perm.data_ptr(), pm.value, 4 * m)
         _memcpy_d2d(tst.data_ptr(), ts.value, 4 * (nt + 1))
+        self.key_shift = int(n[2])
         return keys.to(torch.int64) & 0xFFFFFFFF, perm, tst
 
     def fft2d(self, data, direction):

@@ -41,11 +41,13 @@ struct Ctx {
     int grid_lds_budget = 65536;  // gridding accumulators
     int tile_grid = 8;
     int tile_degrid = 32;
+    int dense_min = 256;     // points per bin that send it to the MFMA gridding path (0: off)
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
+    int last_cshift = 0;
     // workspace (grow-only)
     DevBuf keys, keys2, vals, vals2, hist, scan, tstart, geo, val, grid, zq, phi3, phi1, phi2,
-        part, tw1, tw2, fftmp;
+        part, tw1, tw2, fftmp, dlist;
     int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
     // stage timing (HIP events on the context stream)
     bool timing = false;
@@ -59,7 +61,7 @@ struct Ctx {
     ~Ctx()
     {
         DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
-                          &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp};
+                          &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (ev_ok)
@@ -95,6 +97,7 @@ static void env_overrides(Ctx &C)
 {
     if (const char *e = std::getenv("CG_LDS_BUDGET")) C.lds_budget = std::atoi(e);
     if (const char *e = std::getenv("CG_GRID_LDS")) C.grid_lds_budget = std::atoi(e);
+    if (const char *e = std::getenv("CG_DENSE_MIN")) C.dense_min = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_GRID")) C.tile_grid = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
 }
@@ -168,15 +171,17 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
     CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
     CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                             as<int32_t>(C.vals2), M, sort_key_bits(ntiles), as<int32_t>(C.hist),
+                             as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles), as<int32_t>(C.hist),
                              as<int32_t>(C.scan), C.stream));
-    CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart), C.stream));
+    CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, as<int32_t>(C.tstart),
+                              C.stream));
     return CG_OK;
 }
 
-static GridArgs grid_args(const WPlan &P, const Tiling &t, int64_t m)
+static GridArgs grid_args(const WPlan &P, const Tiling &t, int64_t m, int cshift)
 {
     GridArgs g{};
+    g.cshift = cshift;
     g.m = (int)m;
     g.nf1 = P.nf1;
     g.nf2 = P.nf2;
@@ -229,6 +234,14 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     C.last_m = nrow;
     CoordArgs a = make_coord_args(P, t);
     const int64_t M = nrow;
+    // bins with many points per start cell go through the MFMA path, added
+    // on top of what the sparse kernel writes; it needs the points of a bin
+    // ordered by start cell, which the sparse kernel alone is better without
+    // (same-cell neighbours in a wavefront collide on the same LDS words)
+    const bool dense = sizeof(T) == 4 && C.dense_min > 0 && M > 0 &&
+                       grid_dense_supported(grid_args(P, t, M, a.cshift), P.conv.kw, eb);
+    if (!dense) a.cshift = 0;
+    C.last_cshift = a.cshift;
     CG_TRY(clk.mark(ST_SORT));
     if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
     else {
@@ -243,9 +256,16 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                                                 C.stream)));
     CG_TRY(clk.mark(ST_GRID));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
-    GridArgs g = grid_args(P, t, M);
+    GridArgs g = grid_args(P, t, M, a.cshift);
+    g.dense_min = dense ? C.dense_min : 0;
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
+    if (dense) {
+        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 1)));
+        CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
+                                 as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
+                                 (cx_t<float> *)C.grid.p, g, P.conv.kw, C.stream));
+    }
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     CG_TRY(clk.mark(ST_FFT));
@@ -294,6 +314,7 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     C.last_tiling = t;
     C.last_m = nrow;
     CoordArgs a = make_coord_args(P, t);
+    C.last_cshift = a.cshift;
     const int64_t M = nrow;
     CG_TRY(clk.mark(ST_SORT));
     if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
@@ -329,7 +350,7 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                               as<cx_t<T>>(C.tw1), la, lb, C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_GRID));
-    GridArgs g = grid_args(P, t, M);
+    GridArgs g = grid_args(P, t, M, a.cshift);
     if (M > 0)
         CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                           as<int32_t>(C.vals), as<int32_t>(C.tstart),
@@ -386,6 +407,8 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     a.nf3 = 1;
     a.kw = kw;
     set_coord_tiling(a, t);
+    if (type == 1) a.cshift = 0;  // the 2-D gridder is the sparse kernel only
+    C.last_cshift = a.cshift;
     a.pirange = 1;
     a.has_w = 0;
     const int64_t ntiles = t.ntiles();
@@ -401,10 +424,10 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY(ensure(C.scan, sizeof(int32_t) * scan_tmp_size(hs)));
         CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
         CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                                 as<int32_t>(C.vals2), M, sort_key_bits(ntiles),
+                                 as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles),
                                  as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream));
-        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, as<int32_t>(C.tstart),
-                                  C.stream));
+        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift,
+                                  as<int32_t>(C.tstart), C.stream));
     } else {
         CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (ntiles + 1), C.stream));
     }
@@ -421,7 +444,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(launch_fseries_grid(qd, P.nf1, as<double>(C.phi1), C.stream));
     CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    GridArgs g = grid_args(P, t, M);
+    GridArgs g = grid_args(P, t, M, a.cshift);
     // fast-axis columns that hold modes
     const int la = N1 - N1 / 2, lb = P.nf1 - N1 / 2;
     if (type == 1) {
@@ -703,6 +726,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "tile_grid")) C->tile_grid = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
+    else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
     else return CG_ERR_ARG;
     return CG_OK;
 }
@@ -853,6 +877,7 @@ int cg_ctx_sorted(void *ctx, void **keys, void **perm, void **tstart, long long 
     *tstart = C->tstart.p;
     n[0] = C->last_m;
     n[1] = C->last_tiling.ntiles();
+    n[2] = C->last_cshift;
     return CG_OK;
 }
 

@@ -100,6 +100,38 @@ __device__ __forceinline__ void dft_small(cx_t<T> *a)
         a[6] = csub<T>(e[2], w2);
         a[3] = cadd<T>(e[3], w3);
         a[7] = csub<T>(e[3], w3);
+    } else if constexpr (R == 16) {
+        // 16 = 4 x 4: X[k1 + 4 k2] = sum_n2 w16^(n2 k1) [sum_n1 x[4 n1 + n2] w4^(n1 k1)] w4^(n2 k2)
+        const T c1 = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+        const T s1 = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+        const T r2 = (T)0.70710678118654752440084436210485;
+        cx_t<T> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            cx_t<T> t[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+            dft_small<T, 4, DIR>(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = t[k1];
+        }
+        // w16^(DIR m) for m = n2 k1 in {1, 2, 3, 4, 6, 9}
+        auto tw = [&](cx_t<T> v, T wr, T wi) { return mkc(v.x * wr - v.y * wi, v.x * wi + v.y * wr); };
+        const T d = (T)DIR;
+        y[1][1] = tw(y[1][1], c1, d * s1);
+        y[1][2] = tw(y[1][2], r2, d * r2);
+        y[1][3] = tw(y[1][3], s1, d * c1);
+        y[2][1] = tw(y[2][1], r2, d * r2);
+        y[2][2] = cmul_i<T, DIR>(y[2][2]);
+        y[2][3] = tw(y[2][3], -r2, d * r2);
+        y[3][1] = tw(y[3][1], s1, d * c1);
+        y[3][2] = tw(y[3][2], -r2, d * r2);
+        y[3][3] = tw(y[3][3], -c1, -d * s1);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            cx_t<T> t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            dft_small<T, 4, DIR>(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) a[k1 + 4 * k2] = t[k2];
+        }
     } else if constexpr (R == 3) {
         const T h = (T)0.86602540378443864676372317075294;  // sqrt(3)/2
         cx_t<T> t = cadd<T>(a[1], a[2]);
@@ -348,6 +380,209 @@ __global__ __launch_bounds__(1024) void k_fft_cols_step(cx_t<T> *data, int n2, i
     }
 }
 
+// ------------------------------------------------------------ compile-time plans
+//
+// Power-of-two lengths get kernels whose radix plan, strides and thread
+// counts are compile-time constants: radix-16 passes for fp32 (one 16-point
+// butterfly per thread per pass) or radix-8 passes for fp64, then one
+// smaller pass.  Twiddles come from LDS.  A length-n table is split in two
+// levels, w^t = H[t / B] * L[t % B] with B = 2^s, so one complex multiply
+// replaces a global load.
+
+template <typename T>
+struct TwTab {
+    const cx_t<T> *H, *L;
+    int sh;  // B = 1 << sh
+    __device__ __forceinline__ cx_t<T> get(int t) const
+    {
+        const cx_t<T> h = H[t >> sh], l = L[t & ((1 << sh) - 1)];
+        return cmul<T>(h, l);
+    }
+};
+
+// load the two-level table of e^{2 pi i t / n}, t < n, from the full table
+// tw (length n, any n) into LDS: H has ceil(n / B) entries, L has B
+template <typename T>
+__device__ __forceinline__ TwTab<T> load_twtab(cx_t<T> *lds, const cx_t<T> *__restrict__ tw, int n,
+                                               int sh)
+{
+    const int B = 1 << sh, nh = (n + B - 1) >> sh;
+    for (int t = threadIdx.x; t < B; t += blockDim.x) lds[t] = t < n ? tw[t] : mkc((T)1, (T)0);
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) lds[B + h] = tw[h << sh];
+    return TwTab<T>{lds + B, lds, sh};
+}
+
+template <int NB, bool BFAST, int N, bool PAD>
+__device__ __forceinline__ int ct_pos(int b, int e)
+{
+    if constexpr (BFAST) return b + e * NB;
+    const int x = b * N + e;
+    return PAD ? x + (x >> 5) : x;
+}
+
+// one Stockham pass of radix R over NB transforms of length N held in LDS,
+// NT threads; twiddle w_N^t read as tab.get(t * ts) (tab may describe a
+// longer transform whose length is ts * N)
+template <typename T, int DIR, int N, int R, int P, int NB, bool BFAST, bool PAD, int NT>
+__device__ __forceinline__ void ct_pass(cx_t<T> *buf, const TwTab<T> &tab, int ts)
+{
+    constexpr int TN = N / R;
+    constexpr int NBF = NB * TN;
+    constexpr int K = (NBF + NT - 1) / NT;
+    constexpr int TWS = N / (P * R);
+    cx_t<T> u[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (NBF % NT == 0 || q < NBF) {
+            const int b = BFAST ? q % NB : q / TN;
+            const int i = BFAST ? q / NB : q % TN;
+#pragma unroll
+            for (int r = 0; r < R; r++) u[k][r] = buf[ct_pos<NB, BFAST, N, PAD>(b, i + r * TN)];
+        }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = threadIdx.x + k * NT;
+        if (NBF % NT == 0 || q < NBF) {
+            const int b = BFAST ? q % NB : q / TN;
+            const int i = BFAST ? q / NB : q % TN;
+            const int kk = i % P;
+            if constexpr (P > 1) {
+                // W^r for r = 4 r1 + r2 as (W^4)^r1 W^r2: two table lookups
+                cx_t<T> w1 = tab.get(kk * TWS * ts);
+                if (DIR < 0) w1.y = -w1.y;
+                const cx_t<T> w2 = cmul<T>(w1, w1);
+                const cx_t<T> pw[4] = {mkc((T)1, (T)0), w1, w2, cmul<T>(w2, w1)};
+                if constexpr (R <= 4) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) u[k][r] = cmul<T>(u[k][r], pw[r]);
+                } else {
+                    cx_t<T> w4 = tab.get(4 * kk * TWS * ts);
+                    if (DIR < 0) w4.y = -w4.y;
+                    const cx_t<T> w8 = cmul<T>(w4, w4);
+                    const cx_t<T> qw[4] = {mkc((T)1, (T)0), w4, w8, cmul<T>(w8, w4)};
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        cx_t<T> v = u[k][r];
+                        if (r & 3) v = cmul<T>(v, pw[r & 3]);
+                        if (r >> 2) v = cmul<T>(v, qw[r >> 2]);
+                        u[k][r] = v;
+                    }
+                }
+            }
+            dft_small<T, R, DIR>(u[k]);
+            const int j = (i - kk) * R + kk;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[ct_pos<NB, BFAST, N, PAD>(b, j + r * P)] = u[k][r];
+        }
+    }
+    lds_barrier();
+}
+
+// all passes: radix RMAX while possible, then the remainder
+template <typename T, int DIR, int N, int P, int NB, bool BFAST, bool PAD, int NT, int RMAX>
+__device__ __forceinline__ void ct_fft(cx_t<T> *buf, const TwTab<T> &tab, int ts)
+{
+    if constexpr (P < N) {
+        constexpr int R = (N / P) >= RMAX ? RMAX : (N / P);
+        ct_pass<T, DIR, N, R, P, NB, BFAST, PAD, NT>(buf, tab, ts);
+        ct_fft<T, DIR, N, P * R, NB, BFAST, PAD, NT, RMAX>(buf, tab, ts);
+    }
+}
+
+template <typename T>
+constexpr int ct_rmax() { return sizeof(T) == 4 ? 16 : 8; }
+
+// rows of N = 2^LOGN points, one row per workgroup, N / EPT threads,
+// persistent over the nrows rows of all planes
+template <typename T, int DIR, int LOGN>
+__global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t<T> *data, int64_t nrows,
+                                                      const cx_t<T> *__restrict__ tw, Live live)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int EPT = fft_ept<T>();
+    constexpr int NT = N / EPT;
+    constexpr int SH = LOGN / 2;
+    constexpr int E = N + (N >> 5);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
+    const TwTab<T> tab = load_twtab<T>(buf + E, tw, N, SH);
+    for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
+        const cx_t<T> *src = data + g * (int64_t)N;
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * NT;
+            buf[k + (k >> 5)] = (DIR > 0 || live(k)) ? src[k] : mkc((T)0, (T)0);
+        }
+        lds_barrier();
+        ct_fft<T, DIR, N, 1, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
+        cx_t<T> *dst = data + g * (int64_t)N;
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * NT;
+            if (DIR < 0 || live(k)) dst[k] = buf[k + (k >> 5)];
+        }
+        lds_barrier();
+    }
+}
+
+// one step of the split column FFT with sub-transform length L = 2^LOGL on
+// CB adjacent live columns (see k_fft_cols_step); tw2 is the length-n2 table
+template <typename T, int DIR, int LOGL, int CB>
+__global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_step_ct(cx_t<T> *data, int n2, int n1, int se, int sj,
+                                                           int twmode, const cx_t<T> *__restrict__ tw2,
+                                                           Live live)
+{
+    constexpr int L = 1 << LOGL;
+    constexpr int EPT = fft_ept<T>();
+    constexpr int E = CB * L;
+    constexpr int NT = E / EPT;
+    constexpr int SH2 = 7;  // inter-step table: B = 128
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
+    // sub-transform table (w_L^t = w_n2^(t n2/L), read with stride n2/L from
+    // the two-level n2 table) and the inter-step twiddles w_n2^(j e)
+    const TwTab<T> tab = load_twtab<T>(buf + E, tw2, n2, SH2);
+    lds_barrier();
+    const int j = blockIdx.y;
+    cx_t<T> *pl = data + (size_t)blockIdx.z * n2 * n1;
+    const int nlive = live.count(n1);
+    const int c0 = blockIdx.x * CB;
+    const size_t row0 = (size_t)j * sj;
+    const int tstride = n2 >> LOGL;
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+        const int k = threadIdx.x + u * NT;
+        const int e = k / CB, b = k % CB;
+        const int c = c0 + b;
+        cx_t<T> v = c < nlive ? pl[(row0 + (size_t)e * se) * n1 + live.at(c)] : mkc((T)0, (T)0);
+        if (twmode == 2) {
+            cx_t<T> w = tab.get(j * e);
+            if (DIR < 0) w.y = -w.y;
+            v = cmul<T>(v, w);
+        }
+        buf[k] = v;
+    }
+    lds_barrier();
+    ct_fft<T, DIR, L, 1, CB, true, false, NT, ct_rmax<T>()>(buf, tab, tstride);
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+        const int k = threadIdx.x + u * NT;
+        const int e = k / CB, b = k % CB;
+        const int c = c0 + b;
+        if (c >= nlive) continue;
+        cx_t<T> v = buf[k];
+        if (twmode == 1) {
+            cx_t<T> w = tab.get(j * e);
+            if (DIR < 0) w.y = -w.y;
+            v = cmul<T>(v, w);
+        }
+        pl[(row0 + (size_t)e * se) * n1 + live.at(c)] = v;
+    }
+}
+
 // split order <-> natural order of the column axis, out of place:
 // to_natural: dst[k] = src[pos(k)], else dst[pos(k)] = src[k]
 template <typename T>
@@ -433,10 +668,100 @@ bool fft_supported(int n)
     return true;
 }
 
+static int log2_exact(int n)
+{
+    if (n <= 0 || (n & (n - 1))) return -1;
+    return ilog2(n);
+}
+
+// compile-time row kernel for n1 = 2^LOGN (returns 1 if LOGN is not covered)
+template <typename T, int DIR, int LOGN>
+static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live, hipStream_t s)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int NT = N / fft_ept<T>();
+    if constexpr (NT < 64 || NT > 1024) {
+        return 1;
+    } else {
+        constexpr int SH = LOGN / 2;
+        const size_t lds = (size_t)(N + (N >> 5) + (1 << SH) + (N >> SH)) * sizeof(cx_t<T>);
+        if (lds > (size_t)kLdsMax) return 1;
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows_ct<T, DIR, LOGN>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        int per_cu = std::max<int>(1, (int)(kLdsMax / lds));
+        per_cu = std::min(per_cu, std::max(1, 2048 / NT));
+        const int64_t grid = std::min<int64_t>(nrows, (int64_t)256 * per_cu);
+        hipLaunchKernelGGL((k_fft_rows_ct<T, DIR, LOGN>), dim3((unsigned)grid), dim3(NT), lds, s, data,
+                           nrows, tw, live);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    }
+}
+
+template <typename T, int DIR>
+static int rows_ct_dispatch(cx_t<T> *data, int64_t nrows, int n1, const cx_t<T> *tw, Live live,
+                            hipStream_t s)
+{
+    switch (log2_exact(n1)) {
+    case 9: return rows_ct<T, DIR, 9>(data, nrows, tw, live, s);
+    case 10: return rows_ct<T, DIR, 10>(data, nrows, tw, live, s);
+    case 11: return rows_ct<T, DIR, 11>(data, nrows, tw, live, s);
+    case 12: return rows_ct<T, DIR, 12>(data, nrows, tw, live, s);
+    case 13: return rows_ct<T, DIR, 13>(data, nrows, tw, live, s);
+    case 14: return rows_ct<T, DIR, 14>(data, nrows, tw, live, s);
+    default: return 1;
+    }
+}
+
+// compile-time column step for sub-transforms of L = 2^LOGL (1 if not covered)
+template <typename T, int DIR, int LOGL>
+static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
+                        int twmode, const cx_t<T> *tw2, Live live, hipStream_t s)
+{
+    constexpr int L = 1 << LOGL;
+    constexpr int ELEM = (int)sizeof(cx_t<T>);
+    constexpr int CB0 = 512 / ELEM;
+    constexpr int CB = CB0 * L * ELEM <= 65536 ? CB0 : 65536 / (L * ELEM);
+    constexpr int NT = CB * L / fft_ept<T>();
+    if constexpr (NT < 64 || NT > 1024) {
+        return 1;
+    } else {
+        const size_t lds = (size_t)(CB * L + 128 + (n2 + 127) / 128) * ELEM;
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_ct<T, DIR, LOGL, CB>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int nlive = live.count(n1);
+        dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)nsub, (unsigned)batch);
+        hipLaunchKernelGGL((k_fft_cols_step_ct<T, DIR, LOGL, CB>), grid, dim3(NT), lds, s, data, n2,
+                           n1, se, sj, twmode, tw2, live);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    }
+}
+
+template <typename T, int DIR>
+static int cols_step_ct_dispatch(cx_t<T> *data, int batch, int n2, int n1, int L, int se, int sj,
+                                 int nsub, int twmode, const cx_t<T> *tw2, Live live, hipStream_t s)
+{
+    switch (log2_exact(L)) {
+    case 4: return cols_step_ct<T, DIR, 4>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    case 5: return cols_step_ct<T, DIR, 5>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    case 6: return cols_step_ct<T, DIR, 6>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    case 7: return cols_step_ct<T, DIR, 7>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    case 8: return cols_step_ct<T, DIR, 8>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    default: return 1;
+    }
+}
+
 template <typename T, int DIR>
 static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw, Live live,
                     hipStream_t s)
 {
+    if (fft_knob("CG_FFT_CT", 1)) {
+        const int r = rows_ct_dispatch<T, DIR>(data, (int64_t)n2 * batch, n1, tw, live, s);
+        if (r != 1) return r;
+    }
     FftShape S{};
     S.n = n1;
     S.nrad = factor_radices(n1, S.rad);
@@ -508,7 +833,13 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
+    const bool ct = fft_knob("CG_FFT_CT", 1) != 0;
     auto step = [&](int L, int se, int sj, int nsub, int twmode) -> int {
+        if (ct) {
+            const int r = cols_step_ct_dispatch<T, DIR>(data, batch, n2, n1, L, se, sj, nsub, twmode,
+                                                        tw2, live, s);
+            if (r != 1) return r;
+        }
         FftShape S{};
         S.n = L;
         S.nrad = factor_radices(L, S.rad);

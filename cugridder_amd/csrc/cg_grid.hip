@@ -110,6 +110,8 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
             const int sidx = sby_ * g.ntx + sbx_;
             s_beg[b] = tile_start[sidx];
             s_cnt[b] = tile_start[sidx + 1] - s_beg[b];
+            // dense bins are added afterwards by k_grid_dense
+            if (g.dense_min > 0 && s_cnt[b] >= g.dense_min) s_cnt[b] = 0;
             s_ox[b] = ox;
             s_oy[b] = oy;
             tot += s_cnt[b];
@@ -212,6 +214,199 @@ static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, c
                        fw, g, is2d);
     CG_HIP(hipGetLastError());
     return CG_OK;
+}
+
+// ------------------------------------------------------------ dense bins
+//
+// A bin holding many points per start cell (dense uv coverage) is gridded as
+// a product per start cell: the points j of one cell share the footprint
+// origin, so their contributions to the cell's kw x kw x nf3 footprint are
+//   C[tap][p] = sum_j A[tap][j] B[j][p],  A = v_j psi_x psi_y,  B = psi_z,
+// a (64 taps) x (2 points) x (32 planes) FP32 MFMA step per point pair
+// (v_mfma_f32_32x32x2f32, real and imaginary parts as two products; taps are
+// t1 + 8 t2, planes the whole w axis).  C is then added once per cell into
+// the bin's footprint region in LDS (FP64), and the region once per bin into
+// the grid with FP32 global atomics, on top of what the sparse kernel wrote.
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kDenseThreads = 512;
+
+__global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ tile_start,
+                                                    int64_t ntiles, int dense_min,
+                                                    int32_t *__restrict__ list)
+{
+    // list[0] counts the dense bins, list[1..] holds them (any order)
+    const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (t >= ntiles) return;
+    if (tile_start[t + 1] - tile_start[t] >= dense_min) list[1 + atomicAdd(list, 1)] = (int32_t)t;
+}
+
+__global__ void k_zero_i32(int32_t *p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0;
+}
+
+template <int KW>
+__global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__restrict__ geo,
+                                                              const float2 *__restrict__ val,
+                                                              const uint32_t *__restrict__ keys,
+                                                              const int32_t *__restrict__ tile_start,
+                                                              const int32_t *__restrict__ list,
+                                                              float2 *__restrict__ fw, GridArgs g)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int RX = g.tx + KW - 1, RY = g.ty + KW - 1, NP = g.nf3;
+    const int rcells = RX * RY * NP;
+    double *are = reinterpret_cast<double *>(smem_raw);
+    double *aim = are + rcells;
+    int *run = reinterpret_cast<int *>(aim + rcells);  // tx*ty + 1 run starts
+    __shared__ int s_next;
+    const int ncell = g.tx * g.ty;
+    const int nlist = list[0];
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5, i = lane & 31;
+    const float beta = (float)g.beta, ec = (float)g.es_c, hw = 0.5f * KW;
+    // lane roles: A[i][half] with tap i = t1 + 8 t2 (two 32-tap blocks),
+    // B[half][j] with plane j = i
+    const int t1 = i & 7, t2a = i >> 3, t2b = 4 + (i >> 3);
+
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int tile = list[1 + li];
+        const int bx = tile % g.ntx, by = tile / g.ntx;
+        const int beg = tile_start[tile], end = tile_start[tile + 1];
+        for (int k = threadIdx.x; k < 2 * rcells; k += blockDim.x) are[k] = 0.0;
+        // first sorted position of every start cell of the bin
+        for (int c = threadIdx.x; c <= ncell; c += blockDim.x) {
+            const uint32_t key = ((uint32_t)tile << g.cshift) + (uint32_t)c;
+            int lo = beg, hi = end;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (keys[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            run[c] = lo;
+        }
+        if (threadIdx.x == 0) s_next = 0;
+        __syncthreads();
+        for (;;) {
+            int c = 0;
+            if (lane == 0) c = atomicAdd(&s_next, 1);
+            c = __shfl(c, 0);
+            if (c >= ncell) break;
+            const int a = run[c], b = run[c + 1];
+            if (a >= b) continue;
+            v16f r0 = {}, q0 = {}, r1 = {}, q1 = {};
+            for (int k0 = a; k0 < b; k0 += 2) {
+                const int k = k0 + half;
+                float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f, bz = 0.f;
+                if (k < b) {
+                    const float4 G = geo[k];
+                    const float2 v = val[k];
+                    const float kx = t1 < KW ? es_val<float>(G.x + (float)t1, beta, ec, hw) : 0.f;
+                    const float ky0 = t2a < KW ? es_val<float>(G.y + (float)t2a, beta, ec, hw) : 0.f;
+                    const float ky1 = t2b < KW ? es_val<float>(G.y + (float)t2b, beta, ec, hw) : 0.f;
+                    const float w0 = kx * ky0, w1 = kx * ky1;
+                    a0r = v.x * w0;
+                    a0i = v.y * w0;
+                    a1r = v.x * w1;
+                    a1i = v.y * w1;
+                    if (i < NP) {
+                        int d = i - (int)(__float_as_uint(G.w) >> 16);
+                        if (d < 0) d += NP;
+                        if (d < KW) bz = es_val<float>(G.z + (float)d, beta, ec, hw);
+                    }
+                }
+                r0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0r, bz, r0, 0, 0, 0);
+                q0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0i, bz, q0, 0, 0, 0);
+                r1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1r, bz, r1, 0, 0, 0);
+                q1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1i, bz, q1, 0, 0, 0);
+            }
+            // D[tap][plane]: lane holds plane i, taps (v/4)*8 + half*4 + v%4 (+32)
+            const int lx = c % g.tx, ly = c / g.tx;
+            if (i < NP) {
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int tap = (v >> 2) * 8 + half * 4 + (v & 3);
+                    const int u1 = tap & 7, u2 = tap >> 3;  // block 0: t2 < 4; block 1: t2 + 4
+                    if (u1 < KW && u2 < KW) {
+                        const int idx = ((ly + u2) * RX + lx + u1) * NP + i;
+                        atomicAdd(are + idx, (double)r0[v]);
+                        atomicAdd(aim + idx, (double)q0[v]);
+                    }
+                    if (u1 < KW && u2 + 4 < KW) {
+                        const int idx = ((ly + u2 + 4) * RX + lx + u1) * NP + i;
+                        atomicAdd(are + idx, (double)r1[v]);
+                        atomicAdd(aim + idx, (double)q1[v]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // footprint region -> grid, x fastest so a wavefront covers row runs
+        for (int k = threadIdx.x; k < rcells; k += blockDim.x) {
+            const int rx = k % RX;
+            const int rest = k / RX;
+            const int ry = rest % RY, p = rest / RY;
+            const int idx = (ry * RX + rx) * NP + p;
+            int gx = bx * g.tx + rx;
+            if (gx >= g.nf1) gx -= g.nf1;
+            int gy = by * g.ty + ry;
+            if (gy >= g.nf2) gy -= g.nf2;
+            float2 *dst = fw + ((size_t)p * g.nf2 + gy) * g.nf1 + gx;
+            atomicAdd(&dst->x, (float)are[idx]);
+            atomicAdd(&dst->y, (float)aim[idx]);
+        }
+        __syncthreads();
+    }
+}
+
+static size_t dense_lds_bytes(const GridArgs &g, int kw)
+{
+    const size_t rcells = (size_t)(g.tx + kw - 1) * (g.ty + kw - 1) * g.nf3;
+    return 16 * rcells + 4 * (size_t)(g.tx * g.ty + 1);
+}
+
+bool grid_dense_supported(const GridArgs &g, int kw, int elem)
+{
+    return elem == 8 && kw >= 2 && kw <= 8 && g.nf3 >= 2 * kw && g.nf3 <= 32 && g.cshift > 0 &&
+           g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 && dense_lds_bytes(g, kw) <= 163840 - 64;
+}
+
+template <int KW>
+static int grid_dense_kw(const float4 *geo, const float2 *val, const uint32_t *keys,
+                         const int32_t *ts, int32_t *list, float2 *fw, const GridArgs &g,
+                         hipStream_t s)
+{
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list);
+    hipLaunchKernelGGL(k_dense_bins, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, ts,
+                       ntiles, g.dense_min, list);
+    CG_HIP(hipGetLastError());
+    const size_t lds = dense_lds_bytes(g, KW);
+    if (lds > 65536)
+        CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int per_cu = std::max<int>(1, (int)(163840 / lds));
+    const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
+    hipLaunchKernelGGL((k_grid_dense<KW>), dim3(nblk), dim3(kDenseThreads), lds, s, geo, val, keys,
+                       ts, list, fw, g);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uint32_t *keys,
+                      const int32_t *ts, int32_t *list, cx_t<float> *fw, const GridArgs &g, int kw,
+                      hipStream_t s)
+{
+    if (!grid_dense_supported(g, kw, 8) || g.dense_min <= 0) return CG_ERR_ARG;
+    switch (kw) {
+#define CG_CASE(K) \
+    case K: return grid_dense_kw<K>(geo, val, keys, ts, list, fw, g, s);
+        CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8)
+#undef CG_CASE
+    default: return CG_ERR_ARG;
+    }
 }
 
 template <typename T>

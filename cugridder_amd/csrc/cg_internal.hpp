@@ -13,14 +13,17 @@ struct CoordArgs {
     double i_center, gamma, o_center;
     int nf1, nf2, nf3, kw;
     int tx, ty, ntx, nty;
+    int cshift;        // key = tile << cshift | cell of the start inside the tile
     int pirange;       // 1: coordinates already in [-pi,pi) radians (2-D NUFFT)
     int has_w;         // 0: 2-D (no w axis)
 };
 
 CoordArgs make_coord_args(const WPlan &p, const Tiling &t);
+// fill the tiling fields of a; the start cell joins the sort key when the
+// key still fits 32 bits
 void set_coord_tiling(CoordArgs &a, const Tiling &t);
-// bits of the sort key (the tile index)
-int sort_key_bits(int64_t ntiles);
+// bits of the sort key
+int sort_key_bits(const CoordArgs &a, int64_t ntiles);
 
 // ---- prep (cg_prep.hip)
 // min/max of the scaled w (FP64) over nrow rows of uvw [nrow][3]; partial
@@ -28,7 +31,8 @@ int sort_key_bits(int64_t ntiles);
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
                    double &wmin, double &wmax, hipStream_t s);
-// keys[j] = tile of the footprint start of row j; vals[j] = j
+// keys[j] = (tile of the footprint start of row j) << cshift | start cell
+// inside the tile; vals[j] = j
 template <typename TU>
 int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
                 int32_t *vals, hipStream_t s);
@@ -36,8 +40,8 @@ int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
 template <typename TU>
 int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a,
                    uint32_t *keys, int32_t *vals, hipStream_t s);
-// tile_start[t] = first sorted position with key >= t, t in [0, ntiles]
-int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles,
+// tile_start[t] = first sorted position with key >> shift >= t, t in [0, ntiles]
+int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
                        int32_t *tile_start, hipStream_t s);
 
 // Sorted point records: geo = {d1, d2, d3, packed(lx | ly<<8 | z0<<16)} and a
@@ -72,6 +76,8 @@ struct GridArgs {
     int m;  // number of points (rows)
     int nf1, nf2, nf3;
     int tx, ty, ntx, nty, sbx, sby, pc, nchunk;
+    int cshift;     // sort key = tile << cshift | start cell inside the tile
+    int dense_min;  // > 0: bins with at least this many points go to the dense path
     double beta, es_c;
 };
 template <typename T>
@@ -82,6 +88,13 @@ int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
                   const int32_t *tile_start, const cx_t<T> *fw, TV *out,
                   const GridArgs &g, int kw, hipStream_t s);
 int grid_lds_bytes(const GridArgs &g, int elem);
+// Dense bins (fp32, kw <= 8, nf3 <= 32, cshift > 0): the sparse kernel
+// launched with g.dense_min > 0 leaves them out; this adds them to fw with
+// per-start-cell MFMA products.  list / count need ntiles + 1 ints of scratch.
+bool grid_dense_supported(const GridArgs &g, int kw, int elem);
+int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uint32_t *keys,
+                      const int32_t *tile_start, int32_t *list, cx_t<float> *fw, const GridArgs &g,
+                      int kw, hipStream_t s);
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
 
 // ---- FFT (cg_fft.hip): in-place unnormalised 2-D FFT of `batch` planes

@@ -36,20 +36,25 @@ CoordArgs make_coord_args(const WPlan &p, const Tiling &t)
     return a;
 }
 
+static int bits_for(int64_t n)
+{
+    int b = 1;
+    while (((int64_t)1 << b) < n) b++;
+    return b;
+}
+
 void set_coord_tiling(CoordArgs &a, const Tiling &t)
 {
     a.tx = t.tx;
     a.ty = t.ty;
     a.ntx = t.ntx;
     a.nty = t.nty;
+    int cs = 0;
+    while ((1 << cs) < t.tx * t.ty) cs++;
+    a.cshift = bits_for(t.ntiles()) + cs <= 32 ? cs : 0;
 }
 
-int sort_key_bits(int64_t ntiles)
-{
-    int b = 1;
-    while (((int64_t)1 << b) < ntiles) b++;
-    return b;
-}
+int sort_key_bits(const CoordArgs &a, int64_t ntiles) { return bits_for(ntiles) + a.cshift; }
 
 // include/conv.h:24 SHIFT_RESCALE (p = pirange)
 __device__ __forceinline__ double shift_rescale(double x, int N, int pirange)
@@ -101,7 +106,9 @@ __device__ __forceinline__ Foot make_foot(double u, double v, double w, const Co
 
 __device__ __forceinline__ uint32_t tile_key(const Foot &f, const CoordArgs &a)
 {
-    return (uint32_t)((f.y0 / a.ty) * a.ntx + (f.x0 / a.tx));
+    const uint32_t tile = (uint32_t)((f.y0 / a.ty) * a.ntx + (f.x0 / a.tx));
+    if (!a.cshift) return tile;
+    return (tile << a.cshift) | (uint32_t)((f.y0 % a.ty) * a.tx + f.x0 % a.tx);
 }
 
 // ------------------------------------------------------------ w min/max
@@ -235,24 +242,24 @@ int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a, uint
 // ------------------------------------------------------------ tile bounds
 
 __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t *keys, int64_t m,
-                                                     int64_t ntiles, int32_t *start)
+                                                     int64_t ntiles, int shift, int32_t *start)
 {
     int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t > ntiles) return;
-    int64_t lo = 0, hi = m;  // first index with keys[i] >= t
+    int64_t lo = 0, hi = m;  // first index with keys[i] >> shift >= t
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)keys[mid] < t) lo = mid + 1;
+        if ((int64_t)(keys[mid] >> shift) < t) lo = mid + 1;
         else hi = mid;
     }
     start[t] = (int32_t)lo;
 }
 
-int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles,
+int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
                        int32_t *tile_start, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div(ntiles + 1, 256)), dim3(256), 0, s,
-                       sorted_keys, m, ntiles, tile_start);
+                       sorted_keys, m, ntiles, shift, tile_start);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }

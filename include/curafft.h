@@ -103,7 +103,9 @@ int cg_ctx_destroy(void *ctx);
 /* options: "timing" (0/1, per-stage HIP events), "lds_budget" (degridding
  * staging bytes), "grid_lds_budget" (gridding accumulator bytes),
  * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
- * "stop_after" (test hook, see below) */
+ * "dense_min" (points per gridding bin from which the complex64 gridder
+ * uses per-start-cell MFMA products; 0 disables), "stop_after" (test hook,
+ * see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
 /* per-stage times summed over calls, out[14] = {ms2dirty[7], dirty2ms[7]}
  * in ms, stages: plan, sort, records, grid/degrid, fft rows, fft columns,
@@ -139,8 +141,8 @@ int cg_plan_query(int nrow, int nxdirty, int nydirty, double fov, double freq,
 
 /* Test hooks.  Device pointers into the context workspace from the last call
  * (valid until the next call): the w-plane grid and its cell count; the
- * sorted tile keys (uint32), row permutation (int32), tile starts (int32)
- * with n[0] = rows, n[1] = tiles.  Option "stop_after" (1: after gridding or
+ * sorted keys (uint32: tile << n[2] | start cell inside the tile), row
+ * permutation (int32), tile starts (int32) with n[0] = rows, n[1] = tiles.  Option "stop_after" (1: after gridding or
  * the dirty2ms prologue, 2: after the plane FFTs) stops a call early. */
 int cg_ctx_grid(void *ctx, void **ptr, long long *cells);
 /* Order of the spectrum's column axis in the grid after the forward column

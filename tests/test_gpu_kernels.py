@@ -31,9 +31,10 @@ def test_radix_sort_is_stable_and_exact(gpu, m, bits):
     assert np.array_equal(tk.cpu().numpy().view(np.uint32), keys[ref].astype(np.uint32))
 
 
-def _host_keys(uvw, freq, plan, nf1, nf2, tx, ty, sign=-1):
-    """Tile key of each row's footprint start, evaluated in FP64 on the host
-    with the same operation order as the device (include/conv.h:24)."""
+def _host_keys(uvw, freq, plan, nf1, nf2, tx, ty, cs):
+    """Sort key of each row, evaluated in FP64 on the host with the same
+    operation order as the device (include/conv.h:24): the tile of the
+    footprint start, then (cs > 0) the start cell inside the tile."""
     two_pi = 6.28318530717958648
     scale = 2.0 * np.pi * (freq / 299792458.0)
 
@@ -45,7 +46,11 @@ def _host_keys(uvw, freq, plan, nf1, nf2, tx, ty, sign=-1):
     kw = plan["kw"]
     x0 = np.ceil(g(uvw[:, 1] * scale * plan["pix_y"], nf1) - 0.5 * kw).astype(np.int64) % nf1
     y0 = np.ceil(g(uvw[:, 0] * scale * plan["pix_x"], nf2) - 0.5 * kw).astype(np.int64) % nf2
-    return (y0 // ty) * (nf1 // tx) + x0 // tx
+    tile = (y0 // ty) * (nf1 // tx) + x0 // tx
+    if cs == 0:
+        return tile
+    assert cs == (tx * ty).bit_length() - 1
+    return (tile << cs) | ((y0 % ty) * tx + x0 % tx)
 
 
 @pytest.mark.parametrize("nrow,nx,ny,fov,eps,freq", [(5000, 256, 256, 10.0, 1e-6, 1e9),
@@ -63,16 +68,22 @@ def test_tile_keys_and_permutation_bit_exact(gpu, nrow, nx, ny, fov, eps, freq):
                  fov, freq, eps)
     p = ctx.last_plan()
     keys, perm, ts = ctx.sorted_view()
-    hk = _host_keys(uvw, freq, p, p["nf1"], p["nf2"], p["tile_x"], p["tile_y"])
+    cs = ctx.key_shift
+    # the start cell joins the key exactly when the dense gridding path runs
+    assert (cs > 0) == (p["kw"] <= 8 and p["nf3"] <= 32)
+    hk = _host_keys(uvw, freq, p, p["nf1"], p["nf2"], p["tile_x"], p["tile_y"], cs)
     ref = np.argsort(hk, kind="stable")
     assert np.array_equal(perm.cpu().numpy(), ref)
     assert np.array_equal(keys.cpu().numpy(), hk[ref])
     ntiles = (p["nf1"] // p["tile_x"]) * (p["nf2"] // p["tile_y"])
-    assert np.array_equal(ts.cpu().numpy(), np.searchsorted(hk[ref], np.arange(ntiles + 1)))
+    assert np.array_equal(ts.cpu().numpy(), np.searchsorted(hk[ref] >> cs, np.arange(ntiles + 1)))
 
 
 @pytest.mark.parametrize("shape", [(1, 8, 8), (2, 30, 500), (3, 64, 2048), (1, 96, 16384),
-                                   (2, 2000, 12), (1, 1250, 54), (4, 16, 5000)])
+                                   (2, 2000, 12), (1, 1250, 54), (4, 16, 5000),
+                                   # power-of-two rows and split columns (compile-time plans)
+                                   (1, 2048, 64), (1, 16384, 32), (1, 4096, 512), (2, 256, 1024),
+                                   (1, 32768 // 64, 8192)])
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_fft2d_vs_rocfft(gpu, shape, dtype):
     torch = gpu
@@ -112,6 +123,42 @@ def test_gridding_stage_vs_oracle(gpu, single, case):
     ref = O.ms2dirty_grid(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, fov, eps, single=single)
     assert got.shape == ref.shape
     assert rel_l2(got, ref) < (3e-6 if single else 1e-13)
+
+
+@pytest.mark.parametrize("case", [(3000, 64, 96, 10.0, 1e-6, 1), (20000, 256, 256, 2.0, 1e-5, 1),
+                                  (131072, 1024, 1024, 2.0, 1e-5, 64)])
+def test_dense_gridding_vs_oracle(gpu, case):
+    """Bins holding at least dense_min points are gridded by per-start-cell
+    MFMA products (complex64): the planes must match the oracle's spreading
+    and the all-sparse path."""
+    torch = gpu
+    from cugridder_amd import Context
+    nrow, nx, ny, fov, eps, dmin = case
+    if nrow == 131072:
+        rng = np.random.default_rng(1)
+        uvw = np.empty((nrow, 3))
+        uvw[:, :2] = rng.normal(0, 1500, (nrow, 2))
+        uvw[:, 2] = rng.uniform(-300, 300, nrow)
+        ms = (rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)).astype(np.complex64)
+        wgt = rng.uniform(0.5, 1.5, nrow).astype(np.float32)
+        f0 = 1e8
+    else:
+        k = wst_case(nrow, nx, ny, fov)
+        uvw, ms, wgt, f0 = k["uvw"], k["ms"].astype(np.complex64), k["wgt"].astype(np.float32), k["f0"]
+    tu = torch.tensor(uvw, device="cuda")
+    tv = torch.tensor(ms, device="cuda")
+    tw = torch.tensor(wgt, device="cuda")
+    grids = {}
+    for d in (0, dmin):
+        ctx = Context(stop_after=1, dense_min=d)
+        ctx.ms2dirty(tu, tv, tw, nx, ny, fov, f0, eps)
+        grids[d] = ctx.grid_view(torch.complex64).cpu().numpy()
+    p = ctx.last_plan()
+    assert p["kw"] <= 8 and p["nf3"] <= 32, "case must be eligible for the dense path"
+    ref = O.ms2dirty_grid(uvw, f0, ms.astype(np.complex128), wgt.astype(np.float64), nx, ny, fov,
+                          eps, single=True)
+    assert rel_l2(grids[dmin], ref) < 3e-6
+    assert rel_l2(grids[dmin], grids[0]) < 3e-6
 
 
 @pytest.mark.parametrize("single", [False, True])
