@@ -113,24 +113,35 @@ def make_nufft2d(torch, dev, M=1 << 25, N=4096, seed=2):
 
 # ---------------------------------------------------------------- roofline
 
-def kernel_bytes(stage, direction, W, plan):
-    """Algorithmic (compulsory) HBM bytes of one launch of the stage's kernel,
-    complex64 grid: records are 24 B per row (FP32 offsets + packed cell +
-    complex value), the grid C = nf1*nf2*nf3 cells of 8 B."""
+def stage_kernel(stage, direction, W, plan):
+    """(kernel name, launches, algorithmic HBM bytes per launch) of a pipeline
+    stage, complex64 grid.  Records are 24 B per row (FP32 offsets + packed
+    cell, complex value); C = nf1*nf2*nf3 cells of 8 B; f = N1/nf1 is the
+    fraction of fast-axis columns that hold image pixels (only those are
+    column-transformed, stored by the forward row FFT and read by the
+    inverse one)."""
     M = W["M"]
-    C = plan["nf1"] * plan["nf2"] * plan["nf3"]
+    nf1, nf2, nf3 = plan["nf1"], plan["nf2"], plan["nf3"]
+    C = nf1 * nf2 * nf3
+    n1 = W["ny"]
+    f = n1 / nf1
     npix = W["nx"] * W["ny"]
+    p2 = lambda n: n & (n - 1) == 0
     if stage == "grid" and direction == "ms2dirty":
-        return 24 * M + 8 * C                     # read records, write grid once
+        return "k_grid + k_grid_dense", 1, 24 * M + 8 * C       # read records, write grid once
     if stage == "grid" and direction == "dirty2ms":
-        return 8 * C + 24 * M + 4 * M + 8 * M      # read grid + records + perm, write vis
-    if stage in ("fft_rows", "fft_cols"):
-        return 16 * C                              # read + write every cell
+        return "k_degrid", 1, 8 * C + 24 * M + 4 * M + 8 * M    # grid + records + perm, write vis
+    if stage == "fft_rows":
+        return ("k_fft_rows_ct" if p2(nf1) else "k_fft_rows"), 1, int(8 * C * (1 + f))
+    if stage == "fft_cols":
+        if nf2 > 128:                                            # four-step: two launches
+            return "k_fft_cols_step_ct", 2, int(16 * C * f)
+        return "k_fft_cols", 1, int(16 * C * f)
     if stage == "image" and direction == "ms2dirty":
-        return 8 * plan["nf3"] * npix + 8 * npix   # read P values per pixel, write image
+        return "k_ms2dirty_epi", 1, 8 * nf3 * npix + 8 * npix   # P values per pixel, write image
     if stage == "image" and direction == "dirty2ms":
-        return 8 * npix + 8 * C                    # read image, write every cell
-    return None
+        return "k_dirty2ms_pro", 1, 8 * npix + int(8 * C * f)   # read image, write live columns
+    return None, 1, None
 
 
 def run_cpu_baseline(W, workload):
@@ -171,6 +182,8 @@ def run_cpu_baseline(W, workload):
 
 
 def traffic_from_profiles(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if not os.path.exists(path):
         return None
@@ -262,21 +275,23 @@ def main():
 
     roof = None
     if args.workload != "nufft2d":
+        # the stage with the most time; per-launch figures for its kernel
         best = None
         for d in ("ms2dirty", "dirty2ms"):
             for k in ("grid", "fft_rows", "fft_cols", "image"):
                 if best is None or stage_ms[d][k] > best[2]:
                     best = (d, k, stage_ms[d][k])
         d, k, t_ms = best
-        nbytes = kernel_bytes(k, d, W, plan)
-        achieved = nbytes / (t_ms * 1e-3) / 1e9
-        kname = {"grid": "k_grid" if d == "ms2dirty" else "k_degrid", "fft_rows": "k_fft_rows",
-                 "fft_cols": "k_fft_cols",
-                 "image": "k_ms2dirty_epi" if d == "ms2dirty" else "k_dirty2ms_pro"}[k]
+        kname, launches, nbytes = stage_kernel(k, d, W, plan)
+        t_launch = t_ms / launches
+        achieved = nbytes / (t_launch * 1e-3) / 1e9
+        traffic = None
+        if " + " not in kname:
+            traffic = traffic_from_profiles(args.workload, kname)
         roof = {"kernel": "%s (%s)" % (kname, d), "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(args.workload, kname),
-                "algorithmic_bytes": nbytes, "avg_ms": round(t_ms, 4)}
+                "traffic": traffic, "algorithmic_bytes": nbytes, "avg_ms": round(t_launch, 4),
+                "launches_per_call": launches}
     else:
         t_grid = stage_ms["ms2dirty"]["grid"]
         nbytes = 16 * W["M"] + 8 * (2 * W["N"]) ** 2

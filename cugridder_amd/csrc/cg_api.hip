@@ -37,11 +37,11 @@ enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_FFT2, ST_IMAG
 
 struct Ctx {
     hipStream_t stream = nullptr;
-    int lds_budget = 65536;       // degridding tile + halo staging
+    int lds_budget = 155648;      // degridding tile + halo staging
     int grid_lds_budget = 65536;  // gridding accumulators
     int tile_grid = 8;
-    int tile_degrid = 32;
-    int dense_min = 256;     // points per bin that send it to the MFMA gridding path (0: off)
+    int tile_degrid = 16;
+    int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
     int last_cshift = 0;

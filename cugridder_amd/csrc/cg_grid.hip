@@ -268,8 +268,13 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
     const int half = lane >> 5, i = lane & 31;
     const float beta = (float)g.beta, ec = (float)g.es_c, hw = 0.5f * KW;
     // lane roles: A[i][half] with tap i = t1 + 8 t2 (two 32-tap blocks),
-    // B[half][j] with plane j = i
+    // B[half][j] with plane j = i.  Each lane evaluates one kernel value of
+    // its point (i < 8: x tap i, 8 <= i < 16: y tap i - 8, 16 <= i < 24: z tap
+    // i - 16) and the operands are gathered from those lanes.
     const int t1 = i & 7, t2a = i >> 3, t2b = 4 + (i >> 3);
+    const int src_kx = half * 32 + t1, src_ky0 = half * 32 + 8 + t2a, src_ky1 = half * 32 + 8 + t2b;
+    const int tap = i & 7;
+    const int axis = i >> 3;  // 0 x, 1 y, 2 z, 3 none
 
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int tile = list[1 + li];
@@ -300,22 +305,31 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
             for (int k0 = a; k0 < b; k0 += 2) {
                 const int k = k0 + half;
                 float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f, bz = 0.f;
-                if (k < b) {
-                    const float4 G = geo[k];
-                    const float2 v = val[k];
-                    const float kx = t1 < KW ? es_val<float>(G.x + (float)t1, beta, ec, hw) : 0.f;
-                    const float ky0 = t2a < KW ? es_val<float>(G.y + (float)t2a, beta, ec, hw) : 0.f;
-                    const float ky1 = t2b < KW ? es_val<float>(G.y + (float)t2b, beta, ec, hw) : 0.f;
+                const bool ok = k < b;
+                float4 G = make_float4(0.f, 0.f, 0.f, 0.f);
+                float2 v = make_float2(0.f, 0.f);
+                if (ok) {
+                    G = geo[k];
+                    v = val[k];
+                }
+                // this lane's kernel value, then the operands from the lanes
+                // holding them (every lane takes part in the shuffles)
+                const float off = axis == 0 ? G.x : axis == 1 ? G.y : G.z;
+                const float kv = (ok && axis < 3 && tap < KW) ? es_val<float>(off + (float)tap, beta, ec, hw)
+                                                             : 0.f;
+                const float kx = __shfl(kv, src_kx);
+                const float ky0 = __shfl(kv, src_ky0);
+                const float ky1 = __shfl(kv, src_ky1);
+                int d = i - (int)(__float_as_uint(G.w) >> 16);
+                if (d < 0) d += NP;
+                const float kz = __shfl(kv, half * 32 + 16 + (d & 7));
+                if (ok) {
                     const float w0 = kx * ky0, w1 = kx * ky1;
                     a0r = v.x * w0;
                     a0i = v.y * w0;
                     a1r = v.x * w1;
                     a1i = v.y * w1;
-                    if (i < NP) {
-                        int d = i - (int)(__float_as_uint(G.w) >> 16);
-                        if (d < 0) d += NP;
-                        if (d < KW) bz = es_val<float>(G.z + (float)d, beta, ec, hw);
-                    }
+                    if (i < NP && d < KW) bz = kz;
                 }
                 r0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0r, bz, r0, 0, 0, 0);
                 q0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0i, bz, q0, 0, 0, 0);
@@ -426,7 +440,7 @@ int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, cx_t
 // ------------------------------------------------------------ degridding
 
 template <typename T, int KW, int PPT>
-__global__ __launch_bounds__(256) void k_degrid(const geo_t<T> *__restrict__ geo,
+__global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ geo,
                                                 const cx_t<T> *__restrict__ fac,
                                                 const int32_t *__restrict__ perm,
                                                 const int32_t *__restrict__ tile_start,
@@ -541,8 +555,8 @@ static int degrid_kw(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *per
     if (lds > 65536)
         CG_HIP(hipFuncSetAttribute((const void *)k_degrid<T, KW, 4>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(256), lds, s, geo, fac, perm, ts, fw,
-                       out, g, is2d);
+    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(grid_threads(lds)), lds, s, geo, fac,
+                       perm, ts, fw, out, g, is2d);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
