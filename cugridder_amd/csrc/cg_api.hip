@@ -42,12 +42,13 @@ struct Ctx {
     int tile_grid = 8;
     int tile_degrid = 16;
     int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
+    int row_mask = 1;        // let the FFT skip grid rows no point reaches
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
     int last_cshift = 0;
     // workspace (grow-only)
     DevBuf keys, keys2, vals, vals2, hist, scan, tstart, geo, val, grid, zq, phi3, phi1, phi2,
-        part, tw1, tw2, fftmp, dlist;
+        part, tw1, tw2, fftmp, dlist, rmask;
     int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
     // stage timing (HIP events on the context stream)
     bool timing = false;
@@ -61,7 +62,8 @@ struct Ctx {
     ~Ctx()
     {
         DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
-                          &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist};
+                          &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist,
+                          &rmask};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (ev_ok)
@@ -135,6 +137,18 @@ struct StageClock {
         int _r = (x);          \
         if (_r) return _r;     \
     } while (0)
+
+// rows of the grid that points of non-empty tiles reach (null: all rows);
+// needs the tile starts of the last sort
+static int row_mask(Ctx &C, const Tiling &t, int kw, int nf2, const uint8_t *&mask)
+{
+    mask = nullptr;
+    if (!C.row_mask) return CG_OK;
+    CG_TRY(ensure(C.rmask, (size_t)nf2));
+    CG_TRY(launch_row_mask(as<int32_t>(C.tstart), t, kw, nf2, as<uint8_t>(C.rmask), C.stream));
+    mask = as<uint8_t>(C.rmask);
+    return CG_OK;
+}
 
 // twiddle tables (cached per length and precision)
 template <typename T>
@@ -268,14 +282,17 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     }
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
+    const uint8_t *rm = nullptr;
+    CG_TRY(row_mask(C, t, P.conv.kw, P.nf2, rm));
     CG_TRY(clk.mark(ST_FFT));
-    // only the columns holding image pixels are transformed along n2
+    // only the columns holding image pixels are transformed along n2, and
+    // only the rows points reach hold non-zero data
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, C.stream));
+                              as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_IMAGE));
     const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
@@ -340,14 +357,17 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                                                   as<cx_t<T>>(C.grid), w, C.stream)));
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
+    // the degridder reads only the rows its tiles' points reach
+    const uint8_t *rm = nullptr;
+    if (M > 0) CG_TRY(row_mask(C, t, kw, P.nf2, rm));
     // columns first (only those holding image pixels), then full rows
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
     CG_TRY(clk.mark(ST_FFT));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, C.stream));
+                              as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_GRID));
     GridArgs g = grid_args(P, t, M, a.cshift);
@@ -445,18 +465,20 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     GridArgs g = grid_args(P, t, M, a.cshift);
-    // fast-axis columns that hold modes
+    // fast-axis columns that hold modes; grid rows that points reach
     const int la = N1 - N1 / 2, lb = P.nf1 - N1 / 2;
+    const uint8_t *rm = nullptr;
+    CG_TRY(row_mask(C, t, kw, P.nf2, rm));
     if (type == 1) {
         CG_TRY(clk.mark(ST_GRID));
         CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                               as<cx_t<T>>(C.grid), g, kw, C.stream));
         CG_TRY(clk.mark(ST_FFT));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, true,
-                                  as<cx_t<T>>(C.tw1), la, lb, C.stream));
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
         CG_TRY(clk.mark(ST_FFT2));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, false,
-                                  as<cx_t<T>>(C.tw2), la, lb, C.stream));
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 1, C.stream)));
@@ -466,10 +488,10 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 2, C.stream)));
         CG_TRY(clk.mark(ST_FFT));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, false,
-                                  as<cx_t<T>>(C.tw2), la, lb, C.stream));
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
         CG_TRY(clk.mark(ST_FFT2));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, true,
-                                  as<cx_t<T>>(C.tw1), la, lb, C.stream));
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
@@ -727,6 +749,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
+    else if (!std::strcmp(name, "row_mask")) C->row_mask = value;
     else return CG_ERR_ARG;
     return CG_OK;
 }

@@ -499,7 +499,8 @@ constexpr int ct_rmax() { return sizeof(T) == 4 ? 16 : 8; }
 // persistent over the nrows rows of all planes
 template <typename T, int DIR, int LOGN>
 __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t<T> *data, int64_t nrows,
-                                                      const cx_t<T> *__restrict__ tw, Live live)
+                                                      const cx_t<T> *__restrict__ tw, Live live,
+                                                      const uint8_t *__restrict__ rowmask, int n2)
 {
     constexpr int N = 1 << LOGN;
     constexpr int EPT = fft_ept<T>();
@@ -510,6 +511,8 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     const TwTab<T> tab = load_twtab<T>(buf + E, tw, N, SH);
     for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
+        // a masked row is zero (dir > 0) or unread (dir < 0): skip it
+        if (rowmask && !rowmask[g % n2]) continue;
         const cx_t<T> *src = data + g * (int64_t)N;
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
@@ -533,7 +536,8 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
 template <typename T, int DIR, int LOGL, int CB>
 __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_step_ct(cx_t<T> *data, int n2, int n1, int se, int sj,
                                                            int twmode, const cx_t<T> *__restrict__ tw2,
-                                                           Live live)
+                                                           Live live, const uint8_t *__restrict__ rowmask,
+                                                           int maskmode)
 {
     constexpr int L = 1 << LOGL;
     constexpr int EPT = fft_ept<T>();
@@ -557,7 +561,10 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
         const int k = threadIdx.x + u * NT;
         const int e = k / CB, b = k % CB;
         const int c = c0 + b;
-        cx_t<T> v = c < nlive ? pl[(row0 + (size_t)e * se) * n1 + live.at(c)] : mkc((T)0, (T)0);
+        const size_t row = row0 + (size_t)e * se;
+        // maskmode 1: rows with mask 0 are known zeros, not read
+        const bool rd = c < nlive && !(maskmode == 1 && !rowmask[row]);
+        cx_t<T> v = rd ? pl[row * n1 + live.at(c)] : mkc((T)0, (T)0);
         if (twmode == 2) {
             cx_t<T> w = tab.get(j * e);
             if (DIR < 0) w.y = -w.y;
@@ -573,13 +580,16 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
         const int e = k / CB, b = k % CB;
         const int c = c0 + b;
         if (c >= nlive) continue;
+        const size_t row = row0 + (size_t)e * se;
+        // maskmode 2: rows with mask 0 are not needed downstream
+        if (maskmode == 2 && !rowmask[row]) continue;
         cx_t<T> v = buf[k];
         if (twmode == 1) {
             cx_t<T> w = tab.get(j * e);
             if (DIR < 0) w.y = -w.y;
             v = cmul<T>(v, w);
         }
-        pl[(row0 + (size_t)e * se) * n1 + live.at(c)] = v;
+        pl[row * n1 + live.at(c)] = v;
     }
 }
 
@@ -676,7 +686,8 @@ static int log2_exact(int n)
 
 // compile-time row kernel for n1 = 2^LOGN (returns 1 if LOGN is not covered)
 template <typename T, int DIR, int LOGN>
-static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live, hipStream_t s)
+static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live,
+                   const uint8_t *rowmask, int n2, hipStream_t s)
 {
     constexpr int N = 1 << LOGN;
     constexpr int NT = N / fft_ept<T>();
@@ -693,7 +704,7 @@ static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live, h
         per_cu = std::min(per_cu, std::max(1, 2048 / NT));
         const int64_t grid = std::min<int64_t>(nrows, (int64_t)256 * per_cu);
         hipLaunchKernelGGL((k_fft_rows_ct<T, DIR, LOGN>), dim3((unsigned)grid), dim3(NT), lds, s, data,
-                           nrows, tw, live);
+                           nrows, tw, live, rowmask, n2);
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
@@ -701,15 +712,15 @@ static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live, h
 
 template <typename T, int DIR>
 static int rows_ct_dispatch(cx_t<T> *data, int64_t nrows, int n1, const cx_t<T> *tw, Live live,
-                            hipStream_t s)
+                            const uint8_t *m, int n2, hipStream_t s)
 {
     switch (log2_exact(n1)) {
-    case 9: return rows_ct<T, DIR, 9>(data, nrows, tw, live, s);
-    case 10: return rows_ct<T, DIR, 10>(data, nrows, tw, live, s);
-    case 11: return rows_ct<T, DIR, 11>(data, nrows, tw, live, s);
-    case 12: return rows_ct<T, DIR, 12>(data, nrows, tw, live, s);
-    case 13: return rows_ct<T, DIR, 13>(data, nrows, tw, live, s);
-    case 14: return rows_ct<T, DIR, 14>(data, nrows, tw, live, s);
+    case 9: return rows_ct<T, DIR, 9>(data, nrows, tw, live, m, n2, s);
+    case 10: return rows_ct<T, DIR, 10>(data, nrows, tw, live, m, n2, s);
+    case 11: return rows_ct<T, DIR, 11>(data, nrows, tw, live, m, n2, s);
+    case 12: return rows_ct<T, DIR, 12>(data, nrows, tw, live, m, n2, s);
+    case 13: return rows_ct<T, DIR, 13>(data, nrows, tw, live, m, n2, s);
+    case 14: return rows_ct<T, DIR, 14>(data, nrows, tw, live, m, n2, s);
     default: return 1;
     }
 }
@@ -717,7 +728,8 @@ static int rows_ct_dispatch(cx_t<T> *data, int64_t nrows, int n1, const cx_t<T> 
 // compile-time column step for sub-transforms of L = 2^LOGL (1 if not covered)
 template <typename T, int DIR, int LOGL>
 static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
-                        int twmode, const cx_t<T> *tw2, Live live, hipStream_t s)
+                        int twmode, const cx_t<T> *tw2, Live live, const uint8_t *rowmask,
+                        int maskmode, hipStream_t s)
 {
     constexpr int L = 1 << LOGL;
     constexpr int ELEM = (int)sizeof(cx_t<T>);
@@ -734,7 +746,7 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
         const int nlive = live.count(n1);
         dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)nsub, (unsigned)batch);
         hipLaunchKernelGGL((k_fft_cols_step_ct<T, DIR, LOGL, CB>), grid, dim3(NT), lds, s, data, n2,
-                           n1, se, sj, twmode, tw2, live);
+                           n1, se, sj, twmode, tw2, live, rowmask, rowmask ? maskmode : 0);
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
@@ -742,24 +754,25 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
 
 template <typename T, int DIR>
 static int cols_step_ct_dispatch(cx_t<T> *data, int batch, int n2, int n1, int L, int se, int sj,
-                                 int nsub, int twmode, const cx_t<T> *tw2, Live live, hipStream_t s)
+                                 int nsub, int twmode, const cx_t<T> *tw2, Live live,
+                                 const uint8_t *m, int mm, hipStream_t s)
 {
     switch (log2_exact(L)) {
-    case 4: return cols_step_ct<T, DIR, 4>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
-    case 5: return cols_step_ct<T, DIR, 5>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
-    case 6: return cols_step_ct<T, DIR, 6>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
-    case 7: return cols_step_ct<T, DIR, 7>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
-    case 8: return cols_step_ct<T, DIR, 8>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, s);
+    case 4: return cols_step_ct<T, DIR, 4>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
+    case 5: return cols_step_ct<T, DIR, 5>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
+    case 6: return cols_step_ct<T, DIR, 6>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
+    case 7: return cols_step_ct<T, DIR, 7>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
+    case 8: return cols_step_ct<T, DIR, 8>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
     default: return 1;
     }
 }
 
 template <typename T, int DIR>
 static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw, Live live,
-                    hipStream_t s)
+                    const uint8_t *rowmask, hipStream_t s)
 {
     if (fft_knob("CG_FFT_CT", 1)) {
-        const int r = rows_ct_dispatch<T, DIR>(data, (int64_t)n2 * batch, n1, tw, live, s);
+        const int r = rows_ct_dispatch<T, DIR>(data, (int64_t)n2 * batch, n1, tw, live, rowmask, n2, s);
         if (r != 1) return r;
     }
     FftShape S{};
@@ -801,7 +814,7 @@ static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw,
 
 template <typename T, int DIR>
 static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2, Live live,
-                    hipStream_t s)
+                    const uint8_t *rowmask, hipStream_t s)
 {
     const int elem = (int)sizeof(cx_t<T>);
     const int nlive = live.count(n1);
@@ -834,10 +847,13 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         return CG_OK;
     }
     const bool ct = fft_knob("CG_FFT_CT", 1) != 0;
+    // the natural-order step (the first one forward, the last one inverse)
+    // reads / writes only the masked-in rows
     auto step = [&](int L, int se, int sj, int nsub, int twmode) -> int {
         if (ct) {
+            const int mm = sj == 1 ? (DIR > 0 ? 1 : 2) : 0;
             const int r = cols_step_ct_dispatch<T, DIR>(data, batch, n2, n1, L, se, sj, nsub, twmode,
-                                                        tw2, live, s);
+                                                        tw2, live, rowmask, mm, s);
             if (r != 1) return r;
         }
         FftShape S{};
@@ -879,15 +895,16 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
 
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
-                    const cx_t<T> *tw, int live_a, int live_b, hipStream_t s)
+                    const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
+                    hipStream_t s)
 {
     if (batch <= 0) return CG_OK;
     Live live{live_a, live_b};
-    if (rows) return dir > 0 ? fft_rows<T, 1>(data, batch, n2, n1, tw, live, s)
-                             : fft_rows<T, -1>(data, batch, n2, n1, tw, live, s);
+    if (rows) return dir > 0 ? fft_rows<T, 1>(data, batch, n2, n1, tw, live, rowmask, s)
+                             : fft_rows<T, -1>(data, batch, n2, n1, tw, live, rowmask, s);
     if (n2 <= 1) return CG_OK;
-    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, s)
-                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, s);
+    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, rowmask, s)
+                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, rowmask, s);
 }
 
 template <typename T>
@@ -912,14 +929,14 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
     if (split && !tmp) return CG_ERR_ARG;
     int r;
     if (dir > 0) {
-        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, s))) return r;
-        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, s))) return r;
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, s))) return r;
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, s))) return r;
         if (split) return col_permute<T>(data, tmp, batch, n2, n1, na, nb, 1, s);
         return CG_OK;
     }
     if (split && (r = col_permute<T>(data, tmp, batch, n2, n1, na, nb, 0, s))) return r;
-    if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, s))) return r;
-    return launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, s);
+    if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, s))) return r;
+    return launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, s);
 }
 
 template <typename T>
@@ -942,8 +959,8 @@ int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s)
 
 template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, float2 *, hipStream_t);
 template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, double2 *, hipStream_t);
-template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, hipStream_t);
-template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, hipStream_t);
+template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, hipStream_t);
+template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, hipStream_t);
 template bool fft_supported<float>(int);
 template bool fft_supported<double>(int);
 template int launch_twiddles<float>(float2 *, int, hipStream_t);

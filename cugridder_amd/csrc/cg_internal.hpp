@@ -40,6 +40,10 @@ int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
 template <typename TU>
 int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a,
                    uint32_t *keys, int32_t *vals, hipStream_t s);
+// mask[y] (nf2 bytes) = 1 for the grid rows that points of non-empty tiles
+// reach; the other rows are zero after gridding and unread by degridding
+int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
+                    hipStream_t s);
 // tile_start[t] = first sorted position with key >> shift >= t, t in [0, ntiles]
 int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
                        int32_t *tile_start, hipStream_t s);
@@ -110,9 +114,13 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
 // only those inputs.  A dir > 0 column pass leaves a split column axis in
 // split order (frequency ka + na kb at row ka nb + kb, see fft_col_split);
 // a dir < 0 column pass expects that order.
+// rowmask (nf2 bytes, may be null): rows with mask 0 are zero on input to a
+// dir > 0 pass and not needed on output of a dir < 0 pass, so they may be
+// skipped (the row pass and the natural-order column step use it).
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
-                    const cx_t<T> *tw, int live_a, int live_b, hipStream_t s);
+                    const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
+                    hipStream_t s);
 // n = na * nb when columns of length n are transformed by the split
 bool fft_col_split(int n, int &na, int &nb);
 template <typename T>

@@ -264,6 +264,44 @@ int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, i
     return CG_OK;
 }
 
+// ------------------------------------------------------------ row mask
+
+// mask[y] = 1 for every grid row y (along nf2) that the footprint of a point
+// of a non-empty tile can reach: rows [y0, y0 + ty + kw - 1) of the tile,
+// periodic.  All other rows of the grid are zero (ms2dirty) or never read
+// (dirty2ms), so the FFT passes may skip them.
+__global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_row_mask(const int32_t *__restrict__ tile_start,
+                                                  int64_t ntiles, int ntx, int ty, int kw, int nf2,
+                                                  uint8_t *__restrict__ mask)
+{
+    const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (t >= ntiles || tile_start[t + 1] == tile_start[t]) return;
+    const int y0 = (int)(t / ntx) * ty;
+    const int span = min(ty + kw - 1, nf2);
+    for (int r = 0; r < span; r++) {
+        int y = y0 + r;
+        if (y >= nf2) y -= nf2;
+        mask[y] = 1;
+    }
+}
+
+int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(nf2, 256)), dim3(256), 0, s, mask, nf2);
+    const int64_t ntiles = t.ntiles();
+    hipLaunchKernelGGL(k_row_mask, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, tile_start,
+                       ntiles, t.ntx, t.ty, kw, nf2, mask);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
 // ------------------------------------------------------------ records
 
 template <typename T> __device__ __forceinline__ geo_t<T> make_geo(double d1, double d2, double d3, uint32_t pk);

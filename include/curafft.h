@@ -104,8 +104,9 @@ int cg_ctx_destroy(void *ctx);
  * staging bytes), "grid_lds_budget" (gridding accumulator bytes),
  * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
  * "dense_min" (points per gridding bin from which the complex64 gridder
- * uses per-start-cell MFMA products; 0 disables), "stop_after" (test hook,
- * see below) */
+ * uses per-start-cell MFMA products; 0 disables), "row_mask" (1: the FFT
+ * skips grid rows that no point reaches; 0: transforms every row),
+ * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
 /* per-stage times summed over calls, out[14] = {ms2dirty[7], dirty2ms[7]}
  * in ms, stages: plan, sort, records, grid/degrid, fft rows, fft columns,
