@@ -93,6 +93,36 @@ def make_wstack_full(torch, dev, nrow=1 << 27, npix=8192, seed=3):
                 eps=1e-6, M=nrow)
 
 
+def make_batch(torch, dev, nchan=64, seed=4):
+    """64 independent channels: visibility counts floor(2^U), U ~ U(18, 23);
+    uvw of the full W-stacking tracks (metres) observed at frequencies
+    1.0 .. 1.6 GHz; images drawn from {512, 1024, 2048, 4096}^2 at the full
+    configuration's pixel size (3 deg / 8192); complex64, eps 1e-5."""
+    base = make_wstack_full(torch, dev, nrow=1 << 23, npix=64, seed=seed)
+    rng = np.random.default_rng(seed)
+    counts = np.floor(2.0 ** rng.uniform(18, 23, nchan)).astype(np.int64)
+    freqs = np.linspace(1.0e9, 1.6e9, nchan)
+    sizes = rng.choice([512, 1024, 2048, 4096], nchan)
+    pix_deg = 3.0 / 8192
+    items = []
+    for c in range(nchan):
+        m = int(counts[c])
+        start = int(rng.integers(0, (1 << 23) - m + 1))
+        n = int(sizes[c])
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed * 1000 + c)
+        vis = torch.complex(torch.randn(m, generator=g, device=dev),
+                            torch.randn(m, generator=g, device=dev))
+        img = torch.complex(torch.randn(n, n, generator=g, device=dev), torch.zeros(n, n, device=dev))
+        items.append(dict(uvw=base["uvw"][start:start + m], vis=vis, wgt=None, img=img,
+                          dirty=torch.empty(n, n, dtype=torch.complex64, device=dev),
+                          out=torch.empty(m, dtype=torch.complex64, device=dev),
+                          fov=pix_deg * n, freq=float(freqs[c])))
+    del base
+    return dict(name="batch of %d channels, 2^18..2^23 rows, images 512^2..4096^2, 1.0-1.6 GHz, "
+                     "eps 1e-5" % nchan, items=items, eps=1e-5, M=int(counts.sum()))
+
+
 def make_nufft2d(torch, dev, M=1 << 25, N=4096, seed=2):
     """2^25 points in [-pi,pi)^2: 80% from 16 Gaussians (means uniform,
     std 0.3, wrapped) + 20% uniform; strengths complex64 N(0,1); 4096^2."""
@@ -200,7 +230,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d"), default="wstack")
+    ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d", "batch"), default="wstack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nrow", type=int, default=1 << 27, help="rows of the wstack workload")
     ap.add_argument("--copy-probe", action="store_true",
@@ -219,6 +249,8 @@ def main():
     log("generating workload %s" % args.workload)
     if args.workload == "plain":
         W = make_plain(torch, dev)
+    elif args.workload == "batch":
+        W = make_batch(torch, dev)
     elif args.workload == "wstack":
         W = make_wstack_full(torch, dev, nrow=args.nrow)
     else:
@@ -233,7 +265,34 @@ def main():
         log("copy probe: 1 GiB read + 1 GiB written")
     ctx = Context(timing=True)
 
-    if args.workload == "nufft2d":
+    single_ms = None
+    if args.workload == "batch":
+        # the same channels one call at a time (context defaults, no graph)
+        def single_step():
+            for it in W["items"]:
+                ctx.ms2dirty(it["uvw"], it["vis"], it["wgt"], *it["dirty"].shape, it["fov"],
+                             it["freq"], W["eps"], out=it["dirty"])
+                ctx.dirty2ms(it["uvw"], it["img"], it["wgt"], it["fov"], it["freq"], W["eps"],
+                             out=it["out"])
+        single_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        single_step()
+        torch.cuda.synchronize()
+        single_ms = (time.perf_counter() - t0) * 1e3
+        log("one call per channel: %.1f ms per step" % single_ms)
+        tb = time.perf_counter()
+        fw = [dict(it, dirty=it["dirty"]) for it in W["items"]]
+        bw = [dict(it, dirty=it["img"], vis=it["out"]) for it in W["items"]]
+        b1 = ctx.batch("ms2dirty", fw, W["eps"])
+        b2 = ctx.batch("dirty2ms", bw, W["eps"])
+        torch.cuda.synchronize()
+        log("batch build (autotune + graph capture): %.1f s" % (time.perf_counter() - tb))
+
+        def step():
+            b1.run()
+            b2.run()
+    elif args.workload == "nufft2d":
         out_fk = torch.empty(W["N"], W["N"], dtype=torch.complex64, device=dev)
         out_c = torch.empty(W["M"], dtype=torch.complex64, device=dev)
 
@@ -274,7 +333,9 @@ def main():
                                               for d, s in stage_ms.items()}))
 
     roof = None
-    if args.workload != "nufft2d":
+    if args.workload == "batch":
+        pass
+    elif args.workload != "nufft2d":
         # the stage with the most time; per-launch figures for its kernel
         best = None
         for d in ("ms2dirty", "dirty2ms"):
@@ -310,7 +371,14 @@ def main():
     unit = "Mpts/s" if args.workload == "nufft2d" else "Mvis/s"
     cfg = {"workload": W["name"], "nrow": W["M"], "eps": W["eps"],
            "precision": "complex64 gridding/FFT, FP64 coordinates, phases and corrections"}
-    if args.workload != "nufft2d":
+    if args.workload == "batch":
+        info, is_graph = b1.info()
+        cfg.update({"channels": len(W["items"]), "graph": is_graph,
+                    "one_call_per_channel_ms": round(single_ms, 2),
+                    "one_call_per_channel_Mvis_s": round(W["M"] / (single_ms * 1e-3) / 1e6, 3),
+                    "tuned_tile_grid": sorted(set(int(x) for x in info[:, 0])),
+                    "tuned_dense_min": sorted(set(int(x) for x in info[:, 2]))})
+    elif args.workload != "nufft2d":
         cfg.update({"image": "%dx%d" % (W["nx"], W["ny"]), "fov_deg": W["fov"], "freq_hz": W["freq"],
                     "nf": [plan["nf1"], plan["nf2"], plan["nf3"]], "kw": plan["kw"]})
     out = {"metric": METRIC, "value": round(value, 3), "unit": unit, "n_gpus": 1,

@@ -34,6 +34,17 @@ def _nd_or_none(dtype):
     return _P
 
 
+class BatchItem(ctypes.Structure):
+    """cg_batch_item of include/curafft.h."""
+    _fields_ = [("nrow", ctypes.c_int), ("nxdirty", ctypes.c_int), ("nydirty", ctypes.c_int),
+                ("fov", ctypes.c_double), ("freq", ctypes.c_double), ("d_uvw", ctypes.c_void_p),
+                ("d_vis", ctypes.c_void_p), ("d_wgt", ctypes.c_void_p), ("d_dirty", ctypes.c_void_p)]
+
+
+BATCH_AUTOTUNE = 1
+BATCH_GRAPH = 2
+
+
 def lib():
     """The loaded libcurafft.so with argtypes declared."""
     global _lib
@@ -75,13 +86,23 @@ def lib():
                                  c_double, c_double, c_int]
     L.dirty2ms_mixed.argtypes = [c_int, c_int, c_int, c_double, c_double, f64, c64, f32n, c64,
                                  c_double, c_double, c_int]
+    # several channels: freq[nchan], vis/wgt [nrow][nchan]
+    L.ms2dirty_mc_2.argtypes = [c_int, c_int, c_int, c_int, c_double, f64, f64, c128, f64n, c128,
+                                c_double, c_double, c_int]
+    L.dirty2ms_mc_2.argtypes = [c_int, c_int, c_int, c_int, c_double, f64, f64, c128, f64n, c128,
+                                c_double, c_double, c_int]
+    L.ms2dirty_mc_mixed.argtypes = [c_int, c_int, c_int, c_int, c_double, f64, f64, c64, f32n,
+                                    c64, c_double, c_double, c_int]
+    L.dirty2ms_mc_mixed.argtypes = [c_int, c_int, c_int, c_int, c_double, f64, f64, c64, f32n,
+                                    c64, c_double, c_double, c_int]
     L.nufft2d1.argtypes = [c_int, f64, f64, c128, c_int, c_int, c128, c_double, c_double]
     L.nufft2d2.argtypes = [c_int, f64, f64, c128, c_int, c_int, c128, c_double, c_double]
     L.nufft2d1f.argtypes = [c_int, f32, f32, c64, c_int, c_int, c64, c_float, c_float]
     L.nufft2d2f.argtypes = [c_int, f32, f32, c64, c_int, c_int, c64, c_float, c_float]
     for name in ("ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2", "ms2dirtyf_1",
                  "ms2dirtyf_2", "dirty2msf_1", "dirty2msf_2", "ms2dirty_mixed",
-                 "dirty2ms_mixed", "nufft2d1", "nufft2d2", "nufft2d1f", "nufft2d2f"):
+                 "dirty2ms_mixed", "ms2dirty_mc_2", "dirty2ms_mc_2", "ms2dirty_mc_mixed",
+                 "dirty2ms_mc_mixed", "nufft2d1", "nufft2d2", "nufft2d1f", "nufft2d2f"):
         getattr(L, name).restype = c_int
     # contexts / device pointers
     L.cg_ctx_create.argtypes = [ctypes.POINTER(c_void_p), c_void_p]
@@ -96,8 +117,19 @@ def lib():
     L.cg_dirty2ms_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                   c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                   c_double, c_double, c_int]
+    L.cg_ms2dirty_mc_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, f64,
+                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                     c_double, c_double, c_int]
+    L.cg_dirty2ms_mc_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, f64,
+                                     c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                     c_double, c_double, c_int]
     L.cg_nufft2d_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_int, c_void_p, c_double, c_double]
+    L.cg_batch_create.argtypes = [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int,
+                                  ctypes.POINTER(BatchItem), c_double, c_double, c_int, c_int]
+    L.cg_batch_run.argtypes = [c_void_p]
+    L.cg_batch_info.argtypes = [c_void_p, f64]
+    L.cg_batch_destroy.argtypes = [c_void_p]
     L.cg_plan_query.argtypes = [c_int, c_int, c_int, c_double, c_double, c_double, c_double,
                                 c_double, c_double, c_int, c_int, f64]
     L.cg_ctx_grid.argtypes = [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(ctypes.c_longlong)]
@@ -107,11 +139,20 @@ def lib():
     L.cg_sort_pairs_dev.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int]
     L.cg_memcpy_dev.argtypes = [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]
     L.cg_fft_col_split.argtypes = [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    # learned kernel
+    L.cg_learn_grad.argtypes = [c_int, f32, f64, f64, c_int, f32, f64]
+    L.cg_learn_train.argtypes = [c_int, c_int, f64, f64, c_int, c_int, c_int, c_double, f32, f64n]
+    L.cg_learn_eval.argtypes = [f32, f32, c_int, f32]
+    L.cg_learn_tabulate.argtypes = [c_int, c_int, f32, f32, f64]
+    L.cg_ctx_set_kernel.argtypes = [c_void_p, c_int, c_int, f32n, c_double]
     for name in ("cg_ctx_create", "cg_ctx_destroy", "cg_ctx_set_option", "cg_ctx_stage_times",
                  "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
+                 "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_batch_create", "cg_batch_run",
+                 "cg_batch_info", "cg_batch_destroy",
                  "cg_nufft2d_dev", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
                  "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split",
-                 "cg_version"):
+                 "cg_version", "cg_learn_nparams", "cg_learn_grad", "cg_learn_train",
+                 "cg_learn_eval", "cg_learn_tabulate", "cg_ctx_set_kernel"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

@@ -40,6 +40,16 @@ class Context:
     def set_option(self, name, value):
         check(lib().cg_ctx_set_option(self._h, name.encode(), int(value)), "set_option " + name)
 
+    def set_kernel(self, kw, params, eps, deg=None):
+        """Installs the learned kernel of support kw (1153 FP32 parameters),
+        certified by the caller at relative error eps; eps <= 0 removes it.
+        complex64 / mixed calls at epsilon >= eps whose ES kernel is wider
+        use it."""
+        from .learned import default_degree
+        deg = default_degree(kw) if deg is None else deg
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        check(lib().cg_ctx_set_kernel(self._h, int(kw), int(deg), p, float(eps)), "set_kernel")
+
     def close(self):
         if self._h:
             lib().cg_ctx_destroy(self._h)
@@ -65,31 +75,48 @@ class Context:
         wp = prec if wgt is None else (1 if wgt.dtype == torch.float64 else 0)
         return prec, up, wp
 
+    @staticmethod
+    def _freqs(freq):
+        f = np.ascontiguousarray(np.atleast_1d(np.asarray(freq, dtype=np.float64)).ravel())
+        if f.size == 0:
+            raise ValueError("freq is empty")
+        return f
+
     def ms2dirty(self, uvw, vis, wgt, nxdirty, nydirty, fov, freq, epsilon, sigma=2.0, sign=-1,
                  out=None):
-        """Complex dirty image [nx, ny] from device visibilities."""
+        """Complex dirty image [nx, ny] from device visibilities.  freq is one
+        frequency (vis [nrow]) or nchan of them (vis, wgt [nrow, nchan]); all
+        channels are summed into the image."""
         torch = self._torch
         prec, up, wp = self._prec(vis.dtype, uvw.dtype, wgt)
+        f = self._freqs(freq)
         if out is None:
             out = torch.empty((nxdirty, nydirty), dtype=vis.dtype, device=vis.device)
-        ier = lib().cg_ms2dirty_dev(self._h, prec, uvw.shape[0], nxdirty, nydirty, float(fov),
-                                    float(freq), _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
-                                    _ptr(out), float(epsilon), float(sigma), int(sign))
+        ier = lib().cg_ms2dirty_mc_dev(self._h, prec, uvw.shape[0], f.size, nxdirty, nydirty,
+                                       float(fov), f, _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
+                                       _ptr(out), float(epsilon), float(sigma), int(sign))
         check(ier, "ms2dirty")
         return out
 
     def dirty2ms(self, uvw, dirty, wgt, fov, freq, epsilon, sigma=2.0, sign=-1, out=None):
-        """Complex visibilities [nrow] from a device image [nx, ny]."""
+        """Complex visibilities from a device image [nx, ny]: [nrow] for one
+        frequency, [nrow, nchan] for nchan of them."""
         torch = self._torch
         prec, up, wp = self._prec(dirty.dtype, uvw.dtype, wgt)
+        f = self._freqs(freq)
         nx, ny = dirty.shape
         if out is None:
-            out = torch.empty(uvw.shape[0], dtype=dirty.dtype, device=dirty.device)
-        ier = lib().cg_dirty2ms_dev(self._h, prec, uvw.shape[0], nx, ny, float(fov), float(freq),
-                                    _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
-                                    float(epsilon), float(sigma), int(sign))
+            shape = (uvw.shape[0],) if f.size == 1 else (uvw.shape[0], f.size)
+            out = torch.empty(shape, dtype=dirty.dtype, device=dirty.device)
+        ier = lib().cg_dirty2ms_mc_dev(self._h, prec, uvw.shape[0], f.size, nx, ny, float(fov), f,
+                                       _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
+                                       float(epsilon), float(sigma), int(sign))
         check(ier, "dirty2ms")
         return out
+
+    def batch(self, direction, items, epsilon, sigma=2.0, sign=-1, autotune=True, graph=True):
+        """A Batch of independent channels on this context's stream."""
+        return Batch(self, direction, items, epsilon, sigma, sign, autotune, graph)
 
     def nufft2d1(self, x, y, c, N1, N2, epsilon, sigma=2.0, out=None):
         torch = self._torch
@@ -204,3 +231,58 @@ def _memcpy_d2d(dst, src, nbytes):
     check(lib().cg_memcpy_dev(ctypes.c_void_p(dst), ctypes.c_void_p(src),
                               ctypes.c_void_p(s.cuda_stream), nbytes), "memcpy")
     s.synchronize()
+
+
+class Batch:
+    """Independent channels run as one launch (cg_batch_*): items are dicts
+    with uvw (float64 [nrow, 3]), vis (complex64 [nrow]), wgt (float32 [nrow]
+    or None), dirty (complex64 [nx, ny]), fov (deg) and freq (Hz) as device
+    tensors / numbers.  direction "ms2dirty" writes every dirty, "dirty2ms"
+    every vis.  The tensors must stay alive and unchanged in shape while the
+    batch exists."""
+
+    def __init__(self, ctx, direction, items, epsilon, sigma=2.0, sign=-1, autotune=True,
+                 graph=True):
+        from ._lib import BATCH_AUTOTUNE, BATCH_GRAPH, BatchItem
+        torch = ctx._torch
+        self._ctx = ctx
+        self.items = list(items)
+        arr = (BatchItem * len(self.items))()
+        for k, it in enumerate(self.items):
+            if it["uvw"].dtype != torch.float64 or it["vis"].dtype != torch.complex64 or \
+                    it["dirty"].dtype != torch.complex64:
+                raise TypeError("batch items are float64 uvw with complex64 vis and dirty")
+            if it.get("wgt") is not None and it["wgt"].dtype != torch.float32:
+                raise TypeError("batch weights are float32")
+            nx, ny = it["dirty"].shape
+            arr[k] = BatchItem(it["uvw"].shape[0], nx, ny, float(it["fov"]), float(it["freq"]),
+                               _ptr(it["uvw"]), _ptr(it["vis"]), _ptr(it.get("wgt")),
+                               _ptr(it["dirty"]))
+        self._arr = arr
+        flags = (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GRAPH if graph else 0)
+        self._h = ctypes.c_void_p()
+        d = {"ms2dirty": 0, "dirty2ms": 1}[direction]
+        check(lib().cg_batch_create(ctypes.byref(self._h), ctx._h, d, len(self.items), arr,
+                                    float(epsilon), float(sigma), int(sign), flags),
+              "cg_batch_create")
+
+    def run(self):
+        check(lib().cg_batch_run(self._h), "cg_batch_run")
+
+    def info(self):
+        """Per item: tile_grid, tile_degrid, dense threshold, tuned ms, nf3;
+        and whether the batch runs as a graph."""
+        out = np.zeros(5 * len(self.items))
+        g = lib().cg_batch_info(self._h, out)
+        return out.reshape(-1, 5), bool(g)
+
+    def close(self):
+        if self._h:
+            lib().cg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

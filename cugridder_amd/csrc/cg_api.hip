@@ -48,7 +48,7 @@ struct Ctx {
     int last_cshift = 0;
     // workspace (grow-only)
     DevBuf keys, keys2, vals, vals2, hist, scan, tstart, geo, val, grid, zq, phi3, phi1, phi2,
-        part, tw1, tw2, fftmp, dlist, rmask;
+        part, tw1, tw2, fftmp, dlist, rmask, chans;
     int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
     // stage timing (HIP events on the context stream)
     bool timing = false;
@@ -58,14 +58,24 @@ struct Ctx {
     int64_t stage_calls[2] = {0, 0};
     WPlan last_plan;
     Tiling last_tiling;
+    // learned kernels by support width: used (single / mixed precision) when
+    // certified at an eps no larger than the call's and narrower than ES's
+    struct Learned {
+        double eps = 0;  // 0: none
+        LearnedKernel K;
+        DevBuf coef;
+    } learned[LEARN_MAXKW + 1];
+    int use_learned = 1;
 
     ~Ctx()
     {
         DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
                           &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist,
-                          &rmask};
+                          &rmask, &chans};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
+        for (auto &L : learned)
+            if (L.coef.p) (void)hipFree(L.coef.p);
         if (ev_ok)
             for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -132,12 +142,6 @@ struct StageClock {
     }
 };
 
-#define CG_TRY(x)              \
-    do {                       \
-        int _r = (x);          \
-        if (_r) return _r;     \
-    } while (0)
-
 // rows of the grid that points of non-empty tiles reach (null: all rows);
 // needs the tile starts of the last sort
 static int row_mask(Ctx &C, const Tiling &t, int kw, int nf2, const uint8_t *&mask)
@@ -192,9 +196,14 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     return CG_OK;
 }
 
-static GridArgs grid_args(const WPlan &P, const Tiling &t, int64_t m, int cshift)
+static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, int cshift)
 {
     GridArgs g{};
+    if (P.conv.tab_nq > 0) {
+        const Ctx::Learned &L = C.learned[P.conv.kw];
+        g.kpoly = as<float>(const_cast<DevBuf &>(L.coef));
+        g.kdeg = L.K.deg;
+    }
     g.cshift = cshift;
     g.m = (int)m;
     g.nf1 = P.nf1;
@@ -213,54 +222,138 @@ static GridArgs grid_args(const WPlan &P, const Tiling &t, int64_t m, int cshift
     return g;
 }
 
+// Plan for nrow rows observed at nchan frequencies (host array freq): the
+// w-range of the points is the raw w-range of the rows scaled per channel
+// (exact: an FP64 product is monotonic in each factor).
 template <typename T, typename TU>
-static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nx, int ny, double fov, double freq,
-                     const TU *uvw, double eps, double sigma, int sign, bool single)
+static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *freq, int nx,
+                     int ny, double fov, const TU *uvw, double eps, double sigma, int sign,
+                     bool single)
 {
-    if (nrow < 0 || nrow > (int64_t)INT32_MAX - 1) return CG_ERR_SIZE;
+    if (nrow < 0 || nchan < 1 || !freq) return CG_ERR_ARG;
+    if (nrow * (int64_t)nchan > (int64_t)INT32_MAX - 1) return CG_ERR_SIZE;
+    for (int c = 0; c < nchan; c++)
+        if (!(freq[c] > 0)) return CG_ERR_ARG;
     CG_TRY(ensure(C.part, sizeof(double) * 2056));
-    double wmin = 0, wmax = 0;
-    const double scale_w = 2.0 * kPi * (freq / kC) * sign;
-    CG_TRY(launch_wminmax<TU>(uvw, nrow, scale_w, as<double>(C.part), wmin, wmax, C.stream));
-    int ier = make_wplan(P, nrow, nx, ny, fov, freq, eps, sigma, sign, single, wmin, wmax);
+    double wlo = 0, whi = 0, wmin = 0, wmax = 0;
+    CG_TRY(launch_wminmax<TU>(uvw, nrow, 1.0, as<double>(C.part), wlo, whi, C.stream));
+    if (nrow > 0) {
+        wmin = INFINITY;
+        wmax = -INFINITY;
+        for (int c = 0; c < nchan; c++) {
+            const double s = 2.0 * kPi * (freq[c] / kC) * sign;
+            const double a = wlo * s, b = whi * s;
+            wmin = std::min(wmin, std::min(a, b));
+            wmax = std::max(wmax, std::max(a, b));
+        }
+    }
+    int ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin, wmax);
     if (ier >= 2) return ier;
+    // the narrowest learned kernel certified for eps and no wider than the
+    // ES kernel (at equal support it is the more accurate one)
+    if (sizeof(T) == 4 && C.use_learned) {
+        for (int kw = 2; kw <= P.conv.kw; kw++) {
+            const Ctx::Learned &L = C.learned[kw];
+            if (!(L.eps > 0) || L.eps > eps) continue;
+            ConvOpts o = P.conv;
+            o.kw = kw;
+            o.halfwidth = kw / 2.0;
+            o.tab_nq = L.K.nq;
+            for (int q = 0; q < L.K.nq; q++) o.tab_psi[q] = L.K.qpsi[q];
+            ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin,
+                             wmax, &o);
+            if (ier >= 2) return ier;
+            break;
+        }
+    }
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
     return ier;
 }
 
+// per-channel coordinate scales 2 pi f_c / c on the device (nchan > 1)
+static int channel_scales(Ctx &C, int nchan, const double *freq, CoordArgs &a)
+{
+    a.nchan = nchan;
+    a.chan_scale = nullptr;
+    if (nchan <= 1) return CG_OK;
+    std::vector<double> h(nchan);
+    for (int c = 0; c < nchan; c++) h[c] = 2.0 * kPi * (freq[c] / kC);
+    CG_TRY(ensure(C.chans, sizeof(double) * nchan));
+    CG_HIP(hipMemcpyAsync(C.chans.p, h.data(), sizeof(double) * nchan, hipMemcpyHostToDevice, C.stream));
+    CG_HIP(hipStreamSynchronize(C.stream));
+    a.chan_scale = as<double>(C.chans);
+    return CG_OK;
+}
+
 // ------------------------------------------------------------ ms2dirty
 
-template <typename T, typename TU, typename TW>
-int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, const TU *uvw,
-                 const cx_t<T> *vis, const TW *wgt, cx_t<T> *dirty, double eps, double sigma,
-                 int sign, bool single)
+// Tuning of one call: tile edges, LDS budgets and the dense threshold
+// (defaults from the context options, or per item in a batch).
+struct Tune {
+    int tile_grid, tile_degrid, grid_lds, lds, dense_min;
+};
+static Tune ctx_tune(const Ctx &C)
 {
-    StageClock clk(C, 0);
-    CG_TRY(clk.mark(ST_PLAN));
+    return Tune{C.tile_grid, C.tile_degrid, C.grid_lds_budget, C.lds_budget, C.dense_min};
+}
+
+// Everything a call decides before its kernels run.  Planning reads the
+// w-range back to the host (it synchronises); executing a plan only
+// launches kernels and copies on the context stream, so a sequence of
+// executions can be captured into a graph once the workspace has grown.
+struct CallPlan {
     WPlan P;
-    int ier = plan_call<T, TU>(C, P, nrow, nx, ny, fov, freq, uvw, eps, sigma, sign, single);
-    if (ier >= 2) return ier;
+    Tiling t;
+    CoordArgs a;
+    int64_t M = 0;
+    int dense_min = 0;  // > 0: dense bins go through the MFMA gridder
+    int ier = 0;
+};
+
+// vis and wgt are [nrow][nchan]; every (row, channel) is one point
+template <typename T, typename TU>
+static int plan_ms2dirty(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const double *freq,
+                         int nx, int ny, double fov, const TU *uvw, double eps, double sigma,
+                         int sign, bool single, const Tune &tn)
+{
+    WPlan &P = cp.P;
+    cp.ier = plan_call<T, TU>(C, P, nrow, nchan, freq, nx, ny, fov, uvw, eps, sigma, sign, single);
+    if (cp.ier >= 2) return cp.ier;
     const int eb = (int)sizeof(cx_t<T>);
     // gridding accumulates in FP64 (16 bytes per cell) whatever the precision
-    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, C.grid_lds_budget, C.tile_grid, 0);
-    C.last_plan = P;
-    C.last_tiling = t;
-    C.last_m = nrow;
-    CoordArgs a = make_coord_args(P, t);
-    const int64_t M = nrow;
+    cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, tn.grid_lds, tn.tile_grid, 0);
+    cp.a = make_coord_args(P, cp.t);
+    CG_TRY(channel_scales(C, nchan, freq, cp.a));
+    cp.M = nrow * nchan;
     // bins with many points per start cell go through the MFMA path, added
     // on top of what the sparse kernel writes; it needs the points of a bin
     // ordered by start cell, which the sparse kernel alone is better without
     // (same-cell neighbours in a wavefront collide on the same LDS words)
-    const bool dense = sizeof(T) == 4 && C.dense_min > 0 && M > 0 &&
-                       grid_dense_supported(grid_args(P, t, M, a.cshift), P.conv.kw, eb);
-    if (!dense) a.cshift = 0;
+    const bool dense = sizeof(T) == 4 && tn.dense_min > 0 && cp.M > 0 &&
+                       grid_dense_supported(grid_args(C, P, cp.t, cp.M, cp.a.cshift), P.conv.kw, eb);
+    if (!dense) cp.a.cshift = 0;
+    cp.dense_min = dense ? tn.dense_min : 0;
+    return cp.ier;
+}
+
+template <typename T, typename TU, typename TW>
+static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T> *vis,
+                         const TW *wgt, cx_t<T> *dirty, StageClock &clk)
+{
+    const WPlan &P = cp.P;
+    const Tiling &t = cp.t;
+    const CoordArgs &a = cp.a;
+    const int64_t M = cp.M;
+    const int eb = (int)sizeof(cx_t<T>);
+    C.last_plan = P;
+    C.last_tiling = t;
+    C.last_m = M;
     C.last_cshift = a.cshift;
     CG_TRY(clk.mark(ST_SORT));
     if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
     else {
         CG_TRY(ensure(C.tstart, sizeof(int32_t) * (t.ntiles() + 1)));
-        CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (t.ntiles() + 1), C.stream));
+        CG_TRY(launch_fill_i32(as<int32_t>(C.tstart), t.ntiles() + 1, 0, C.stream));
     }
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
@@ -270,11 +363,11 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                                                 C.stream)));
     CG_TRY(clk.mark(ST_GRID));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
-    GridArgs g = grid_args(P, t, M, a.cshift);
-    g.dense_min = dense ? C.dense_min : 0;
+    GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.dense_min = cp.dense_min;
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
-    if (dense) {
+    if (cp.dense_min > 0) {
         CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 1)));
         CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
                                  as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
@@ -285,14 +378,16 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     const uint8_t *rm = nullptr;
     CG_TRY(row_mask(C, t, P.conv.kw, P.nf2, rm));
     CG_TRY(clk.mark(ST_FFT));
-    // only the columns holding image pixels are transformed along n2, and
-    // only the rows points reach hold non-zero data
+    // only the columns holding image pixels are transformed along n2, only
+    // the rows points reach hold non-zero data, and only the frequencies of
+    // image rows are kept along n2
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
+    const int sa = P.N2 - P.N2 / 2, sb = P.nf2 - P.N2 / 2;
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
+                              as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_IMAGE));
     const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
@@ -308,31 +403,59 @@ int ms2dirty_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
                                                   as<double>(C.phi3), as<double>(C.phi1),
                                                   as<double>(C.phi2), dirty, w, C.stream)));
     CG_TRY(clk.mark(ST_COUNT));
-    CG_TRY(clk.finish());
     return P.ier;
+}
+
+template <typename T, typename TU, typename TW>
+int ms2dirty_dev(Ctx &C, int64_t nrow, int nchan, const double *freq, int nx, int ny, double fov,
+                 const TU *uvw, const cx_t<T> *vis, const TW *wgt, cx_t<T> *dirty, double eps,
+                 double sigma, int sign, bool single)
+{
+    StageClock clk(C, 0);
+    CG_TRY(clk.mark(ST_PLAN));
+    CallPlan cp;
+    const int ier = plan_ms2dirty<T, TU>(C, cp, nrow, nchan, freq, nx, ny, fov, uvw, eps, sigma,
+                                         sign, single, ctx_tune(C));
+    if (ier >= 2) return ier;
+    CG_TRY((exec_ms2dirty<T, TU, TW>(C, cp, uvw, vis, wgt, dirty, clk)));
+    if (C.stop_after == 0) CG_TRY(clk.finish());
+    return cp.ier;
 }
 
 // ------------------------------------------------------------ dirty2ms
 
-template <typename T, typename TU, typename TW>
-int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, const TU *uvw,
-                 cx_t<T> *vis, const TW *wgt, const cx_t<T> *dirty, double eps, double sigma,
-                 int sign, bool single)
+template <typename T, typename TU>
+static int plan_dirty2ms(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const double *freq,
+                         int nx, int ny, double fov, const TU *uvw, double eps, double sigma,
+                         int sign, bool single, const Tune &tn)
 {
-    StageClock clk(C, 1);
-    CG_TRY(clk.mark(ST_PLAN));
-    WPlan P;
-    int ier = plan_call<T, TU>(C, P, nrow, nx, ny, fov, freq, uvw, eps, sigma, sign, single);
-    if (ier >= 2) return ier;
+    WPlan &P = cp.P;
+    cp.ier = plan_call<T, TU>(C, P, nrow, nchan, freq, nx, ny, fov, uvw, eps, sigma, sign, single);
+    if (cp.ier >= 2) return cp.ier;
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = P.conv.kw;
-    Tiling t = choose_tiling(P.nf1, P.nf2, P.nf3, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
+    cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, kw, eb, tn.lds, tn.tile_degrid, kw - 1);
+    cp.a = make_coord_args(P, cp.t);
+    CG_TRY(channel_scales(C, nchan, freq, cp.a));
+    cp.M = nrow * nchan;
+    cp.dense_min = 0;
+    return cp.ier;
+}
+
+template <typename T, typename TU, typename TW>
+static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis, const TW *wgt,
+                         const cx_t<T> *dirty, StageClock &clk)
+{
+    const WPlan &P = cp.P;
+    const Tiling &t = cp.t;
+    const CoordArgs &a = cp.a;
+    const int64_t M = cp.M;
+    const int eb = (int)sizeof(cx_t<T>);
+    const int kw = P.conv.kw;
     C.last_plan = P;
     C.last_tiling = t;
-    C.last_m = nrow;
-    CoordArgs a = make_coord_args(P, t);
+    C.last_m = M;
     C.last_cshift = a.cshift;
-    const int64_t M = nrow;
     CG_TRY(clk.mark(ST_SORT));
     if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
     CG_TRY(clk.mark(ST_RECORDS));
@@ -360,38 +483,232 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nx, int ny, double fov, double freq, 
     // the degridder reads only the rows its tiles' points reach
     const uint8_t *rm = nullptr;
     if (M > 0) CG_TRY(row_mask(C, t, kw, P.nf2, rm));
-    // columns first (only those holding image pixels), then full rows
+    // columns first (only those holding image pixels; frequencies outside
+    // the image rows are zero), then the rows the degridder reads
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
+    const int sa = P.N2 - P.N2 / 2, sb = P.nf2 - P.N2 / 2;
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
     CG_TRY(clk.mark(ST_FFT));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
+                              as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_GRID));
-    GridArgs g = grid_args(P, t, M, a.cshift);
+    GridArgs g = grid_args(C, P, t, M, a.cshift);
     if (M > 0)
         CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                           as<int32_t>(C.vals), as<int32_t>(C.tstart),
                                           as<cx_t<T>>(C.grid), vis, g, kw, C.stream)));
     CG_TRY(clk.mark(ST_COUNT));
-    // stage order differs from ms2dirty (image, column FFT, row FFT, grid):
-    // each stage is still reported under its own slot
-    if (C.timing) {
-        CG_HIP(hipEventSynchronize(C.ev[ST_COUNT]));
-        float ms[ST_COUNT] = {};
-        CG_HIP(hipEventElapsedTime(&ms[ST_PLAN], C.ev[ST_PLAN], C.ev[ST_SORT]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_SORT], C.ev[ST_SORT], C.ev[ST_RECORDS]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_RECORDS], C.ev[ST_RECORDS], C.ev[ST_IMAGE]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_IMAGE], C.ev[ST_IMAGE], C.ev[ST_FFT2]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_FFT2], C.ev[ST_FFT2], C.ev[ST_FFT]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_GRID]));
-        CG_HIP(hipEventElapsedTime(&ms[ST_GRID], C.ev[ST_GRID], C.ev[ST_COUNT]));
-        for (int i = 0; i < ST_COUNT; i++) C.stage_ms[1][i] += ms[i];
-        C.stage_calls[1]++;
-    }
     return P.ier;
+}
+
+// dirty2ms runs its stages in the order image, column FFT, row FFT, grid;
+// each is still reported under its own slot
+static int finish_dirty2ms_clock(Ctx &C)
+{
+    if (!C.timing) return CG_OK;
+    CG_HIP(hipEventSynchronize(C.ev[ST_COUNT]));
+    float ms[ST_COUNT] = {};
+    CG_HIP(hipEventElapsedTime(&ms[ST_PLAN], C.ev[ST_PLAN], C.ev[ST_SORT]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_SORT], C.ev[ST_SORT], C.ev[ST_RECORDS]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_RECORDS], C.ev[ST_RECORDS], C.ev[ST_IMAGE]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_IMAGE], C.ev[ST_IMAGE], C.ev[ST_FFT2]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_FFT2], C.ev[ST_FFT2], C.ev[ST_FFT]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_GRID]));
+    CG_HIP(hipEventElapsedTime(&ms[ST_GRID], C.ev[ST_GRID], C.ev[ST_COUNT]));
+    for (int i = 0; i < ST_COUNT; i++) C.stage_ms[1][i] += ms[i];
+    C.stage_calls[1]++;
+    return CG_OK;
+}
+
+template <typename T, typename TU, typename TW>
+int dirty2ms_dev(Ctx &C, int64_t nrow, int nchan, const double *freq, int nx, int ny, double fov,
+                 const TU *uvw, cx_t<T> *vis, const TW *wgt, const cx_t<T> *dirty, double eps,
+                 double sigma, int sign, bool single)
+{
+    StageClock clk(C, 1);
+    CG_TRY(clk.mark(ST_PLAN));
+    CallPlan cp;
+    const int ier = plan_dirty2ms<T, TU>(C, cp, nrow, nchan, freq, nx, ny, fov, uvw, eps, sigma,
+                                         sign, single, ctx_tune(C));
+    if (ier >= 2) return ier;
+    CG_TRY((exec_dirty2ms<T, TU, TW>(C, cp, uvw, vis, wgt, dirty, clk)));
+    if (C.stop_after == 0) CG_TRY(finish_dirty2ms_clock(C));
+    return cp.ier;
+}
+
+// ------------------------------------------------------------ batches
+//
+// Many independent channels (each its own uvw, vis, image size and
+// frequency, complex64 with FP64 uvw) run as one launch: every item is
+// planned once (optionally autotuned), executed once eagerly so that the
+// batch's private workspace reaches its final size, then the executions of
+// all items are captured into one HIP graph that cg_batch_run replays.
+
+// installs a tabulated kernel in the context (device copy of its
+// coefficients); eps <= 0 removes the kernel of that support
+static int set_learned(Ctx &C, const LearnedKernel &K, double eps)
+{
+    Ctx::Learned &L = C.learned[K.kw];
+    if (!(eps > 0)) {
+        L.eps = 0;
+        return CG_OK;
+    }
+    const size_t bytes = sizeof(float) * K.kw * (K.deg + 1);
+    CG_TRY(ensure(L.coef, bytes));
+    CG_HIP(hipMemcpyAsync(L.coef.p, K.coef, bytes, hipMemcpyHostToDevice, C.stream));
+    CG_HIP(hipStreamSynchronize(C.stream));
+    L.K = K;
+    L.eps = eps;
+    return CG_OK;
+}
+
+struct Batch {
+    Ctx C;                // private workspace: the graph holds its pointers
+    int dir = 0;          // 0 ms2dirty, 1 dirty2ms
+    std::vector<cg_batch_item> items;
+    std::vector<CallPlan> plans;
+    std::vector<Tune> tunes;
+    std::vector<float> tuned_ms;
+    double eps = 0, sigma = 2, tune_ms = 0;
+    int sign = -1;
+    hipStream_t user = nullptr, cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~Batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+static int batch_plan(Batch &B, size_t i, const Tune &tn, CallPlan &cp)
+{
+    const cg_batch_item &it = B.items[i];
+    const double f = it.freq;
+    if (B.dir == 0)
+        return plan_ms2dirty<float, double>(B.C, cp, it.nrow, 1, &f, it.nxdirty, it.nydirty, it.fov,
+                                            (const double *)it.d_uvw, B.eps, B.sigma, B.sign, true, tn);
+    return plan_dirty2ms<float, double>(B.C, cp, it.nrow, 1, &f, it.nxdirty, it.nydirty, it.fov,
+                                        (const double *)it.d_uvw, B.eps, B.sigma, B.sign, true, tn);
+}
+
+static int batch_exec(Batch &B, size_t i)
+{
+    const cg_batch_item &it = B.items[i];
+    StageClock clk(B.C, B.dir);
+    if (B.dir == 0)
+        return exec_ms2dirty<float, double, float>(B.C, B.plans[i], (const double *)it.d_uvw,
+                                                   (const float2 *)it.d_vis, (const float *)it.d_wgt,
+                                                   (float2 *)it.d_dirty, clk);
+    return exec_dirty2ms<float, double, float>(B.C, B.plans[i], (const double *)it.d_uvw,
+                                               (float2 *)it.d_vis, (const float *)it.d_wgt,
+                                               (const float2 *)it.d_dirty, clk);
+}
+
+// time one execution of item i under tuning tn (after one untimed run)
+static int batch_time(Batch &B, size_t i, const Tune &tn, float &ms)
+{
+    CallPlan cp;
+    int ier = batch_plan(B, i, tn, cp);
+    if (ier >= 2) return ier;
+    std::swap(B.plans[i], cp);
+    hipEvent_t e0, e1;
+    CG_HIP(hipEventCreate(&e0));
+    CG_HIP(hipEventCreate(&e1));
+    int r = batch_exec(B, i);
+    if (!r && hipEventRecord(e0, B.C.stream) != hipSuccess) r = CG_ERR_HIP;
+    if (!r) r = batch_exec(B, i);
+    if (!r && hipEventRecord(e1, B.C.stream) != hipSuccess) r = CG_ERR_HIP;
+    if (!r && hipEventSynchronize(e1) != hipSuccess) r = CG_ERR_HIP;
+    if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = CG_ERR_HIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::swap(B.plans[i], cp);
+    return r;
+}
+
+// candidate tunings per direction; results are shared by items of the same
+// image size and power-of-two visibility count
+static int batch_autotune(Batch &B)
+{
+    std::vector<Tune> cands;
+    const Tune d = ctx_tune(B.C);
+    if (B.dir == 0) {
+        for (int dm : {32, 64, 256}) cands.push_back(Tune{8, d.tile_degrid, d.grid_lds, d.lds, dm});
+        cands.push_back(Tune{16, d.tile_degrid, d.grid_lds, d.lds, d.dense_min});
+    } else {
+        for (int td : {16, 32}) cands.push_back(Tune{d.tile_grid, td, d.grid_lds, d.lds, d.dense_min});
+    }
+    std::vector<std::pair<long long, size_t>> seen;  // shape key -> item tuned
+    for (size_t i = 0; i < B.items.size(); i++) {
+        const cg_batch_item &it = B.items[i];
+        int lg = 0;
+        while ((1LL << (lg + 1)) <= (long long)std::max(1, it.nrow)) lg++;
+        const long long key = ((long long)it.nxdirty << 40) | ((long long)it.nydirty << 20) | lg;
+        bool done = false;
+        for (auto &kv : seen)
+            if (kv.first == key) {
+                B.tunes[i] = B.tunes[kv.second];
+                B.tuned_ms[i] = B.tuned_ms[kv.second];
+                done = true;
+                break;
+            }
+        if (done) continue;
+        float best = 1e30f;
+        for (const Tune &tn : cands) {
+            float ms = 0;
+            CG_TRY(batch_time(B, i, tn, ms));
+            if (ms < best) {
+                best = ms;
+                B.tunes[i] = tn;
+            }
+        }
+        B.tuned_ms[i] = best;
+        seen.push_back({key, i});
+    }
+    return CG_OK;
+}
+
+static int batch_build(Batch &B, int flags)
+{
+    const size_t n = B.items.size();
+    B.plans.assign(n, CallPlan());
+    B.tunes.assign(n, ctx_tune(B.C));
+    B.tuned_ms.assign(n, 0.f);
+    for (size_t i = 0; i < n; i++) CG_TRY(batch_plan(B, i, B.tunes[i], B.plans[i]));
+    if (flags & CG_BATCH_AUTOTUNE) CG_TRY(batch_autotune(B));
+    int ier = 0;
+    for (size_t i = 0; i < n; i++) {
+        const int r = batch_plan(B, i, B.tunes[i], B.plans[i]);
+        if (r >= 2) return r;
+        ier = std::max(ier, r);
+    }
+    // one eager pass: the workspace reaches its final size
+    for (size_t i = 0; i < n; i++) CG_TRY(batch_exec(B, i));
+    CG_HIP(hipStreamSynchronize(B.C.stream));
+    if (flags & CG_BATCH_GRAPH) {
+        // capture on a private stream (the context's may be the null stream)
+        CG_HIP(hipStreamCreateWithFlags(&B.cap, hipStreamNonBlocking));
+        B.C.stream = B.cap;
+        CG_HIP(hipStreamBeginCapture(B.cap, hipStreamCaptureModeThreadLocal));
+        int r = CG_OK;
+        for (size_t i = 0; i < n && !r; i++) r = batch_exec(B, i);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(B.cap, &g);
+        B.C.stream = B.user;
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        CG_HIP(e);
+        B.graph = g;
+        CG_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    }
+    return ier;
 }
 
 // ------------------------------------------------------------ 2-D NUFFT
@@ -449,7 +766,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift,
                                   as<int32_t>(C.tstart), C.stream));
     } else {
-        CG_HIP(hipMemsetAsync(C.tstart.p, 0, sizeof(int32_t) * (ntiles + 1), C.stream));
+        CG_TRY(launch_fill_i32(as<int32_t>(C.tstart), ntiles + 1, 0, C.stream));
     }
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
@@ -464,9 +781,11 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(launch_fseries_grid(qd, P.nf1, as<double>(C.phi1), C.stream));
     CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    GridArgs g = grid_args(P, t, M, a.cshift);
-    // fast-axis columns that hold modes; grid rows that points reach
+    GridArgs g = grid_args(C, P, t, M, a.cshift);
+    // fast-axis columns that hold modes; grid rows that points reach; mode
+    // frequencies along nf2
     const int la = N1 - N1 / 2, lb = P.nf1 - N1 / 2;
+    const int sa = N2 - N2 / 2, sb = P.nf2 - N2 / 2;
     const uint8_t *rm = nullptr;
     CG_TRY(row_mask(C, t, kw, P.nf2, rm));
     if (type == 1) {
@@ -475,10 +794,10 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
                               as<cx_t<T>>(C.grid), g, kw, C.stream));
         CG_TRY(clk.mark(ST_FFT));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, true,
-                                  as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
         CG_TRY(clk.mark(ST_FFT2));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, +1, false,
-                                  as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), fk, as<double>(C.phi1),
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 1, C.stream)));
@@ -488,10 +807,10 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
                                             as<double>(C.phi2), N1, N2, P.nf1, P.nf2, 2, C.stream)));
         CG_TRY(clk.mark(ST_FFT));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, false,
-                                  as<cx_t<T>>(C.tw2), la, lb, rm, C.stream));
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
         CG_TRY(clk.mark(ST_FFT2));
         CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), 1, P.nf2, P.nf1, -1, true,
-                                  as<cx_t<T>>(C.tw1), la, lb, rm, C.stream));
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
         CG_TRY(clk.mark(ST_IMAGE));
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
@@ -523,28 +842,30 @@ struct HostBuf {
     }
 };
 
-// host-pointer ms2dirty / dirty2ms: H2D, run, D2H, free (reference semantics)
+// host-pointer ms2dirty / dirty2ms: H2D, run, D2H, free (reference semantics);
+// vis and wgt are [nrow][nchan] for nchan frequencies freq[nchan]
 template <typename T, typename TU, typename TW>
-static int host_ms2dirty(int nrow, int nx, int ny, double fov, double freq, const TU *uvw,
-                         const void *vis, const TW *wgt, void *dirty, double eps, double sigma,
-                         int sign, bool single)
+static int host_ms2dirty(int nrow, int nchan, const double *freq, int nx, int ny, double fov,
+                         const TU *uvw, const void *vis, const TW *wgt, void *dirty, double eps,
+                         double sigma, int sign, bool single)
 {
-    if (nrow < 0 || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis)) return CG_ERR_ARG;
+    if (nrow < 0 || nchan < 1 || !freq || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis))
+        return CG_ERR_ARG;
     try {
         Ctx C;
         env_overrides(C);
-        const size_t M = (size_t)nrow;
+        const size_t R = (size_t)nrow, M = R * (size_t)nchan;
         HostBuf du, dv, dw, dd;
-        CG_TRY(du.alloc(sizeof(TU) * 3 * M));
+        CG_TRY(du.alloc(sizeof(TU) * 3 * R));
         CG_TRY(dv.alloc(sizeof(cx_t<T>) * M));
         if (wgt) CG_TRY(dw.alloc(sizeof(TW) * M));
         CG_TRY(dd.alloc(sizeof(cx_t<T>) * (size_t)nx * ny));
         if (M) {
-            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * M, hipMemcpyHostToDevice));
+            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * R, hipMemcpyHostToDevice));
             CG_HIP(hipMemcpy(dv.p, vis, sizeof(cx_t<T>) * M, hipMemcpyHostToDevice));
             if (wgt) CG_HIP(hipMemcpy(dw.p, wgt, sizeof(TW) * M, hipMemcpyHostToDevice));
         }
-        int ier = ms2dirty_dev<T, TU, TW>(C, nrow, nx, ny, fov, freq, (const TU *)du.p,
+        int ier = ms2dirty_dev<T, TU, TW>(C, nrow, nchan, freq, nx, ny, fov, (const TU *)du.p,
                                           (const cx_t<T> *)dv.p, wgt ? (const TW *)dw.p : nullptr,
                                           (cx_t<T> *)dd.p, eps, sigma, sign, single);
         if (ier >= 2) return ier;
@@ -556,26 +877,27 @@ static int host_ms2dirty(int nrow, int nx, int ny, double fov, double freq, cons
 }
 
 template <typename T, typename TU, typename TW>
-static int host_dirty2ms(int nrow, int nx, int ny, double fov, double freq, const TU *uvw,
-                         void *vis, const TW *wgt, const void *dirty, double eps, double sigma,
-                         int sign, bool single)
+static int host_dirty2ms(int nrow, int nchan, const double *freq, int nx, int ny, double fov,
+                         const TU *uvw, void *vis, const TW *wgt, const void *dirty, double eps,
+                         double sigma, int sign, bool single)
 {
-    if (nrow < 0 || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis)) return CG_ERR_ARG;
+    if (nrow < 0 || nchan < 1 || !freq || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis))
+        return CG_ERR_ARG;
     try {
         Ctx C;
         env_overrides(C);
-        const size_t M = (size_t)nrow;
+        const size_t R = (size_t)nrow, M = R * (size_t)nchan;
         HostBuf du, dv, dw, dd;
-        CG_TRY(du.alloc(sizeof(TU) * 3 * M));
+        CG_TRY(du.alloc(sizeof(TU) * 3 * R));
         CG_TRY(dv.alloc(sizeof(cx_t<T>) * M));
         if (wgt) CG_TRY(dw.alloc(sizeof(TW) * M));
         CG_TRY(dd.alloc(sizeof(cx_t<T>) * (size_t)nx * ny));
         if (M) {
-            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * M, hipMemcpyHostToDevice));
+            CG_HIP(hipMemcpy(du.p, uvw, sizeof(TU) * 3 * R, hipMemcpyHostToDevice));
             if (wgt) CG_HIP(hipMemcpy(dw.p, wgt, sizeof(TW) * M, hipMemcpyHostToDevice));
         }
         CG_HIP(hipMemcpy(dd.p, dirty, sizeof(cx_t<T>) * (size_t)nx * ny, hipMemcpyHostToDevice));
-        int ier = dirty2ms_dev<T, TU, TW>(C, nrow, nx, ny, fov, freq, (const TU *)du.p,
+        int ier = dirty2ms_dev<T, TU, TW>(C, nrow, nchan, freq, nx, ny, fov, (const TU *)du.p,
                                           (cx_t<T> *)dv.p, wgt ? (const TW *)dw.p : nullptr,
                                           (const cx_t<T> *)dd.p, eps, sigma, sign, single);
         if (ier >= 2) return ier;
@@ -631,50 +953,58 @@ extern "C" {
 int ms2dirty_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
                double *vis, double *wgt, double *dirty, double epsilon, double sigma, int sign)
 {
-    return host_ms2dirty<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_ms2dirty<double, double, double>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                                  dirty, epsilon, sigma, sign, false);
 }
 int ms2dirty_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
                double *vis, double *dirty, double epsilon, double sigma, int sign)
 {
-    return host_ms2dirty<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis,
+    const double fq = freq;
+    return host_ms2dirty<double, double, double>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis,
                                                  nullptr, dirty, epsilon, sigma, sign, false);
 }
 int dirty2ms_2(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
                double *vis, double *wgt, double *dirty, double epsilon, double sigma, int sign)
 {
-    return host_dirty2ms<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_dirty2ms<double, double, double>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                                  dirty, epsilon, sigma, sign, false);
 }
 int dirty2ms_1(int nrow, int nxdirty, int nydirty, double fov, double freq, double *uvw,
                double *vis, double *dirty, double epsilon, double sigma, int sign)
 {
-    return host_dirty2ms<double, double, double>(nrow, nxdirty, nydirty, fov, freq, uvw, vis,
+    const double fq = freq;
+    return host_dirty2ms<double, double, double>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis,
                                                  nullptr, dirty, epsilon, sigma, sign, false);
 }
 // single precision (float ABI, FP32 uvw)
 int ms2dirtyf_2(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
                 float *wgt, float *dirty, float epsilon, float sigma, int sign)
 {
-    return host_ms2dirty<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_ms2dirty<float, float, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                               dirty, epsilon, sigma, sign, true);
 }
 int ms2dirtyf_1(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
                 float *dirty, float epsilon, float sigma, int sign)
 {
-    return host_ms2dirty<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr,
+    const double fq = freq;
+    return host_ms2dirty<float, float, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, nullptr,
                                               dirty, epsilon, sigma, sign, true);
 }
 int dirty2msf_2(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
                 float *wgt, float *dirty, float epsilon, float sigma, int sign)
 {
-    return host_dirty2ms<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_dirty2ms<float, float, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                               dirty, epsilon, sigma, sign, true);
 }
 int dirty2msf_1(int nrow, int nxdirty, int nydirty, float fov, float freq, float *uvw, float *vis,
                 float *dirty, float epsilon, float sigma, int sign)
 {
-    return host_dirty2ms<float, float, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, nullptr,
+    const double fq = freq;
+    return host_dirty2ms<float, float, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, nullptr,
                                               dirty, epsilon, sigma, sign, true);
 }
 
@@ -683,15 +1013,49 @@ int ms2dirty_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq, 
                    const float *vis, const float *wgt, float *dirty, double epsilon, double sigma,
                    int sign)
 {
-    return host_ms2dirty<float, double, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_ms2dirty<float, double, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                                dirty, epsilon, sigma, sign, true);
 }
 int dirty2ms_mixed(int nrow, int nxdirty, int nydirty, double fov, double freq, const double *uvw,
                    float *vis, const float *wgt, const float *dirty, double epsilon, double sigma,
                    int sign)
 {
-    return host_dirty2ms<float, double, float>(nrow, nxdirty, nydirty, fov, freq, uvw, vis, wgt,
+    const double fq = freq;
+    return host_dirty2ms<float, double, float>(nrow, 1, &fq, nxdirty, nydirty, fov, uvw, vis, wgt,
                                                dirty, epsilon, sigma, sign, true);
+}
+
+// several channels: freq[nchan] (host), vis and wgt [nrow][nchan]; the
+// dirty image sums all channels (no counterpart in the reference C-ABI,
+// whose binding passes freq[0] only, python/curagridder/cursl.py:103)
+int ms2dirty_mc_2(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                  const double *uvw, const double *vis, const double *wgt, double *dirty,
+                  double epsilon, double sigma, int sign)
+{
+    return host_ms2dirty<double, double, double>(nrow, nchan, freq, nxdirty, nydirty, fov, uvw, vis,
+                                                 wgt, dirty, epsilon, sigma, sign, false);
+}
+int dirty2ms_mc_2(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                  const double *uvw, double *vis, const double *wgt, const double *dirty,
+                  double epsilon, double sigma, int sign)
+{
+    return host_dirty2ms<double, double, double>(nrow, nchan, freq, nxdirty, nydirty, fov, uvw, vis,
+                                                 wgt, dirty, epsilon, sigma, sign, false);
+}
+int ms2dirty_mc_mixed(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                      const double *uvw, const float *vis, const float *wgt, float *dirty,
+                      double epsilon, double sigma, int sign)
+{
+    return host_ms2dirty<float, double, float>(nrow, nchan, freq, nxdirty, nydirty, fov, uvw, vis,
+                                               wgt, dirty, epsilon, sigma, sign, true);
+}
+int dirty2ms_mc_mixed(int nrow, int nchan, int nxdirty, int nydirty, double fov, const double *freq,
+                      const double *uvw, float *vis, const float *wgt, const float *dirty,
+                      double epsilon, double sigma, int sign)
+{
+    return host_dirty2ms<float, double, float>(nrow, nchan, freq, nxdirty, nydirty, fov, uvw, vis,
+                                               wgt, dirty, epsilon, sigma, sign, true);
 }
 
 // 2-D NUFFT type 1 / 2 (points in [-pi,pi)), host pointers
@@ -750,6 +1114,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
     else if (!std::strcmp(name, "row_mask")) C->row_mask = value;
+    else if (!std::strcmp(name, "use_learned")) C->use_learned = value;
     else return CG_ERR_ARG;
     return CG_OK;
 }
@@ -793,56 +1158,156 @@ int cg_ctx_last_plan(void *ctx, double *out)
 }
 
 // prec: 0 = complex64 vis/image with FP32 gridding and FFT, 1 = complex128;
-// uvw_prec / wgt_prec: 0 float, 1 double.  d_wgt may be NULL.
-int cg_ms2dirty_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
-                    double freq, const void *d_uvw, int uvw_prec, const void *d_vis,
-                    const void *d_wgt, int wgt_prec, void *d_dirty, double epsilon, double sigma,
-                    int sign)
+// uvw_prec / wgt_prec: 0 float, 1 double.  d_wgt may be NULL.  freq is a
+// host array of nchan frequencies; d_vis and d_wgt are [nrow][nchan].
+int cg_ms2dirty_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                       double fov, const double *freq, const void *d_uvw, int uvw_prec,
+                       const void *d_vis, const void *d_wgt, int wgt_prec, void *d_dirty,
+                       double epsilon, double sigma, int sign)
 {
     Ctx *C = (Ctx *)ctx;
     if (!C) return CG_ERR_ARG;
     bool single = prec == 0;
     if (prec == 0 && uvw_prec == 1 && wgt_prec == 0)
-        return ms2dirty_dev<float, double, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return ms2dirty_dev<float, double, float>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                   (const double *)d_uvw, (const float2 *)d_vis,
                                                   (const float *)d_wgt, (float2 *)d_dirty, epsilon,
                                                   sigma, sign, single);
     if (prec == 0 && uvw_prec == 0 && wgt_prec == 0)
-        return ms2dirty_dev<float, float, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return ms2dirty_dev<float, float, float>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                  (const float *)d_uvw, (const float2 *)d_vis,
                                                  (const float *)d_wgt, (float2 *)d_dirty, epsilon,
                                                  sigma, sign, single);
     if (prec == 1 && uvw_prec == 1 && wgt_prec == 1)
-        return ms2dirty_dev<double, double, double>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return ms2dirty_dev<double, double, double>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                     (const double *)d_uvw, (const double2 *)d_vis,
                                                     (const double *)d_wgt, (double2 *)d_dirty,
                                                     epsilon, sigma, sign, single);
     return CG_ERR_ARG;
 }
 
-int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
-                    double freq, const void *d_uvw, int uvw_prec, void *d_vis, const void *d_wgt,
-                    int wgt_prec, const void *d_dirty, double epsilon, double sigma, int sign)
+int cg_dirty2ms_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                       double fov, const double *freq, const void *d_uvw, int uvw_prec, void *d_vis,
+                       const void *d_wgt, int wgt_prec, const void *d_dirty, double epsilon,
+                       double sigma, int sign)
 {
     Ctx *C = (Ctx *)ctx;
     if (!C) return CG_ERR_ARG;
     bool single = prec == 0;
     if (prec == 0 && uvw_prec == 1 && wgt_prec == 0)
-        return dirty2ms_dev<float, double, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return dirty2ms_dev<float, double, float>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                   (const double *)d_uvw, (float2 *)d_vis,
                                                   (const float *)d_wgt, (const float2 *)d_dirty,
                                                   epsilon, sigma, sign, single);
     if (prec == 0 && uvw_prec == 0 && wgt_prec == 0)
-        return dirty2ms_dev<float, float, float>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return dirty2ms_dev<float, float, float>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                  (const float *)d_uvw, (float2 *)d_vis,
                                                  (const float *)d_wgt, (const float2 *)d_dirty,
                                                  epsilon, sigma, sign, single);
     if (prec == 1 && uvw_prec == 1 && wgt_prec == 1)
-        return dirty2ms_dev<double, double, double>(*C, nrow, nxdirty, nydirty, fov, freq,
+        return dirty2ms_dev<double, double, double>(*C, nrow, nchan, freq, nxdirty, nydirty, fov,
                                                     (const double *)d_uvw, (double2 *)d_vis,
                                                     (const double *)d_wgt, (const double2 *)d_dirty,
                                                     epsilon, sigma, sign, single);
     return CG_ERR_ARG;
+}
+
+int cg_ms2dirty_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
+                    double freq, const void *d_uvw, int uvw_prec, const void *d_vis,
+                    const void *d_wgt, int wgt_prec, void *d_dirty, double epsilon, double sigma,
+                    int sign)
+{
+    return cg_ms2dirty_mc_dev(ctx, prec, nrow, 1, nxdirty, nydirty, fov, &freq, d_uvw, uvw_prec,
+                              d_vis, d_wgt, wgt_prec, d_dirty, epsilon, sigma, sign);
+}
+
+int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, double fov,
+                    double freq, const void *d_uvw, int uvw_prec, void *d_vis, const void *d_wgt,
+                    int wgt_prec, const void *d_dirty, double epsilon, double sigma, int sign)
+{
+    return cg_dirty2ms_mc_dev(ctx, prec, nrow, 1, nxdirty, nydirty, fov, &freq, d_uvw, uvw_prec,
+                              d_vis, d_wgt, wgt_prec, d_dirty, epsilon, sigma, sign);
+}
+
+int cg_batch_create(void **batch, void *ctx, int direction, int n, const cg_batch_item *items,
+                    double epsilon, double sigma, int sign, int flags)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!batch || !C || (direction != 0 && direction != 1) || n <= 0 || !items) return CG_ERR_ARG;
+    *batch = nullptr;
+    for (int i = 0; i < n; i++) {
+        const cg_batch_item &it = items[i];
+        if (it.nrow < 0 || it.nxdirty <= 0 || it.nydirty <= 0 || !it.d_dirty ||
+            (it.nrow > 0 && (!it.d_uvw || !it.d_vis)) || !(it.freq > 0) || !(it.fov > 0))
+            return CG_ERR_ARG;
+    }
+    try {
+        Batch *B = new Batch();
+        B->C.lds_budget = C->lds_budget;
+        B->C.grid_lds_budget = C->grid_lds_budget;
+        B->C.tile_grid = C->tile_grid;
+        B->C.tile_degrid = C->tile_degrid;
+        B->C.dense_min = C->dense_min;
+        B->C.row_mask = C->row_mask;
+        B->C.stream = C->stream;
+        B->C.use_learned = C->use_learned;
+        for (int kw = 0; kw <= LEARN_MAXKW; kw++)
+            if (C->learned[kw].eps > 0) {
+                const int r = set_learned(B->C, C->learned[kw].K, C->learned[kw].eps);
+                if (r) {
+                    delete B;
+                    return r;
+                }
+            }
+        B->user = C->stream;
+        B->dir = direction;
+        B->items.assign(items, items + n);
+        B->eps = epsilon;
+        B->sigma = sigma;
+        B->sign = sign;
+        const int ier = batch_build(*B, flags);
+        if (ier >= 2) {
+            delete B;
+            return ier;
+        }
+        *batch = B;
+        return ier;
+    } catch (const std::bad_alloc &) {
+        return CG_ERR_ALLOC;
+    }
+}
+
+int cg_batch_run(void *batch)
+{
+    Batch *B = (Batch *)batch;
+    if (!B) return CG_ERR_ARG;
+    if (B->exec) {
+        CG_HIP(hipGraphLaunch(B->exec, B->user));
+        return CG_OK;
+    }
+    for (size_t i = 0; i < B->items.size(); i++) CG_TRY(batch_exec(*B, i));
+    return CG_OK;
+}
+
+int cg_batch_info(void *batch, double *out)
+{
+    Batch *B = (Batch *)batch;
+    if (!B || !out) return CG_ERR_ARG;
+    for (size_t i = 0; i < B->items.size(); i++) {
+        const Tune &t = B->tunes[i];
+        out[5 * i + 0] = t.tile_grid;
+        out[5 * i + 1] = t.tile_degrid;
+        out[5 * i + 2] = B->plans[i].dense_min;
+        out[5 * i + 3] = B->tuned_ms[i];
+        out[5 * i + 4] = B->plans[i].P.nf3;
+    }
+    return B->exec ? 1 : 0;
+}
+
+int cg_batch_destroy(void *batch)
+{
+    delete (Batch *)batch;
+    return CG_OK;
 }
 
 int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x, const void *d_y,
@@ -959,6 +1424,50 @@ int cg_fft_col_split(int n, int *na, int *nb)
     return 0;
 }
 
-int cg_version(void) { return 101; }
+int cg_version(void) { return 102; }
+
+// ---- learned kernel
+int cg_learn_nparams(void) { return LEARN_NPARAMS; }
+
+int cg_learn_grad(int kw, const float *params, const double *u, const double *theta, int nsamp,
+                  float *grad, double *loss)
+{
+    return learn_grad(kw, params, u, theta, nsamp, grad, loss, nullptr);
+}
+
+int cg_learn_train(int kw, int mode, const double *a, const double *b, int n, int batch,
+                   int steps, double lr, float *params, double *losses)
+{
+    return learn_train(kw, mode, a, b, n, batch, steps, lr, params, losses, nullptr);
+}
+
+int cg_learn_eval(const float *params, const float *s, int n, float *y)
+{
+    return learn_eval(params, s, n, y, nullptr);
+}
+
+int cg_learn_tabulate(int kw, int deg, const float *params, float *coef, double *qpsi)
+{
+    if (!coef || !qpsi) return CG_ERR_ARG;
+    LearnedKernel K;
+    CG_TRY(learn_kernel_build(kw, deg, params, K, nullptr));
+    std::memcpy(coef, K.coef, sizeof(float) * kw * (deg + 1));
+    std::memcpy(qpsi, K.qpsi, sizeof(double) * K.nq);
+    return CG_OK;
+}
+
+int cg_ctx_set_kernel(void *ctx, int kw, int deg, const float *params, double eps)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || kw < 2 || kw > LEARN_MAXKW) return CG_ERR_ARG;
+    if (!(eps > 0)) {
+        C->learned[kw].eps = 0;
+        return CG_OK;
+    }
+    if (!params) return CG_ERR_ARG;
+    LearnedKernel K;
+    CG_TRY(learn_kernel_build(kw, deg, params, K, C->stream));
+    return set_learned(*C, K, eps);
+}
 
 }  // extern "C"

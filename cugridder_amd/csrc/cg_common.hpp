@@ -44,6 +44,13 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
         }                                                                   \
     } while (0)
 
+// propagates a non-zero status code
+#define CG_TRY(x)              \
+    do {                       \
+        int _r = (x);          \
+        if (_r) return _r;     \
+    } while (0)
+
 namespace cg {
 void report_hip_error(hipError_t e, const char *what, const char *file, int line);
 }

@@ -537,7 +537,8 @@ template <typename T, int DIR, int LOGL, int CB>
 __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_step_ct(cx_t<T> *data, int n2, int n1, int se, int sj,
                                                            int twmode, const cx_t<T> *__restrict__ tw2,
                                                            Live live, const uint8_t *__restrict__ rowmask,
-                                                           int maskmode)
+                                                           int mode_in, int mode_out, int spec_a,
+                                                           int spec_b)
 {
     constexpr int L = 1 << LOGL;
     constexpr int EPT = fft_ept<T>();
@@ -556,14 +557,21 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
     const int c0 = blockIdx.x * CB;
     const size_t row0 = (size_t)j * sj;
     const int tstride = n2 >> LOGL;
+    // a row is dead when it is known zero on input / not needed on output:
+    // natural-order rows (sj == 1) by the row mask, split-order rows
+    // (frequency j + na e, na = gridDim.y) outside [0, spec_a) u [spec_b, n2)
+    auto dead = [&](size_t row, int e) -> bool {
+        if (sj == 1) return rowmask && !rowmask[row];
+        const int k = j + (int)gridDim.y * e;
+        return k >= spec_a && k < spec_b;
+    };
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
         const int k = threadIdx.x + u * NT;
         const int e = k / CB, b = k % CB;
         const int c = c0 + b;
         const size_t row = row0 + (size_t)e * se;
-        // maskmode 1: rows with mask 0 are known zeros, not read
-        const bool rd = c < nlive && !(maskmode == 1 && !rowmask[row]);
+        const bool rd = c < nlive && !(mode_in && dead(row, e));
         cx_t<T> v = rd ? pl[row * n1 + live.at(c)] : mkc((T)0, (T)0);
         if (twmode == 2) {
             cx_t<T> w = tab.get(j * e);
@@ -581,8 +589,7 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
         const int c = c0 + b;
         if (c >= nlive) continue;
         const size_t row = row0 + (size_t)e * se;
-        // maskmode 2: rows with mask 0 are not needed downstream
-        if (maskmode == 2 && !rowmask[row]) continue;
+        if (mode_out && dead(row, e)) continue;
         cx_t<T> v = buf[k];
         if (twmode == 1) {
             cx_t<T> w = tab.get(j * e);
@@ -726,10 +733,15 @@ static int rows_ct_dispatch(cx_t<T> *data, int64_t nrows, int n1, const cx_t<T> 
 }
 
 // compile-time column step for sub-transforms of L = 2^LOGL (1 if not covered)
+struct DeadRows {
+    const uint8_t *rowmask;  // natural-order rows
+    int spec_a, spec_b;      // live split-order frequencies: k < spec_a or k >= spec_b
+    int mode_in, mode_out;   // skip loads of / stores to dead rows
+};
+
 template <typename T, int DIR, int LOGL>
 static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
-                        int twmode, const cx_t<T> *tw2, Live live, const uint8_t *rowmask,
-                        int maskmode, hipStream_t s)
+                        int twmode, const cx_t<T> *tw2, Live live, const DeadRows &dr, hipStream_t s)
 {
     constexpr int L = 1 << LOGL;
     constexpr int ELEM = (int)sizeof(cx_t<T>);
@@ -746,7 +758,8 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
         const int nlive = live.count(n1);
         dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)nsub, (unsigned)batch);
         hipLaunchKernelGGL((k_fft_cols_step_ct<T, DIR, LOGL, CB>), grid, dim3(NT), lds, s, data, n2,
-                           n1, se, sj, twmode, tw2, live, rowmask, rowmask ? maskmode : 0);
+                           n1, se, sj, twmode, tw2, live, dr.rowmask, dr.mode_in, dr.mode_out,
+                           dr.spec_a, dr.spec_b);
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
@@ -755,14 +768,14 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
 template <typename T, int DIR>
 static int cols_step_ct_dispatch(cx_t<T> *data, int batch, int n2, int n1, int L, int se, int sj,
                                  int nsub, int twmode, const cx_t<T> *tw2, Live live,
-                                 const uint8_t *m, int mm, hipStream_t s)
+                                 const DeadRows &d, hipStream_t s)
 {
     switch (log2_exact(L)) {
-    case 4: return cols_step_ct<T, DIR, 4>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
-    case 5: return cols_step_ct<T, DIR, 5>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
-    case 6: return cols_step_ct<T, DIR, 6>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
-    case 7: return cols_step_ct<T, DIR, 7>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
-    case 8: return cols_step_ct<T, DIR, 8>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, m, mm, s);
+    case 4: return cols_step_ct<T, DIR, 4>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, d, s);
+    case 5: return cols_step_ct<T, DIR, 5>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, d, s);
+    case 6: return cols_step_ct<T, DIR, 6>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, d, s);
+    case 7: return cols_step_ct<T, DIR, 7>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, d, s);
+    case 8: return cols_step_ct<T, DIR, 8>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live, d, s);
     default: return 1;
     }
 }
@@ -814,7 +827,7 @@ static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw,
 
 template <typename T, int DIR>
 static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2, Live live,
-                    const uint8_t *rowmask, hipStream_t s)
+                    const uint8_t *rowmask, int spec_a, int spec_b, hipStream_t s)
 {
     const int elem = (int)sizeof(cx_t<T>);
     const int nlive = live.count(n1);
@@ -847,13 +860,22 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         return CG_OK;
     }
     const bool ct = fft_knob("CG_FFT_CT", 1) != 0;
-    // the natural-order step (the first one forward, the last one inverse)
-    // reads / writes only the masked-in rows
+    // dead rows: forward, the natural-order step skips loading masked-out
+    // rows and the split-order step skips storing dead frequencies; inverse,
+    // the split-order step skips loading dead frequencies and the
+    // natural-order step skips storing masked-out rows
     auto step = [&](int L, int se, int sj, int nsub, int twmode) -> int {
         if (ct) {
-            const int mm = sj == 1 ? (DIR > 0 ? 1 : 2) : 0;
+            DeadRows d{rowmask, spec_a, spec_b, 0, 0};
+            const bool natural = sj == 1;
+            const bool active = natural ? rowmask != nullptr : spec_a < spec_b;
+            if (active) {
+                const bool in = (DIR > 0) == natural;
+                d.mode_in = in ? 1 : 0;
+                d.mode_out = in ? 0 : 1;
+            }
             const int r = cols_step_ct_dispatch<T, DIR>(data, batch, n2, n1, L, se, sj, nsub, twmode,
-                                                        tw2, live, rowmask, mm, s);
+                                                        tw2, live, d, s);
             if (r != 1) return r;
         }
         FftShape S{};
@@ -896,15 +918,24 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
                     const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
-                    hipStream_t s)
+                    int spec_a, int spec_b, hipStream_t s)
 {
     if (batch <= 0) return CG_OK;
     Live live{live_a, live_b};
     if (rows) return dir > 0 ? fft_rows<T, 1>(data, batch, n2, n1, tw, live, rowmask, s)
                              : fft_rows<T, -1>(data, batch, n2, n1, tw, live, rowmask, s);
     if (n2 <= 1) return CG_OK;
-    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, rowmask, s)
-                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, rowmask, s);
+    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, s)
+                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, s);
+}
+
+template <typename T>
+bool fft_spectrum_pruned(int n2)
+{
+    int na, nb;
+    if (n2 <= 1 || !fft_knob("CG_FFT_CT", 1) || !fft_col_split(n2, na, nb)) return false;
+    const int ln = log2_exact(na), lb = log2_exact(nb);
+    return ln >= 4 && ln <= 8 && lb >= 4 && lb <= 8;
 }
 
 template <typename T>
@@ -929,14 +960,14 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
     if (split && !tmp) return CG_ERR_ARG;
     int r;
     if (dir > 0) {
-        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, s))) return r;
-        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, s))) return r;
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, n2, n2, s))) return r;
+        if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, n2, n2, s))) return r;
         if (split) return col_permute<T>(data, tmp, batch, n2, n1, na, nb, 1, s);
         return CG_OK;
     }
     if (split && (r = col_permute<T>(data, tmp, batch, n2, n1, na, nb, 0, s))) return r;
-    if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, s))) return r;
-    return launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, s);
+    if ((r = launch_fft_pass<T>(data, batch, n2, n1, dir, false, tw2, n1, n1, nullptr, n2, n2, s))) return r;
+    return launch_fft_pass<T>(data, batch, n2, n1, dir, true, tw1, n1, n1, nullptr, n2, n2, s);
 }
 
 template <typename T>
@@ -959,8 +990,10 @@ int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s)
 
 template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, float2 *, hipStream_t);
 template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, double2 *, hipStream_t);
-template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, hipStream_t);
-template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, hipStream_t);
+template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, int, int, hipStream_t);
+template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, int, int, hipStream_t);
+template bool fft_spectrum_pruned<float>(int);
+template bool fft_spectrum_pruned<double>(int);
 template bool fft_supported<float>(int);
 template bool fft_supported<double>(int);
 template int launch_twiddles<float>(float2 *, int, hipStream_t);

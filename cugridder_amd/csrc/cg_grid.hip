@@ -42,6 +42,33 @@ __device__ __forceinline__ double es_val<double>(double d, double beta, double c
     return fabs(d) < hw ? exp(beta * sqrt(1.0 - c * d * d)) : 0.0;
 }
 
+// Kernel values of a point: es_val at the tap offsets, or the tabulated
+// (learned) kernel's per-tap polynomial in z = 2 f - 1, where f in [0, 1) is
+// the fractional part of the first tap's offset (off0 = f - kw/2).  The
+// choice and the degree are uniform across a launch.
+template <typename T>
+struct KernelEval {
+    T beta, ec, hw;
+    const float *poly;
+    int deg;
+    __device__ __forceinline__ explicit KernelEval(const GridArgs &g, int kw)
+        : beta((T)g.beta), ec((T)g.es_c), hw((T)(0.5 * kw)), poly(g.kpoly), deg(g.kdeg)
+    {
+    }
+    // value of tap t of a point whose first tap sits at offset off0
+    __device__ __forceinline__ T operator()(T off0, int t) const
+    {
+        if (poly) {
+            const T z = (T)2 * (off0 + hw) - (T)1;
+            const float *c = poly + t * (deg + 1);
+            T v = (T)c[deg];
+            for (int k = deg - 1; k >= 0; k--) v = v * z + (T)c[k];
+            return v;
+        }
+        return es_val<T>(off0 + (T)t, beta, ec, hw);
+    }
+};
+
 // gridding accumulates in FP64 whatever the grid precision: 16 bytes per cell
 int grid_lds_bytes(const GridArgs &g, int) { return g.pc * g.tx * g.ty * 16; }
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem)
@@ -120,7 +147,7 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
     }
     __syncthreads();
 
-    const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
+    const KernelEval<T> ker(g, KW);
     const int total = s_total;
     // all points of all source bins as one list, so every thread has work;
     // points are sorted by overlap class inside a bin, so the lanes of a
@@ -152,13 +179,13 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
         T kx[KW], ky[KW];
 #pragma unroll
         for (int t = 0; t < KW; t++) {
-            kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
-            ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
+            kx[t] = ker(G.x, t);
+            ky[t] = ker(G.y, t);
         }
         for (int tz = Z.lo; tz < Z.hi; tz++) {
             int zz = Z.s + tz;
             if (zz >= np) zz -= np;  // wraps only when one chunk holds every plane
-            const T kz = is2d ? (T)1 : es_val<T>(G.z + (T)tz, beta, ec, hw);
+            const T kz = is2d ? (T)1 : ker(G.z, tz);
             const T vzr = v.x * kz, vzi = v.y * kz;
 #pragma unroll
             for (int t2 = 0; t2 < KW; t2++) {
@@ -266,7 +293,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
     const int nlist = list[0];
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5, i = lane & 31;
-    const float beta = (float)g.beta, ec = (float)g.es_c, hw = 0.5f * KW;
+    const KernelEval<float> ker(g, KW);
     // lane roles: A[i][half] with tap i = t1 + 8 t2 (two 32-tap blocks),
     // B[half][j] with plane j = i.  Each lane evaluates one kernel value of
     // its point (i < 8: x tap i, 8 <= i < 16: y tap i - 8, 16 <= i < 24: z tap
@@ -315,8 +342,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
                 // this lane's kernel value, then the operands from the lanes
                 // holding them (every lane takes part in the shuffles)
                 const float off = axis == 0 ? G.x : axis == 1 ? G.y : G.z;
-                const float kv = (ok && axis < 3 && tap < KW) ? es_val<float>(off + (float)tap, beta, ec, hw)
-                                                             : 0.f;
+                const float kv = (ok && axis < 3 && tap < KW) ? ker(off, tap) : 0.f;
                 const float kx = __shfl(kv, src_kx);
                 const float ky0 = __shfl(kv, src_ky0);
                 const float ky1 = __shfl(kv, src_ky1);
@@ -455,7 +481,7 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
     const int bx = tile % g.ntx, by = tile / g.ntx;
     const int hx = g.tx + KW - 1, hy = g.ty + KW - 1;
     const int x0 = bx * g.tx, y0 = by * g.ty;
-    const T beta = (T)g.beta, ec = (T)g.es_c, hw = (T)(0.5 * KW);
+    const KernelEval<T> ker(g, KW);
     const bool single_stage = g.nchunk == 1;
     const int nzt = is2d ? 1 : KW;
 
@@ -505,14 +531,14 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
                 T kx[KW], ky[KW];
 #pragma unroll
                 for (int t = 0; t < KW; t++) {
-                    kx[t] = es_val<T>(G.x + (T)t, beta, ec, hw);
-                    ky[t] = es_val<T>(G.y + (T)t, beta, ec, hw);
+                    kx[t] = ker(G.x, t);
+                    ky[t] = ker(G.y, t);
                 }
                 for (int tz = 0; tz < nzt; tz++) {
                     int zz = rz + tz;
                     if (zz >= g.nf3) zz -= g.nf3;
                     if (zz >= np) continue;
-                    const T kzv = is2d ? (T)1 : es_val<T>(G.z + (T)tz, beta, ec, hw);
+                    const T kzv = is2d ? (T)1 : ker(G.z, tz);
                     const cx_t<T> *pl = lds + (size_t)zz * hy * hx + (size_t)ly * hx + lx;
                     T sr = 0, si = 0;
 #pragma unroll

@@ -7,8 +7,12 @@ namespace cg {
 
 // Geometry needed by the coordinate kernels (all FP64: north-star step (i)).
 struct CoordArgs {
-    double scale_uv;   // 2 pi f/c
+    double scale_uv;   // 2 pi f/c (one channel)
     double scale_w;    // 2 pi f/c * sign
+    // several channels: point j is row j / nchan at channel j % nchan, with
+    // scale_uv = chan_scale[channel] (device array of 2 pi f_c/c)
+    int nchan;
+    const double *chan_scale;
     double pix_x, pix_y;
     double i_center, gamma, o_center;
     int nf1, nf2, nf3, kw;
@@ -26,7 +30,7 @@ void set_coord_tiling(CoordArgs &a, const Tiling &t);
 int sort_key_bits(const CoordArgs &a, int64_t ntiles);
 
 // ---- prep (cg_prep.hip)
-// min/max of the scaled w (FP64) over nrow rows of uvw [nrow][3]; partial
+// min/max of w * scale_w (FP64) over nrow rows of uvw [nrow][3]; partial
 // buffer needs 2*1024 doubles.  Result copied to host (synchronises).
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
@@ -40,6 +44,8 @@ int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
 template <typename TU>
 int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a,
                    uint32_t *keys, int32_t *vals, hipStream_t s);
+// p[0..n) = v (a kernel, so that it can sit in a captured graph)
+int launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
 // mask[y] (nf2 bytes) = 1 for the grid rows that points of non-empty tiles
 // reach; the other rows are zero after gridding and unread by degridding
 int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
@@ -83,6 +89,11 @@ struct GridArgs {
     int cshift;     // sort key = tile << cshift | start cell inside the tile
     int dense_min;  // > 0: bins with at least this many points go to the dense path
     double beta, es_c;
+    // tabulated kernel (null: ES kernel from beta, es_c): tap t of a point
+    // whose first tap sits at offset -kw/2 + f, f in [0, 1), has the value
+    // sum_k kpoly[t (kdeg + 1) + k] z^k with z = 2 f - 1
+    const float *kpoly;
+    int kdeg;
 };
 template <typename T>
 int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
@@ -117,10 +128,17 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
 // rowmask (nf2 bytes, may be null): rows with mask 0 are zero on input to a
 // dir > 0 pass and not needed on output of a dir < 0 pass, so they may be
 // skipped (the row pass and the natural-order column step use it).
+// Frequencies k along n2 with spec_a <= k < spec_b are dead in the spectrum
+// (not needed after a dir > 0 column pass, zero before a dir < 0 one); the
+// split-order column step skips them when fft_spectrum_pruned(n2).
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
                     const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
-                    hipStream_t s);
+                    int spec_a, int spec_b, hipStream_t s);
+// true when the column pass honours the dead spectrum rows (then a dir < 0
+// column pass does not read them and they need not be written)
+template <typename T>
+bool fft_spectrum_pruned(int n2);
 // n = na * nb when columns of length n are transformed by the split
 bool fft_col_split(int n, int &na, int &nb);
 template <typename T>
@@ -142,6 +160,7 @@ int launch_fseries_grid(const Quad &q, int nf, double *phi, hipStream_t s);
 struct WArgs {
     int N1, N2, nf1, nf2, nf3;
     int cna, cnb;  // split of the column axis (0: natural order)
+    int skip_dead; // 1: the prologue leaves spectrum rows outside the image unwritten
     double pix_x, pix_y, o_center, i_center, gamma, h;
     Quad qd;
 };
@@ -163,5 +182,32 @@ int launch_dirty2ms_prologue(const TV *dirty, const double *zq, const double *ph
 template <typename T, typename TV>
 int launch_deconv2d(cx_t<T> *fw, TV *fk, const double *phi1, const double *phi2,
                     int N1, int N2, int nf1, int nf2, int type, hipStream_t s);
+
+// ---- learned kernel (cg_learn.hip)
+constexpr int LEARN_NPARAMS = 1153;  // 1 -> 32 -> 32 -> 1 tanh MLP
+constexpr int LEARN_MAXKW = 16;
+constexpr int LEARN_MAXQ = 32;       // half-line quadrature nodes, floor(2 + 1.5 kw)
+constexpr int LEARN_MAXDEG = 15;     // per-tap polynomial degree
+struct LearnQuad {
+    int kw, nq;
+    double beta;  // ES exponent the network corrects
+    double xq[LEARN_MAXQ], gq[LEARN_MAXQ];
+};
+double learn_beta(int kw);
+LearnQuad learn_quad(int kw);
+// tabulated form of a trained network: per-tap polynomial coefficients
+// (kw x (deg+1), FP32) and the kernel at the quadrature nodes qx (FP64)
+struct LearnedKernel {
+    int kw = 0, deg = 0, nq = 0;
+    float coef[LEARN_MAXKW * (LEARN_MAXDEG + 1)];
+    double qx[LEARN_MAXQ], qg[LEARN_MAXQ], qpsi[LEARN_MAXQ];
+};
+void cheb_to_monomial(int deg, double *zc, double *M);
+int learn_grad(int kw, const float *params, const double *u, const double *th, int nsamp,
+               float *grad, double *loss, hipStream_t s);
+int learn_train(int kw, int mode, const double *a, const double *b, int n, int batch, int steps,
+                double lr, float *params, double *losses, hipStream_t s);
+int learn_eval(const float *params, const float *s_in, int n, float *y, hipStream_t s);
+int learn_kernel_build(int kw, int deg, const float *params, LearnedKernel &K, hipStream_t s);
 
 }  // namespace cg

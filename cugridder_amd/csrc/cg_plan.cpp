@@ -133,7 +133,8 @@ int kernel_quadrature(const ConvOpts &o, double *xq, double *gq)
     for (int n = 0; n < p; n++) {
         double xn = x[n] * alpha;
         double phi = 0.0;
-        if (std::fabs(xn) <= o.halfwidth) phi = std::exp(o.beta * std::sqrt(1.0 - o.es_c * xn * xn));
+        if (o.tab_nq == p) phi = o.tab_psi[n];
+        else if (std::fabs(xn) <= o.halfwidth) phi = std::exp(o.beta * std::sqrt(1.0 - o.es_c * xn * xn));
         xq[n] = xn;
         gq[n] = alpha * w[n] * phi;
     }
@@ -189,7 +190,7 @@ Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,
 
 int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
                double freq, double eps, double sigma, int sign, bool single,
-               double wmin, double wmax)
+               double wmin, double wmax, const ConvOpts *conv)
 {
     if (nrow < 0 || nxdirty <= 0 || nydirty <= 0 || !(fov > 0) || !(freq > 0))
         return 2;
@@ -207,6 +208,7 @@ int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
     p.N2 = nxdirty;
     int ier = setup_conv_opts(eps, sigma, single, p.conv);
     if (ier >= 2) return ier;
+    if (conv) p.conv = *conv;
     p.ier = ier;
     p.f_over_c = freq / 299792458.0;
     // src/astro/cugridder.cu:48-76

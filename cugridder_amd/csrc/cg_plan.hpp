@@ -16,6 +16,10 @@ int next235beven(int n, int b);
 struct ConvOpts {
     int kw = 0;
     double beta = 0, es_c = 0, halfwidth = 0;
+    // tabulated (learned) kernel: its values at the kernel_quadrature nodes
+    // replace the ES formula when tab_nq equals the node count
+    int tab_nq = 0;
+    double tab_psi[32] = {};
 };
 // returns ier: 0 ok, 1 eps clamped, 2 invalid sigma
 int setup_conv_opts(double eps, double sigma, bool single, ConvOpts &o);
@@ -66,9 +70,11 @@ struct WPlan {
 
 // Fills every field from the call parameters and the w range (min/max of
 // the scaled w = w*f/c*sign*2pi over all rows).  Returns ier (>=2 is fatal).
+// conv, when given, replaces the kernel setup_conv_opts picks for eps (a
+// learned kernel of smaller support); every size follows its kw.
 int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
                double freq, double eps, double sigma, int sign, bool single,
-               double wmin_scaled, double wmax_scaled);
+               double wmin_scaled, double wmax_scaled, const ConvOpts *conv = nullptr);
 
 // Chooses tile extents and plane chunking for an LDS budget (bytes) and
 // complex element size; `nplanes` is the grid depth and `halo` the extra

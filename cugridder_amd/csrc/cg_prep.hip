@@ -21,6 +21,8 @@ CoordArgs make_coord_args(const WPlan &p, const Tiling &t)
     CoordArgs a{};
     a.scale_uv = 2.0 * kPi * p.f_over_c;
     a.scale_w = 2.0 * kPi * p.f_over_c * p.sign;
+    a.nchan = 1;
+    a.chan_scale = nullptr;
     a.pix_x = p.pix_x;
     a.pix_y = p.pix_y;
     a.i_center = p.i_center;
@@ -86,16 +88,30 @@ __device__ __forceinline__ void footprint_1d(double g, int n, int kw, int &i0, d
     i0 = wrapi((int)s, n);
 }
 
+// point j -> row of uvw and the coordinate scales of its channel
+struct PointScale {
+    int64_t row;
+    double suv, sw;
+};
+__device__ __forceinline__ PointScale point_scale(int64_t j, const CoordArgs &a)
+{
+    if (a.nchan <= 1) return PointScale{j, a.scale_uv, a.scale_w};
+    const int64_t r = j / a.nchan;
+    const double s = a.chan_scale[j - r * a.nchan];
+    return PointScale{r, s, a.scale_w < 0 ? -s : s};
+}
+
 // grid axis 1 (fast) <- v * pix_y, axis 2 <- u * pix_x (cugridder.cu:191)
-__device__ __forceinline__ Foot make_foot(double u, double v, double w, const CoordArgs &a)
+__device__ __forceinline__ Foot make_foot(double u, double v, double w, const CoordArgs &a,
+                                          double suv, double sw)
 {
     Foot f;
-    double xs = v * a.scale_uv * a.pix_y;
-    double ys = u * a.scale_uv * a.pix_x;
+    double xs = v * suv * a.pix_y;
+    double ys = u * suv * a.pix_x;
     footprint_1d(shift_rescale(xs, a.nf1, 0), a.nf1, a.kw, f.x0, f.d1);
     footprint_1d(shift_rescale(ys, a.nf2, 0), a.nf2, a.kw, f.y0, f.d2);
     if (a.has_w) {
-        double wp = (w * a.scale_w - a.i_center) / a.gamma;
+        double wp = (w * sw - a.i_center) / a.gamma;
         footprint_1d(shift_rescale(wp, a.nf3, 0), a.nf3, a.kw, f.z0, f.d3);
     } else {
         f.z0 = 0;
@@ -186,11 +202,13 @@ template <typename TU>
 __global__ __launch_bounds__(256) void k_keys(const TU *uvw, int64_t nrow, CoordArgs a,
                                               uint32_t *keys, int32_t *vals)
 {
+    // j runs over the points: rows x channels
     int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (j >= nrow) return;
     CoordArgs b = a;
     b.has_w = 0;  // the key only depends on the (u, v) footprint
-    Foot f = make_foot((double)uvw[3 * j], (double)uvw[3 * j + 1], 0.0, b);
+    const PointScale ps = point_scale(j, a);
+    Foot f = make_foot((double)uvw[3 * ps.row], (double)uvw[3 * ps.row + 1], 0.0, b, ps.suv, ps.sw);
     keys[j] = tile_key(f, a);
     vals[j] = (int32_t)j;
 }
@@ -291,6 +309,20 @@ __global__ __launch_bounds__(256) void k_row_mask(const int32_t *__restrict__ ti
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s)
+{
+    if (n <= 0) return CG_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, p, n, v);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
 int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
                     hipStream_t s)
 {
@@ -328,14 +360,17 @@ __global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, c
 {
     int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (i >= nrow) return;
+    // point j = row * nchan + channel; vis and wgt are [rows][channels]
     int64_t j = perm[i];
     if (j < 0 || j >= nrow) j = 0;  // a valid permutation never takes this
-    double u = (double)uvw[3 * j], v = (double)uvw[3 * j + 1], w = (double)uvw[3 * j + 2];
-    Foot f = make_foot(u, v, w, a);
+    const PointScale ps = point_scale(j, a);
+    const int64_t r = ps.row;
+    double u = (double)uvw[3 * r], v = (double)uvw[3 * r + 1], w = (double)uvw[3 * r + 2];
+    Foot f = make_foot(u, v, w, a, ps.suv, ps.sw);
     uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8) | ((uint32_t)f.z0 << 16);
     geo[i] = make_geo<T>(f.d1, f.d2, f.d3, pk);
     double wt = wgt ? (double)wgt[j] : 1.0;
-    double ws = w * a.scale_w;
+    double ws = w * ps.sw;
     double2 c;
     if (!adjoint) {
         // ms2dirty: vis * wgt (ra_exec.cu:103-111) * e^{i o_c w} (cuft.cu:46-80)

@@ -32,6 +32,7 @@ WArgs make_wargs(const WPlan &p)
     w.h = p.h;
     w.qd = make_quad(p.conv);
     if (!(p.nf2 > 1 && fft_col_split(p.nf2, w.cna, w.cnb))) w.cna = w.cnb = 0;
+    w.skip_dead = fft_spectrum_pruned<float>(p.nf2) ? 1 : 0;
     return w;
 }
 
@@ -207,7 +208,10 @@ __global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dir
     const int row = cell_to_img(spec_freq(r, w.cna, w.cnb), w.N2, w.nf2);
     cx_t<T> *dst = fw + (size_t)r * w.nf1 + w1;
     if (row < 0) {
-        for (int p = 0; p < w.nf3; p++) dst[plane * p] = mkc((T)0, (T)0);
+        // a frequency outside the image rows: zero, unless the column FFT
+        // treats it as zero without reading it
+        if (!w.skip_dead)
+            for (int p = 0; p < w.nf3; p++) dst[plane * p] = mkc((T)0, (T)0);
         return;
     }
     const int dc = col - w.N1 / 2, dr = row - w.N2 / 2;

@@ -95,6 +95,29 @@ int nufft2d1f(int M, const float *x, const float *y, const float *c, int N1,
 int nufft2d2f(int M, const float *x, const float *y, float *c, int N1, int N2,
               const float *fk, float epsilon, float sigma);
 
+/* Several channels (no counterpart in the reference C-ABI: its binding passes
+ * freq[0] only, python/curagridder/cursl.py:103).  freq[nchan] (Hz, host),
+ * vis and wgt are [nrow][nchan] (NIFTY layout); every (row, channel) is one
+ * visibility at uvw * freq[c] / c and the dirty image sums all of them.
+ * wgt may be NULL.  Double: complex128 vis/dirty; mixed: complex64 vis/dirty
+ * with FP32 wgt. */
+int ms2dirty_mc_2(int nrow, int nchan, int nxdirty, int nydirty, double fov,
+                  const double *freq, const double *uvw, const double *vis,
+                  const double *wgt, double *dirty, double epsilon, double sigma,
+                  int sign);
+int dirty2ms_mc_2(int nrow, int nchan, int nxdirty, int nydirty, double fov,
+                  const double *freq, const double *uvw, double *vis,
+                  const double *wgt, const double *dirty, double epsilon,
+                  double sigma, int sign);
+int ms2dirty_mc_mixed(int nrow, int nchan, int nxdirty, int nydirty, double fov,
+                      const double *freq, const double *uvw, const float *vis,
+                      const float *wgt, float *dirty, double epsilon,
+                      double sigma, int sign);
+int dirty2ms_mc_mixed(int nrow, int nchan, int nxdirty, int nydirty, double fov,
+                      const double *freq, const double *uvw, float *vis,
+                      const float *wgt, const float *dirty, double epsilon,
+                      double sigma, int sign);
+
 /* Contexts: a reusable device workspace bound to a HIP stream (NULL = the
  * null stream).  Device-pointer calls below run the same pipeline as the
  * host entry points on inputs already in HBM. */
@@ -128,6 +151,47 @@ int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty,
                     double fov, double freq, const void *d_uvw, int uvw_prec,
                     void *d_vis, const void *d_wgt, int wgt_prec,
                     const void *d_dirty, double epsilon, double sigma, int sign);
+/* the same with nchan channels: freq[nchan] on the host, d_vis and d_wgt
+ * [nrow][nchan] on the device */
+int cg_ms2dirty_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
+                       int nydirty, double fov, const double *freq,
+                       const void *d_uvw, int uvw_prec, const void *d_vis,
+                       const void *d_wgt, int wgt_prec, void *d_dirty,
+                       double epsilon, double sigma, int sign);
+int cg_dirty2ms_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
+                       int nydirty, double fov, const double *freq,
+                       const void *d_uvw, int uvw_prec, void *d_vis,
+                       const void *d_wgt, int wgt_prec, const void *d_dirty,
+                       double epsilon, double sigma, int sign);
+/* Batches of independent channels (no counterpart in the reference, which
+ * images one channel per call): every item has its own rows, image size,
+ * field of view and frequency; complex64 vis/image, FP64 uvw, FP32 wgt (may
+ * be NULL), all on the device.  cg_batch_create plans every item (flags:
+ * CG_BATCH_AUTOTUNE picks tile sizes and the dense-gridding threshold per
+ * image size and visibility count from measured runs; CG_BATCH_GRAPH
+ * captures the whole batch into one HIP graph), runs it once, and keeps a
+ * private workspace; cg_batch_run replays it on the context's stream
+ * (direction 0: vis -> dirty, 1: dirty -> vis).  The device buffers must
+ * stay valid and keep their w-range while the batch exists. */
+typedef struct {
+    int nrow, nxdirty, nydirty;
+    double fov, freq;
+    const void *d_uvw;
+    void *d_vis;
+    const void *d_wgt;
+    void *d_dirty;
+} cg_batch_item;
+#define CG_BATCH_AUTOTUNE 1
+#define CG_BATCH_GRAPH 2
+int cg_batch_create(void **batch, void *ctx, int direction, int n,
+                    const cg_batch_item *items, double epsilon, double sigma,
+                    int sign, int flags);
+int cg_batch_run(void *batch);
+/* out[5 * i ...] = tile_grid, tile_degrid, dense threshold in use, tuned ms
+ * per run, nf3 of item i; returns 1 when the batch runs as a graph */
+int cg_batch_info(void *batch, double *out);
+int cg_batch_destroy(void *batch);
+
 /* type 1 reads d_c and writes d_fk; type 2 reads d_fk and writes d_c */
 int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x,
                    const void *d_y, void *d_c, int N1, int N2, void *d_fk,
@@ -167,6 +231,50 @@ int cg_sort_pairs_dev(void *ctx, void *d_keys, void *d_vals, int m,
 int cg_memcpy_dev(void *dst, const void *src, void *stream, size_t bytes);
 
 int cg_version(void);
+
+/* ---- Learned gridding kernel.  No counterpart in the reference (which has
+ * only the ES kernel, src/fourier/conv_interp_invoker.cu:49-119); these are
+ * for callers who trade a network-defined kernel of smaller support for
+ * fewer taps or a lower error.  The kernel is
+ *   psi(d) = exp(beta (sqrt(1 - x^2) - 1) + y(s)), x = 2 d / kw, s = 2 x^2 - 1,
+ *   |d| < kw/2, y = W3 . tanh(W2 tanh(W1 s + b1) + b2) + b3 (1 -> 32 -> 32 -> 1,
+ *   FP32), beta the ES kernel's for sigma = 2 (conv_interp_invoker.cu:96-104),
+ * so W3 = b3 = 0 is the ES kernel; the parameter vector is W1[32] b1[32] W2[32][32] b2[32] W3[32] b3
+ * (cg_learn_nparams() = 1153 floats; row o of W2 feeds hidden unit o).  Its
+ * correction factors use the Gauss-Legendre rule the reference applies to
+ * the ES kernel (src/fourier/deconv.cu:80-94).  All arrays are host arrays;
+ * the calls are synchronous on the default stream. */
+int cg_learn_nparams(void);
+/* Gridding error functional on samples (u[i], theta[i]) (fractional grid
+ * offset, image mode in radians per grid cell):
+ *   loss = mean |sum_t psi(d_t) e^{-i theta d_t} / psihat(theta) - 1|^2,
+ *   d_t = ceil(u - kw/2) + t - u, psihat the quadrature transform of psi,
+ * and its gradient grad[1153] with respect to the parameters. */
+int cg_learn_grad(int kw, const float *params, const double *u,
+                  const double *theta, int nsamp, float *grad, double *loss);
+/* Adam training on the GPU (lr, betas 0.9 / 0.999), minibatches of `batch`
+ * samples taken in order.  mode 1: minimise the functional above on (a, b) =
+ * (u, theta); mode 0: least squares of y(s(d)) to targets, (a, b) = (d,
+ * log psi(d)).  params is read and overwritten; losses[steps] may be NULL. */
+int cg_learn_train(int kw, int mode, const double *a, const double *b, int n,
+                   int batch, int steps, double lr, float *params,
+                   double *losses);
+/* Network output y at n inputs s (inference; without the ES exponent). */
+int cg_learn_eval(const float *params, const float *s, int n, float *y);
+/* Tabulated form the gridders use: coef[t (deg + 1) + k], the coefficient of
+ * z^k of tap t, with psi(-kw/2 + t + (z + 1)/2) ~ sum_k coef z^k on [-1, 1]
+ * (Chebyshev interpolation of degree deg <= 15); qpsi[p] = psi at the
+ * p = floor(2 + 1.5 kw) quadrature nodes. */
+int cg_learn_tabulate(int kw, int deg, const float *params, float *coef,
+                      double *qpsi);
+/* Installs a learned kernel of support kw in a context, certified by the
+ * caller at relative error eps (eps <= 0 removes it).  Later complex64 /
+ * mixed-precision calls on the context whose epsilon is >= eps and whose ES
+ * kernel is not narrower use it (narrowest first); option "use_learned" 0
+ * turns this off.  complex128 calls and the reference entry points always
+ * use the ES kernel.  Batches copy the context's kernels when created. */
+int cg_ctx_set_kernel(void *ctx, int kw, int deg, const float *params,
+                      double eps);
 
 #ifdef __cplusplus
 }

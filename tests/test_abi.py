@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_python_binding_declares_argtypes():
     L = cugridder_amd.lib()
-    assert L.cg_version() == 101
+    assert L.cg_version() == 102
     for n in declared_functions():
         assert getattr(L, n).restype is ctypes.c_int
 
