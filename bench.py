@@ -9,7 +9,11 @@ The default workload is the full W-stacking configuration (8192^2 image,
 fov 3 deg, 1.4 GHz, 2^27 visibilities from 256-antenna Earth-rotation tracks,
 eps 1e-6).  `--workload plain` is the one-channel 1024^2 case (131072 rows,
 eps 1e-5) and `--workload nufft2d` the 2-D type-1 + type-2 NUFFT (2^25
-points, 4096^2 modes, eps 1e-5).
+points, 4096^2 modes, eps 1e-5).  `--workload batch` runs 64 channels of
+mixed sizes as two graph-captured batches, and `--workload learned` the full
+W-stacking configuration at eps 1e-5 with a learned support-6 kernel that is
+trained on the GPU and certified against the ES support-7 kernel before the
+timed steps (the ES kernel is used when certification fails).
 """
 import argparse
 import json
@@ -143,6 +147,48 @@ def make_nufft2d(torch, dev, M=1 << 25, N=4096, seed=2):
 
 # ---------------------------------------------------------------- roofline
 
+def setup_learned(torch, ctx, W, args, kw=6):
+    """Trains the support-kw kernel on the GPU (2^20 seeded visibilities of a
+    256^2 patch), certifies it at the workload's eps against the ES judge
+    (support 7) and installs it in ctx when accepted; also times one step of
+    the ES support-7 path (eps 1e-6, the judge's accuracy) for comparison.
+    Untimed set-up: returns what the JSON line reports about it."""
+    from cugridder_amd import learned
+    W["eps"] = 1e-5
+    W["name"] = W["name"].replace("eps 1e-6", "eps 1e-5, learned kernel")
+    t0 = time.perf_counter()
+    params, losses = learned.train(kw)
+    t_train = time.perf_counter() - t0
+    log("trained kw %d in %.1f s, final loss %.3e" % (kw, t_train, float(np.mean(losses[-100:]))))
+    cert = learned.certify(kw, params, W["eps"])
+    log("certification: %s" % json.dumps(cert))
+    if cert["accepted"]:
+        ctx.set_kernel(kw, params, W["eps"])
+    out_d = torch.empty(W["nx"], W["ny"], dtype=torch.complex64, device=W["vis"].device)
+    out_v = torch.empty(W["M"], dtype=torch.complex64, device=W["vis"].device)
+    from cugridder_amd import Context
+    es = Context()
+
+    def es_step():
+        es.ms2dirty(W["uvw"], W["vis"], W["wgt"], W["nx"], W["ny"], W["fov"], W["freq"], 1e-6,
+                    out=out_d)
+        es.dirty2ms(W["uvw"], W["img"], W["wgt"], W["fov"], W["freq"], 1e-6, out=out_v)
+
+    es_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    es_step()
+    torch.cuda.synchronize()
+    es_ms = (time.perf_counter() - t0) * 1e3
+    es_kw = es.last_plan()["kw"]
+    es.close()
+    log("ES support %d at eps 1e-6: %.1f ms per step" % (es_kw, es_ms))
+    return {"kw": kw, "accepted": cert["accepted"], "certified_rel_l2": [cert["ms2dirty"],
+            cert["dirty2ms"]], "judge": "ES support 7, eps 1e-6", "train_s": round(t_train, 2),
+            "es_judge_kw": es_kw, "es_judge_ms_per_step": round(es_ms, 2),
+            "es_judge_Mvis_s": round(W["M"] / (es_ms * 1e-3) / 1e6, 3)}
+
+
 def stage_kernel(stage, direction, W, plan):
     """(kernel name, launches, algorithmic HBM bytes per launch) of a pipeline
     stage, complex64 grid.  Records are 24 B per row (FP32 offsets + packed
@@ -230,7 +276,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d", "batch"), default="wstack")
+    ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d", "batch", "learned"),
+                    default="wstack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--nrow", type=int, default=1 << 27, help="rows of the wstack workload")
     ap.add_argument("--copy-probe", action="store_true",
@@ -251,7 +298,7 @@ def main():
         W = make_plain(torch, dev)
     elif args.workload == "batch":
         W = make_batch(torch, dev)
-    elif args.workload == "wstack":
+    elif args.workload in ("wstack", "learned"):
         W = make_wstack_full(torch, dev, nrow=args.nrow)
     else:
         W = make_nufft2d(torch, dev)
@@ -264,6 +311,10 @@ def main():
         del src, dst
         log("copy probe: 1 GiB read + 1 GiB written")
     ctx = Context(timing=True)
+
+    learned_info = None
+    if args.workload == "learned":
+        learned_info = setup_learned(torch, ctx, W, args)
 
     single_ms = None
     if args.workload == "batch":
@@ -381,6 +432,8 @@ def main():
     elif args.workload != "nufft2d":
         cfg.update({"image": "%dx%d" % (W["nx"], W["ny"]), "fov_deg": W["fov"], "freq_hz": W["freq"],
                     "nf": [plan["nf1"], plan["nf2"], plan["nf3"]], "kw": plan["kw"]})
+    if learned_info is not None:
+        cfg["learned"] = learned_info
     out = {"metric": METRIC, "value": round(value, 3), "unit": unit, "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",

@@ -122,8 +122,10 @@ def response(p, u, theta, kw):
     return a / psihat(p, theta, kw)
 
 
-def loss_and_grad(p, u, theta, kw):
-    """mean |F - 1|^2 and its gradient with respect to the parameters."""
+def loss_and_grad(p, u, theta, kw, node_grad=True):
+    """mean |F - 1|^2 and its gradient with respect to the parameters
+    (node_grad False leaves out the path through psihat, for tests that show
+    the deconvolution-factor backward pass is needed)."""
     u = np.asarray(u, np.float64)
     theta = np.asarray(theta, np.float64)
     n = u.size
@@ -146,7 +148,9 @@ def loss_and_grad(p, u, theta, kw):
     # dL/dpsihat = -2 Re(conj(E) F) / ph / n; dpsihat/dpsi_q = cq_mat
     dph = -2.0 * np.real(np.conj(e) * (e + 1.0)) / ph_ / n
     dpsi_q = dph @ cq_mat
-    g = backward(p, ct, (dpsi_t * vt).ravel()) + backward(p, cq, dpsi_q * vq)
+    g = backward(p, ct, (dpsi_t * vt).ravel())
+    if node_grad:
+        g = g + backward(p, cq, dpsi_q * vq)
     return loss, g
 
 
