@@ -63,7 +63,6 @@ struct Ctx {
     struct Learned {
         double eps = 0;  // 0: none
         LearnedKernel K;
-        DevBuf coef;
     } learned[LEARN_MAXKW + 1];
     int use_learned = 1;
 
@@ -74,8 +73,6 @@ struct Ctx {
                           &rmask, &chans};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
-        for (auto &L : learned)
-            if (L.coef.p) (void)hipFree(L.coef.p);
         if (ev_ok)
             for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -200,9 +197,9 @@ static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, in
 {
     GridArgs g{};
     if (P.conv.tab_nq > 0) {
-        const Ctx::Learned &L = C.learned[P.conv.kw];
-        g.kpoly = as<float>(const_cast<DevBuf &>(L.coef));
-        g.kdeg = L.K.deg;
+        const LearnedKernel &K = C.learned[P.conv.kw].K;
+        g.learned = 1;
+        for (int k = 0; k <= LEARN_MAXDEG; k++) g.kq[k] = K.coef[k];
     }
     g.cshift = cshift;
     g.m = (int)m;
@@ -255,11 +252,15 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
         for (int kw = 2; kw <= P.conv.kw; kw++) {
             const Ctx::Learned &L = C.learned[kw];
             if (!(L.eps > 0) || L.eps > eps) continue;
+            // the gridders evaluate exp(beta sqrt(1 - x^2) + y): the node
+            // values take the same normalisation (e^beta times psi)
             ConvOpts o = P.conv;
             o.kw = kw;
             o.halfwidth = kw / 2.0;
+            o.beta = L.K.beta;
+            o.es_c = 4.0 / ((double)kw * kw);
             o.tab_nq = L.K.nq;
-            for (int q = 0; q < L.K.nq; q++) o.tab_psi[q] = L.K.qpsi[q];
+            for (int q = 0; q < L.K.nq; q++) o.tab_psi[q] = L.K.qpsi[q] * std::exp(L.K.beta);
             ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin,
                              wmax, &o);
             if (ier >= 2) return ier;
@@ -547,8 +548,8 @@ int dirty2ms_dev(Ctx &C, int64_t nrow, int nchan, const double *freq, int nx, in
 // batch's private workspace reaches its final size, then the executions of
 // all items are captured into one HIP graph that cg_batch_run replays.
 
-// installs a tabulated kernel in the context (device copy of its
-// coefficients); eps <= 0 removes the kernel of that support
+// installs a tabulated kernel in the context; eps <= 0 removes the kernel
+// of that support
 static int set_learned(Ctx &C, const LearnedKernel &K, double eps)
 {
     Ctx::Learned &L = C.learned[K.kw];
@@ -556,10 +557,6 @@ static int set_learned(Ctx &C, const LearnedKernel &K, double eps)
         L.eps = 0;
         return CG_OK;
     }
-    const size_t bytes = sizeof(float) * K.kw * (K.deg + 1);
-    CG_TRY(ensure(L.coef, bytes));
-    CG_HIP(hipMemcpyAsync(L.coef.p, K.coef, bytes, hipMemcpyHostToDevice, C.stream));
-    CG_HIP(hipStreamSynchronize(C.stream));
     L.K = K;
     L.eps = eps;
     return CG_OK;
@@ -1451,7 +1448,7 @@ int cg_learn_tabulate(int kw, int deg, const float *params, float *coef, double 
     if (!coef || !qpsi) return CG_ERR_ARG;
     LearnedKernel K;
     CG_TRY(learn_kernel_build(kw, deg, params, K, nullptr));
-    std::memcpy(coef, K.coef, sizeof(float) * kw * (deg + 1));
+    std::memcpy(coef, K.coef, sizeof(float) * (deg + 1));
     std::memcpy(qpsi, K.qpsi, sizeof(double) * K.nq);
     return CG_OK;
 }

@@ -42,30 +42,34 @@ __device__ __forceinline__ double es_val<double>(double d, double beta, double c
     return fabs(d) < hw ? exp(beta * sqrt(1.0 - c * d * d)) : 0.0;
 }
 
-// Kernel values of a point: es_val at the tap offsets, or the tabulated
-// (learned) kernel's per-tap polynomial in z = 2 f - 1, where f in [0, 1) is
-// the fractional part of the first tap's offset (off0 = f - kw/2).  The
-// choice and the degree are uniform across a launch.
+// Kernel values of a point: the ES kernel exp(beta sqrt(1 - c d^2)) at the
+// tap offsets d, or the learned kernel, whose exponent adds the network's
+// polynomial sum_k q_k s^k in s = 2 c d^2 - 1 (c = (2/kw)^2).  The choice is
+// uniform across a launch and the coefficients are launch arguments, so
+// they stay in scalar registers.
 template <typename T>
 struct KernelEval {
     T beta, ec, hw;
-    const float *poly;
-    int deg;
+    int learned;
+    T q[LEARN_MAXDEG + 1];
     __device__ __forceinline__ explicit KernelEval(const GridArgs &g, int kw)
-        : beta((T)g.beta), ec((T)g.es_c), hw((T)(0.5 * kw)), poly(g.kpoly), deg(g.kdeg)
+        : beta((T)g.beta), ec((T)g.es_c), hw((T)(0.5 * kw)), learned(g.learned)
     {
+#pragma unroll
+        for (int k = 0; k <= LEARN_MAXDEG; k++) q[k] = (T)g.kq[k];
     }
     // value of tap t of a point whose first tap sits at offset off0
     __device__ __forceinline__ T operator()(T off0, int t) const
     {
-        if (poly) {
-            const T z = (T)2 * (off0 + hw) - (T)1;
-            const float *c = poly + t * (deg + 1);
-            T v = (T)c[deg];
-            for (int k = deg - 1; k >= 0; k--) v = v * z + (T)c[k];
-            return v;
-        }
-        return es_val<T>(off0 + (T)t, beta, ec, hw);
+        const T d = off0 + (T)t;
+        if (!learned) return es_val<T>(d, beta, ec, hw);
+        if (!(fabs(d) < hw)) return (T)0;
+        const T r = (T)1 - ec * d * d;
+        const T s = (T)1 - (T)2 * r;
+        T y = q[LEARN_MAXDEG];
+#pragma unroll
+        for (int k = LEARN_MAXDEG - 1; k >= 0; k--) y = y * s + q[k];
+        return exp(beta * sqrt(r) + y);
     }
 };
 

@@ -81,6 +81,12 @@ int64_t scan_tmp_size(int64_t n);
 // exclusive scan of n ints in place
 int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *scan_tmp, hipStream_t s);
 
+// learned kernel sizes
+constexpr int LEARN_NPARAMS = 1153;  // 1 -> 32 -> 32 -> 1 tanh MLP
+constexpr int LEARN_MAXKW = 16;
+constexpr int LEARN_MAXQ = 32;       // half-line quadrature nodes, floor(2 + 1.5 kw)
+constexpr int LEARN_MAXDEG = 15;     // degree of the network's polynomial in s
+
 // ---- gridding / degridding (cg_grid.hip)
 struct GridArgs {
     int m;  // number of points (rows)
@@ -89,11 +95,11 @@ struct GridArgs {
     int cshift;     // sort key = tile << cshift | start cell inside the tile
     int dense_min;  // > 0: bins with at least this many points go to the dense path
     double beta, es_c;
-    // tabulated kernel (null: ES kernel from beta, es_c): tap t of a point
-    // whose first tap sits at offset -kw/2 + f, f in [0, 1), has the value
-    // sum_k kpoly[t (kdeg + 1) + k] z^k with z = 2 f - 1
-    const float *kpoly;
-    int kdeg;
+    // learned kernel (learned != 0): the value at offset d is
+    // exp(beta sqrt(1 - es_c d^2) + sum_k kq[k] s^k), s = 2 es_c d^2 - 1,
+    // instead of the ES kernel's exp(beta sqrt(1 - es_c d^2))
+    int learned;
+    float kq[LEARN_MAXDEG + 1];
 };
 template <typename T>
 int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
@@ -184,10 +190,6 @@ int launch_deconv2d(cx_t<T> *fw, TV *fk, const double *phi1, const double *phi2,
                     int N1, int N2, int nf1, int nf2, int type, hipStream_t s);
 
 // ---- learned kernel (cg_learn.hip)
-constexpr int LEARN_NPARAMS = 1153;  // 1 -> 32 -> 32 -> 1 tanh MLP
-constexpr int LEARN_MAXKW = 16;
-constexpr int LEARN_MAXQ = 32;       // half-line quadrature nodes, floor(2 + 1.5 kw)
-constexpr int LEARN_MAXDEG = 15;     // per-tap polynomial degree
 struct LearnQuad {
     int kw, nq;
     double beta;  // ES exponent the network corrects
@@ -195,11 +197,13 @@ struct LearnQuad {
 };
 double learn_beta(int kw);
 LearnQuad learn_quad(int kw);
-// tabulated form of a trained network: per-tap polynomial coefficients
-// (kw x (deg+1), FP32) and the kernel at the quadrature nodes qx (FP64)
+// tabulated form of a trained network: y as a polynomial in s (coef[k] of
+// s^k, zero-padded to LEARN_MAXDEG) and the kernel exp(beta (sqrt(1 - x^2)
+// - 1) + y) at the quadrature nodes qx (FP64)
 struct LearnedKernel {
     int kw = 0, deg = 0, nq = 0;
-    float coef[LEARN_MAXKW * (LEARN_MAXDEG + 1)];
+    double beta = 0;
+    float coef[LEARN_MAXDEG + 1];
     double qx[LEARN_MAXQ], qg[LEARN_MAXQ], qpsi[LEARN_MAXQ];
 };
 void cheb_to_monomial(int deg, double *zc, double *M);

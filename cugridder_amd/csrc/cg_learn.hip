@@ -23,8 +23,8 @@
 // the taps, F, loss, dL/dy at the taps, dL/dpsi at the quadrature nodes),
 // k_learn_fit (least squares to a target log-kernel, the warm start),
 // k_mlp_backward (parameter gradients of a list of (s, dL/dy), reduced per
-// block in LDS), k_adam, k_learn_tabulate (per-tap polynomial coefficients
-// for the gridders, from values at Chebyshev points).
+// block in LDS), k_adam, k_learn_tabulate (y as a polynomial in s for the
+// gridders, from its values at Chebyshev points).
 #include <cmath>
 #include <vector>
 
@@ -310,33 +310,30 @@ __global__ __launch_bounds__(64) void k_fill_f64(double *__restrict__ p, int n)
     if (i < n) p[i] = 0.0;
 }
 
-// Per-tap polynomial coefficients: psi(-kw/2 + t + (z + 1)/2) = sum_k c[t][k] z^k,
-// z in [-1, 1].  Values at the deg+1 Chebyshev points zc are mapped to
-// monomial coefficients by M (row k: coefficient k).  One block.
-__global__ __launch_bounds__(256) void k_learn_tabulate(const float *__restrict__ params, int kw,
-                                                        double beta, int deg,
-                                                        const double *__restrict__ zc,
-                                                        const double *__restrict__ M,
-                                                        float *__restrict__ coef)
+// Tabulated form for the gridders: the network output y as a polynomial in
+// s on [-1, 1], y(s) ~ sum_k coef[k] s^k, from its values at the deg+1
+// Chebyshev points sc (FP64 values mapped by M, row k: coefficient k).
+// One block.
+__global__ __launch_bounds__(64) void k_learn_tabulate(const float *__restrict__ params, int deg,
+                                                       const double *__restrict__ sc,
+                                                       const double *__restrict__ M,
+                                                       float *__restrict__ coef)
 {
     __shared__ float p[kNParams];
-    __shared__ double vals[LEARN_MAXKW * (LEARN_MAXDEG + 1)];
+    __shared__ double vals[LEARN_MAXDEG + 1];
     for (int k = threadIdx.x; k < kNParams; k += blockDim.x) p[k] = params[k];
     __syncthreads();
     const int np = deg + 1;
-    for (int k = threadIdx.x; k < kw * np; k += blockDim.x) {
-        const int t = k / np, j = k % np;
-        const double d = -0.5 * kw + t + 0.5 * (zc[j] + 1.0);
+    for (int j = threadIdx.x; j < np; j += blockDim.x) {
         float h1[kH], h2[kH], y;
-        mlp_forward(p, mlp_input(d, kw), h1, h2, y);
-        vals[k] = fabs(d) < 0.5 * kw ? exp(es_prior(d, kw, beta) + (double)y) : 0.0;
+        mlp_forward(p, (float)sc[j], h1, h2, y);
+        vals[j] = (double)y;
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kw * np; k += blockDim.x) {
-        const int t = k / np, c = k % np;
+    for (int c = threadIdx.x; c < np; c += blockDim.x) {
         double a = 0.0;
-        for (int j = 0; j < np; j++) a += M[c * np + j] * vals[t * np + j];
-        coef[k] = (float)a;
+        for (int j = 0; j < np; j++) a += M[c * np + j] * vals[j];
+        coef[c] = (float)a;
     }
 }
 
@@ -568,31 +565,34 @@ int learn_kernel_build(int kw, int deg, const float *params, LearnedKernel &K, h
 {
     if (kw < 2 || kw > LEARN_MAXKW || deg < 1 || deg > LEARN_MAXDEG || !params) return CG_ERR_ARG;
     const int np = deg + 1;
-    std::vector<double> zc(np), M(np * np);
-    cheb_to_monomial(deg, zc.data(), M.data());
+    std::vector<double> sc(np), M(np * np);
+    cheb_to_monomial(deg, sc.data(), M.data());
     const LearnQuad lq = learn_quad(kw);
     LearnWs ws;
     float *dp, *dcoef;
-    double *dzc, *dM, *dpsiq;
+    double *dsc, *dM, *dpsiq;
     CG_TRY(ws.get(dp, kNParams));
-    CG_TRY(ws.get(dcoef, kw * np));
-    CG_TRY(ws.get(dzc, np));
+    CG_TRY(ws.get(dcoef, np));
+    CG_TRY(ws.get(dsc, np));
     CG_TRY(ws.get(dM, np * np));
     CG_TRY(ws.get(dpsiq, LEARN_MAXQ));
     CG_HIP(hipMemcpyAsync(dp, params, sizeof(float) * kNParams, hipMemcpyHostToDevice, s));
-    CG_HIP(hipMemcpyAsync(dzc, zc.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+    CG_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
     CG_HIP(hipMemcpyAsync(dM, M.data(), sizeof(double) * np * np, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_learn_tabulate, dim3(1), dim3(256), 0, s, (const float *)dp, kw, lq.beta,
-                       deg, (const double *)dzc, (const double *)dM, dcoef);
+    hipLaunchKernelGGL(k_learn_tabulate, dim3(1), dim3(64), 0, s, (const float *)dp, deg,
+                       (const double *)dsc, (const double *)dM, dcoef);
     hipLaunchKernelGGL(k_node_psi, dim3(1), dim3(64), 0, s, (const float *)dp, lq, dpsiq);
     CG_HIP(hipGetLastError());
     K = LearnedKernel{};
     K.kw = kw;
     K.deg = deg;
     K.nq = lq.nq;
-    CG_HIP(hipMemcpyAsync(K.coef, dcoef, sizeof(float) * kw * np, hipMemcpyDeviceToHost, s));
+    K.beta = lq.beta;
+    float coef[LEARN_MAXDEG + 1] = {};
+    CG_HIP(hipMemcpyAsync(coef, dcoef, sizeof(float) * np, hipMemcpyDeviceToHost, s));
     CG_HIP(hipMemcpyAsync(K.qpsi, dpsiq, sizeof(double) * lq.nq, hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k <= LEARN_MAXDEG; k++) K.coef[k] = k < np ? coef[k] : 0.f;
     for (int q = 0; q < lq.nq; q++) {
         K.qx[q] = lq.xq[q];
         K.qg[q] = lq.gq[q];

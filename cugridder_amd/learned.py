@@ -41,8 +41,8 @@ C_LIGHT = 299792458.0
 
 
 def default_degree(kw):
-    """Per-tap polynomial degree of the tabulated kernel."""
-    return min(15, kw + 6)
+    """Degree of the network's polynomial in s that the gridders evaluate."""
+    return 15
 
 
 def init_params(seed=0, out_scale=0.0):
@@ -111,13 +111,14 @@ def evaluate(params, s):
 
 
 def tabulate(kw, params, deg=None):
-    """(coef[kw, deg+1], psi at the quadrature nodes) as the gridders use them."""
+    """(coef[deg+1] of y as a polynomial in s, psi at the quadrature nodes)
+    as the gridders use them."""
     deg = default_degree(kw) if deg is None else deg
-    coef = np.zeros(kw * (deg + 1), np.float32)
+    coef = np.zeros(deg + 1, np.float32)
     qpsi = np.zeros(int(2 + 1.5 * kw))
     check(lib().cg_learn_tabulate(kw, deg, np.ascontiguousarray(params, np.float32), coef, qpsi),
           "cg_learn_tabulate")
-    return coef.reshape(kw, deg + 1), qpsi
+    return coef, qpsi
 
 
 def validation_problem(patch=256, nvis=1 << 16, fov_deg=1.0, freq=1.0e9, seed=1):

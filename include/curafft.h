@@ -261,10 +261,10 @@ int cg_learn_train(int kw, int mode, const double *a, const double *b, int n,
                    double *losses);
 /* Network output y at n inputs s (inference; without the ES exponent). */
 int cg_learn_eval(const float *params, const float *s, int n, float *y);
-/* Tabulated form the gridders use: coef[t (deg + 1) + k], the coefficient of
- * z^k of tap t, with psi(-kw/2 + t + (z + 1)/2) ~ sum_k coef z^k on [-1, 1]
- * (Chebyshev interpolation of degree deg <= 15); qpsi[p] = psi at the
- * p = floor(2 + 1.5 kw) quadrature nodes. */
+/* Tabulated form the gridders use: coef[k] (k <= deg <= 15), the
+ * coefficient of s^k in the Chebyshev interpolant of y(s) on [-1, 1]
+ * (the gridders evaluate exp(beta sqrt(1 - x^2) + sum_k coef[k] s^k));
+ * qpsi[p] = psi at the p = floor(2 + 1.5 kw) quadrature nodes. */
 int cg_learn_tabulate(int kw, int deg, const float *params, float *coef,
                       double *qpsi);
 /* Installs a learned kernel of support kw in a context, certified by the

@@ -86,15 +86,31 @@ def test_training_beats_es_at_same_support(L, trained, kw):
 
 @pytest.mark.parametrize("kw", [5, 6])
 def test_tabulation_matches_network(L, trained, kw):
+    """The gridders' form exp(beta sqrt(1 - x^2) + q(s)) of the trained
+    kernel against the network: q(s) close to y(s) where psi matters, and
+    the gridding response of the tabulated kernel within a tenth of the
+    network kernel's own error; the quadrature node values."""
     p64 = trained[kw][0].astype(np.float64)
     coef, qpsi = L.tabulate(kw, trained[kw][0])
-    f = np.linspace(0, 1, 999)[1:-1]  # the support edge itself is excluded
-    z = 2 * f - 1
-    psi0 = LN.psi(p64, np.array([0.0]), kw)[0]
-    for t in range(kw):
-        v = np.polynomial.polynomial.polyval(z, coef[t].astype(np.float64))
-        ref = LN.psi(p64, -0.5 * kw + t + f, kw)
-        assert np.max(np.abs(v - ref)) <= 3e-6 * psi0, t
+    assert coef.shape == (16,)
+    s = np.linspace(-1, 1, 4001)
+    y, _ = LN.forward(p64, s)
+    q = np.polynomial.polynomial.polyval(s, coef.astype(np.float64))
+    logpsi = LN.prior(s, kw) + y
+    wgt = np.exp(logpsi - logpsi.max())
+    assert np.max(np.abs(q - y) * wgt) <= 5e-6
+
+    def psi_tab(d):
+        sd = LN.mlp_input(d, kw)
+        v = np.exp(LN.prior(sd, kw) + np.polynomial.polynomial.polyval(sd, coef.astype(np.float64)))
+        return np.where(np.abs(d) < 0.5 * kw, v, 0.0)
+
+    rng = np.random.default_rng(7)
+    u, th = rng.uniform(0, 1, 20000), rng.uniform(-np.pi / 2, np.pi / 2, 20000)
+    F_net = LN.response(p64, u, th, kw)
+    F_tab = LN.response_fn(psi_tab, u, th, kw)
+    err_net = np.sqrt(np.mean(np.abs(F_net - 1) ** 2))
+    assert np.sqrt(np.mean(np.abs(F_tab - F_net) ** 2)) <= 0.1 * err_net
     xq, _ = LN.quadrature(kw)
     assert np.allclose(qpsi, LN.psi(p64, xq, kw), rtol=3e-6, atol=0)
 
