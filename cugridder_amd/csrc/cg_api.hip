@@ -369,7 +369,7 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
     if (cp.dense_min > 0) {
-        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 1)));
+        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
         CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
                                  as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
                                  (cx_t<float> *)C.grid.p, g, P.conv.kw, C.stream));

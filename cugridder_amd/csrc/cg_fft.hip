@@ -600,6 +600,93 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
     }
 }
 
+// The same step as a persistent kernel: a workgroup walks tiles (column
+// block x fastest, then sub-transform j, then plane) and loads the next
+// tile into registers while the current one is transformed in LDS and
+// stored, so the loads in flight are not limited to what fits in LDS.
+template <typename T, int DIR, int LOGL, int CB>
+__global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_step_pf(
+    cx_t<T> *data, int n2, int n1, int se, int sj, int nsub, int batch, int twmode,
+    const cx_t<T> *__restrict__ tw2, Live live, const uint8_t *__restrict__ rowmask, int mode_in,
+    int mode_out, int spec_a, int spec_b)
+{
+    constexpr int L = 1 << LOGL;
+    constexpr int EPT = fft_ept<T>();
+    constexpr int E = CB * L;
+    constexpr int NT = E / EPT;
+    constexpr int SH2 = 7;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
+    const TwTab<T> tab = load_twtab<T>(buf + E, tw2, n2, SH2);
+    lds_barrier();
+    const int nlive = live.count(n1);
+    const int nx = (nlive + CB - 1) / CB;
+    const int64_t ntile = (int64_t)nx * nsub * batch;
+    const int tstride = n2 >> LOGL;
+    auto dead = [&](int j, size_t row, int e) -> bool {
+        if (sj == 1) return rowmask && !rowmask[row];
+        const int k = j + nsub * e;
+        return k >= spec_a && k < spec_b;
+    };
+    cx_t<T> pre[EPT];
+    auto load = [&](int64_t t) {
+        const int x = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % nsub);
+        const cx_t<T> *pl = data + (size_t)(r / nsub) * n2 * n1;
+        const size_t row0 = (size_t)j * sj;
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * NT;
+            const int e = k / CB, b = k % CB;
+            const int c = x * CB + b;
+            const size_t row = row0 + (size_t)e * se;
+            const bool rd = c < nlive && !(mode_in && dead(j, row, e));
+            pre[u] = rd ? pl[row * n1 + live.at(c)] : mkc((T)0, (T)0);
+        }
+    };
+    int64_t t = blockIdx.x;
+    if (t < ntile) load(t);
+    for (; t < ntile; t += gridDim.x) {
+        const int x = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % nsub);
+        cx_t<T> *pl = data + (size_t)(r / nsub) * n2 * n1;
+        const size_t row0 = (size_t)j * sj;
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * NT;
+            cx_t<T> v = pre[u];
+            if (twmode == 2) {
+                cx_t<T> w = tab.get(j * (k / CB));
+                if (DIR < 0) w.y = -w.y;
+                v = cmul<T>(v, w);
+            }
+            buf[k] = v;
+        }
+        if (t + gridDim.x < ntile) load(t + gridDim.x);
+        lds_barrier();
+        ct_fft<T, DIR, L, 1, CB, true, false, NT, ct_rmax<T>()>(buf, tab, tstride);
+#pragma unroll
+        for (int u = 0; u < EPT; u++) {
+            const int k = threadIdx.x + u * NT;
+            const int e = k / CB, b = k % CB;
+            const int c = x * CB + b;
+            if (c >= nlive) continue;
+            const size_t row = row0 + (size_t)e * se;
+            if (mode_out && dead(j, row, e)) continue;
+            cx_t<T> v = buf[k];
+            if (twmode == 1) {
+                cx_t<T> w = tab.get(j * e);
+                if (DIR < 0) w.y = -w.y;
+                v = cmul<T>(v, w);
+            }
+            pl[row * n1 + live.at(c)] = v;
+        }
+        lds_barrier();
+    }
+}
+
 // split order <-> natural order of the column axis, out of place:
 // to_natural: dst[k] = src[pos(k)], else dst[pos(k)] = src[k]
 template <typename T>
@@ -739,14 +826,13 @@ struct DeadRows {
     int mode_in, mode_out;   // skip loads of / stores to dead rows
 };
 
-template <typename T, int DIR, int LOGL>
-static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
-                        int twmode, const cx_t<T> *tw2, Live live, const DeadRows &dr, hipStream_t s)
+template <typename T, int DIR, int LOGL, int CB>
+static int cols_step_ct_cb(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
+                           int twmode, const cx_t<T> *tw2, Live live, const DeadRows &dr,
+                           hipStream_t s)
 {
     constexpr int L = 1 << LOGL;
     constexpr int ELEM = (int)sizeof(cx_t<T>);
-    constexpr int CB0 = 512 / ELEM;
-    constexpr int CB = CB0 * L * ELEM <= 65536 ? CB0 : 65536 / (L * ELEM);
     constexpr int NT = CB * L / fft_ept<T>();
     if constexpr (NT < 64 || NT > 1024) {
         return 1;
@@ -756,6 +842,19 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
             CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_ct<T, DIR, LOGL, CB>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const int nlive = live.count(n1);
+        if (fft_knob("CG_FFT_COLPF", 1)) {
+            if (lds > 65536)
+                CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_pf<T, DIR, LOGL, CB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const int64_t ntile = (int64_t)((nlive + CB - 1) / CB) * nsub * batch;
+            const int per_cu = std::max(1, std::min((int)(kLdsMax / lds), 2048 / NT));
+            const int64_t grid = std::min<int64_t>(ntile, (int64_t)256 * per_cu);
+            hipLaunchKernelGGL((k_fft_cols_step_pf<T, DIR, LOGL, CB>), dim3((unsigned)grid), dim3(NT),
+                               lds, s, data, n2, n1, se, sj, nsub, batch, twmode, tw2, live,
+                               dr.rowmask, dr.mode_in, dr.mode_out, dr.spec_a, dr.spec_b);
+            CG_HIP(hipGetLastError());
+            return CG_OK;
+        }
         dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)nsub, (unsigned)batch);
         hipLaunchKernelGGL((k_fft_cols_step_ct<T, DIR, LOGL, CB>), grid, dim3(NT), lds, s, data, n2,
                            n1, se, sj, twmode, tw2, live, dr.rowmask, dr.mode_in, dr.mode_out,
@@ -763,6 +862,27 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
+}
+
+// CB adjacent columns per workgroup: 512-byte row segments within 64 KB of
+// LDS by default; CG_FFT_CB = 16 / 32 selects narrower blocks
+template <typename T, int DIR, int LOGL>
+static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
+                        int twmode, const cx_t<T> *tw2, Live live, const DeadRows &dr, hipStream_t s)
+{
+    constexpr int L = 1 << LOGL;
+    constexpr int ELEM = (int)sizeof(cx_t<T>);
+    constexpr int CB0 = 512 / ELEM;
+    constexpr int CB = CB0 * L * ELEM <= 65536 ? CB0 : 65536 / (L * ELEM);
+    const int want = fft_knob("CG_FFT_CB", 0);
+    if (want == 16 && CB > 16)
+        return cols_step_ct_cb<T, DIR, LOGL, 16>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
+                                                 live, dr, s);
+    if (want == 32 && CB > 32)
+        return cols_step_ct_cb<T, DIR, LOGL, 32>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
+                                                 live, dr, s);
+    return cols_step_ct_cb<T, DIR, LOGL, CB>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live,
+                                             dr, s);
 }
 
 template <typename T, int DIR>

@@ -22,6 +22,8 @@
 // each thread interpolates its points from LDS; the result is multiplied by
 // the per-point factor (weight and w-phase, ra_exec.cu:55-93) and scattered
 // back to the caller's row order.
+#include <cstdlib>
+
 #include "cg_internal.hpp"
 
 namespace cg {
@@ -267,23 +269,31 @@ __global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ 
                                                     int64_t ntiles, int dense_min,
                                                     int32_t *__restrict__ list)
 {
-    // list[0] counts the dense bins, list[1..] holds them (any order)
+    // list[0] counts the dense bins, list[1..] holds them (any order);
+    // list[ntiles + 1] is the work counter of k_grid_dense
     const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t >= ntiles) return;
     if (tile_start[t + 1] - tile_start[t] >= dense_min) list[1 + atomicAdd(list, 1)] = (int32_t)t;
 }
 
-__global__ void k_zero_i32(int32_t *p)
+__global__ void k_zero_i32(int32_t *p, int64_t second)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        p[0] = 0;
+        p[second] = 0;
+    }
 }
 
-template <int KW>
+// BATCH: the points of a start cell are loaded 64 at a time (one per lane)
+// and handed to the lanes of each point pair by shuffles, so the MFMA loop
+// waits on no global load; bins are taken from a work counter.
+template <int KW, bool BATCH>
 __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__restrict__ geo,
                                                               const float2 *__restrict__ val,
                                                               const uint32_t *__restrict__ keys,
                                                               const int32_t *__restrict__ tile_start,
-                                                              const int32_t *__restrict__ list,
+                                                              int32_t *__restrict__ list,
+                                                              int64_t ntiles,
                                                               float2 *__restrict__ fw, GridArgs g)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
     double *are = reinterpret_cast<double *>(smem_raw);
     double *aim = are + rcells;
     int *run = reinterpret_cast<int *>(aim + rcells);  // tx*ty + 1 run starts
-    __shared__ int s_next;
+    __shared__ int s_next, s_tile;
     const int ncell = g.tx * g.ty;
     const int nlist = list[0];
     const int lane = threadIdx.x & 63;
@@ -307,7 +317,14 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
     const int tap = i & 7;
     const int axis = i >> 3;  // 0 x, 1 y, 2 z, 3 none
 
-    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    for (int li0 = blockIdx.x;; li0 += gridDim.x) {
+        int li = li0;
+        if (BATCH) {
+            if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+            __syncthreads();
+            li = s_tile;
+        }
+        if (li >= nlist) break;
         const int tile = list[1 + li];
         const int bx = tile % g.ntx, by = tile / g.ntx;
         const int beg = tile_start[tile], end = tile_start[tile + 1];
@@ -333,16 +350,9 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
             const int a = run[c], b = run[c + 1];
             if (a >= b) continue;
             v16f r0 = {}, q0 = {}, r1 = {}, q1 = {};
-            for (int k0 = a; k0 < b; k0 += 2) {
-                const int k = k0 + half;
+            // one MFMA step for the point pair (G, v) of this half
+            auto pair = [&](bool ok, const float4 &G, const float2 &v) {
                 float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f, bz = 0.f;
-                const bool ok = k < b;
-                float4 G = make_float4(0.f, 0.f, 0.f, 0.f);
-                float2 v = make_float2(0.f, 0.f);
-                if (ok) {
-                    G = geo[k];
-                    v = val[k];
-                }
                 // this lane's kernel value, then the operands from the lanes
                 // holding them (every lane takes part in the shuffles)
                 const float off = axis == 0 ? G.x : axis == 1 ? G.y : G.z;
@@ -365,6 +375,41 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
                 q0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0i, bz, q0, 0, 0, 0);
                 r1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1r, bz, r1, 0, 0, 0);
                 q1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1i, bz, q1, 0, 0, 0);
+            };
+            if (BATCH) {
+                for (int kb = a; kb < b; kb += 64) {
+                    const int n = min(64, b - kb);
+                    float4 Gl = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float2 vl = make_float2(0.f, 0.f);
+                    if (lane < n) {
+                        Gl = geo[kb + lane];
+                        vl = val[kb + lane];
+                    }
+                    for (int j0 = 0; j0 < n; j0 += 2) {
+                        const int src = j0 + half;  // <= 63
+                        float4 G;
+                        float2 v;
+                        G.x = __shfl(Gl.x, src);
+                        G.y = __shfl(Gl.y, src);
+                        G.z = __shfl(Gl.z, src);
+                        G.w = __shfl(Gl.w, src);
+                        v.x = __shfl(vl.x, src);
+                        v.y = __shfl(vl.y, src);
+                        pair(src < n, G, v);
+                    }
+                }
+            } else {
+                for (int k0 = a; k0 < b; k0 += 2) {
+                    const int k = k0 + half;
+                    const bool ok = k < b;
+                    float4 G = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float2 v = make_float2(0.f, 0.f);
+                    if (ok) {
+                        G = geo[k];
+                        v = val[k];
+                    }
+                    pair(ok, G, v);
+                }
             }
             // D[tap][plane]: lane holds plane i, taps (v/4)*8 + half*4 + v%4 (+32)
             const int lx = c % g.tx, ly = c / g.tx;
@@ -423,18 +468,27 @@ static int grid_dense_kw(const float4 *geo, const float2 *val, const uint32_t *k
                          hipStream_t s)
 {
     const int64_t ntiles = (int64_t)g.ntx * g.nty;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
     hipLaunchKernelGGL(k_dense_bins, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, ts,
                        ntiles, g.dense_min, list);
     CG_HIP(hipGetLastError());
     const size_t lds = dense_lds_bytes(g, KW);
-    if (lds > 65536)
-        CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int per_cu = std::max<int>(1, (int)(163840 / lds));
     const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
-    hipLaunchKernelGGL((k_grid_dense<KW>), dim3(nblk), dim3(kDenseThreads), lds, s, geo, val, keys,
-                       ts, list, fw, g);
+    const char *e = std::getenv("CG_DENSE_BATCH");
+    if (!e || std::atoi(e) != 0) {
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW, true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_grid_dense<KW, true>), dim3(nblk), dim3(kDenseThreads), lds, s, geo,
+                           val, keys, ts, list, ntiles, fw, g);
+    } else {
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_grid_dense<KW, false>), dim3(nblk), dim3(kDenseThreads), lds, s, geo,
+                           val, keys, ts, list, ntiles, fw, g);
+    }
     CG_HIP(hipGetLastError());
     return CG_OK;
 }

@@ -111,7 +111,7 @@ int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
 int grid_lds_bytes(const GridArgs &g, int elem);
 // Dense bins (fp32, kw <= 8, nf3 <= 32, cshift > 0): the sparse kernel
 // launched with g.dense_min > 0 leaves them out; this adds them to fw with
-// per-start-cell MFMA products.  list / count need ntiles + 1 ints of scratch.
+// per-start-cell MFMA products.  list needs ntiles + 2 ints of scratch.
 bool grid_dense_supported(const GridArgs &g, int kw, int elem);
 int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uint32_t *keys,
                       const int32_t *tile_start, int32_t *list, cx_t<float> *fw, const GridArgs &g,
