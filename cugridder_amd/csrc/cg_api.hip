@@ -139,16 +139,29 @@ struct StageClock {
     }
 };
 
-// rows of the grid that points of non-empty tiles reach (null: all rows);
-// needs the tile starts of the last sort
-static int row_mask(Ctx &C, const Tiling &t, int kw, int nf2, const uint8_t *&mask)
+// rows (mask[0, nf2)) and columns (mask[nf2, nf2 + nf1)) of the grid that
+// points of non-empty tiles reach (null: all); needs the tile starts of the
+// last sort
+static int row_mask(Ctx &C, const Tiling &t, int kw, int nf1, int nf2, const uint8_t *&mask)
 {
     mask = nullptr;
     if (!C.row_mask) return CG_OK;
-    CG_TRY(ensure(C.rmask, (size_t)nf2));
-    CG_TRY(launch_row_mask(as<int32_t>(C.tstart), t, kw, nf2, as<uint8_t>(C.rmask), C.stream));
+    CG_TRY(ensure(C.rmask, (size_t)nf1 + nf2));
+    CG_TRY(launch_row_mask(as<int32_t>(C.tstart), t, kw, nf1, nf2, as<uint8_t>(C.rmask), C.stream));
     mask = as<uint8_t>(C.rmask);
     return CG_OK;
+}
+
+// lets the gridder leave the tiles the FFT will not read unwritten, when
+// every pass of this FFT honours the masks (test hook stop_after = 1 keeps
+// the whole grid written)
+template <typename T>
+static void grid_masks(const Ctx &C, GridArgs &g, const uint8_t *mask, int nf1, int nf2)
+{
+    if (mask && C.stop_after != 1 && fft_rows_masked<T>(nf1) && fft_spectrum_pruned<T>(nf2)) {
+        g.rowmask = mask;
+        g.colmask = mask + nf2;
+    }
 }
 
 // twiddle tables (cached per length and precision)
@@ -364,8 +377,11 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
                                                 C.stream)));
     CG_TRY(clk.mark(ST_GRID));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
+    const uint8_t *rm = nullptr;
+    CG_TRY(row_mask(C, t, P.conv.kw, P.nf1, P.nf2, rm));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
     g.dense_min = cp.dense_min;
+    grid_masks<T>(C, g, rm, P.nf1, P.nf2);
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
     if (cp.dense_min > 0) {
@@ -376,8 +392,6 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     }
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    const uint8_t *rm = nullptr;
-    CG_TRY(row_mask(C, t, P.conv.kw, P.nf2, rm));
     CG_TRY(clk.mark(ST_FFT));
     // only the columns holding image pixels are transformed along n2, only
     // the rows points reach hold non-zero data, and only the frequencies of
@@ -483,7 +497,7 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     // the degridder reads only the rows its tiles' points reach
     const uint8_t *rm = nullptr;
-    if (M > 0) CG_TRY(row_mask(C, t, kw, P.nf2, rm));
+    if (M > 0) CG_TRY(row_mask(C, t, kw, P.nf1, P.nf2, rm));
     // columns first (only those holding image pixels; frequencies outside
     // the image rows are zero), then the rows the degridder reads
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
@@ -784,8 +798,9 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     const int la = N1 - N1 / 2, lb = P.nf1 - N1 / 2;
     const int sa = N2 - N2 / 2, sb = P.nf2 - N2 / 2;
     const uint8_t *rm = nullptr;
-    CG_TRY(row_mask(C, t, kw, P.nf2, rm));
+    CG_TRY(row_mask(C, t, kw, P.nf1, P.nf2, rm));
     if (type == 1) {
+        grid_masks<T>(C, g, rm, P.nf1, P.nf2);
         CG_TRY(clk.mark(ST_GRID));
         CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                               as<cx_t<T>>(C.grid), g, kw, C.stream));

@@ -510,6 +510,10 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
     const TwTab<T> tab = load_twtab<T>(buf + E, tw, N, SH);
+    // rowmask[n2 + x] marks the columns points reach: the others are zero on
+    // input (dir > 0; they need not have been written) or unread on output
+    // (dir < 0; they are not stored)
+    const uint8_t *colmask = rowmask ? rowmask + n2 : nullptr;
     for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
         // a masked row is zero (dir > 0) or unread (dir < 0): skip it
         if (rowmask && !rowmask[g % n2]) continue;
@@ -517,7 +521,8 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
-            buf[k + (k >> 5)] = (DIR > 0 || live(k)) ? src[k] : mkc((T)0, (T)0);
+            const bool ld = DIR > 0 ? !(colmask && !colmask[k]) : live(k);
+            buf[k + (k >> 5)] = ld ? src[k] : mkc((T)0, (T)0);
         }
         lds_barrier();
         ct_fft<T, DIR, N, 1, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
@@ -525,7 +530,8 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
-            if (DIR < 0 || live(k)) dst[k] = buf[k + (k >> 5)];
+            const bool st = DIR < 0 ? !(colmask && !colmask[k]) : live(k);
+            if (st) dst[k] = buf[k + (k >> 5)];
         }
         lds_barrier();
     }
@@ -874,11 +880,12 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
     constexpr int ELEM = (int)sizeof(cx_t<T>);
     constexpr int CB0 = 512 / ELEM;
     constexpr int CB = CB0 * L * ELEM <= 65536 ? CB0 : 65536 / (L * ELEM);
+    // (a narrower block only where it still has at least one wavefront)
     const int want = fft_knob("CG_FFT_CB", 0);
-    if (want == 16 && CB > 16)
+    if (want == 16 && CB > 16 && 16 * L / fft_ept<T>() >= 64)
         return cols_step_ct_cb<T, DIR, LOGL, 16>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
                                                  live, dr, s);
-    if (want == 32 && CB > 32)
+    if (want == 32 && CB > 32 && 32 * L / fft_ept<T>() >= 64)
         return cols_step_ct_cb<T, DIR, LOGL, 32>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
                                                  live, dr, s);
     return cols_step_ct_cb<T, DIR, LOGL, CB>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live,
@@ -1050,6 +1057,19 @@ int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows
 }
 
 template <typename T>
+bool fft_rows_masked(int n1)
+{
+    // the conditions under which fft_rows takes rows_ct (which honours the
+    // row and column masks)
+    const int l = log2_exact(n1);
+    if (!fft_knob("CG_FFT_CT", 1) || l < 9 || l > 14) return false;
+    const int nt = n1 / fft_ept<T>();
+    if (nt < 64 || nt > 1024) return false;
+    const int sh = l / 2;
+    return (size_t)(n1 + (n1 >> 5) + (1 << sh) + (n1 >> sh)) * sizeof(cx_t<T>) <= (size_t)kLdsMax;
+}
+
+template <typename T>
 bool fft_spectrum_pruned(int n2)
 {
     int na, nb;
@@ -1112,6 +1132,8 @@ template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, c
 template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, double2 *, hipStream_t);
 template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, int, int, hipStream_t);
 template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, int, int, hipStream_t);
+template bool fft_rows_masked<float>(int);
+template bool fft_rows_masked<double>(int);
 template bool fft_spectrum_pruned<float>(int);
 template bool fft_spectrum_pruned<double>(int);
 template bool fft_supported<float>(int);

@@ -124,6 +124,14 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
     const int chunk = blockIdx.x % g.nchunk;
     const int tile = blockIdx.x / g.nchunk;
     const int bx = tile % g.ntx, by = tile / g.ntx;
+    // a tile no FFT pass reads (no masked-in row or no masked-in column) is
+    // left unwritten; the decision is uniform across the workgroup
+    if (g.rowmask) {
+        bool need_r = false, need_c = false;
+        for (int r = 0; r < g.ty && !need_r; r++) need_r = g.rowmask[by * g.ty + r] != 0;
+        for (int c = 0; c < g.tx && !need_c; c++) need_c = g.colmask[bx * g.tx + c] != 0;
+        if (!need_r || !need_c) return;
+    }
     const int p0 = chunk * g.pc;
     const int np = min(g.pc, g.nf3 - p0);
     const int cells = np * g.ty * g.tx;

@@ -46,10 +46,11 @@ int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a,
                    uint32_t *keys, int32_t *vals, hipStream_t s);
 // p[0..n) = v (a kernel, so that it can sit in a captured graph)
 int launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
-// mask[y] (nf2 bytes) = 1 for the grid rows that points of non-empty tiles
-// reach; the other rows are zero after gridding and unread by degridding
-int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
-                    hipStream_t s);
+// mask[y] (y < nf2) = 1 for the grid rows and mask[nf2 + x] (x < nf1) = 1
+// for the grid columns that points of non-empty tiles reach; other rows and
+// columns are zero after gridding and unread by degridding
+int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf1, int nf2,
+                    uint8_t *mask, hipStream_t s);
 // tile_start[t] = first sorted position with key >> shift >= t, t in [0, ntiles]
 int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
                        int32_t *tile_start, hipStream_t s);
@@ -95,6 +96,9 @@ struct GridArgs {
     int cshift;     // sort key = tile << cshift | start cell inside the tile
     int dense_min;  // > 0: bins with at least this many points go to the dense path
     double beta, es_c;
+    // tiles whose rows all have rowmask 0, or whose columns all have
+    // colmask 0, are not written (null: every tile is written)
+    const uint8_t *rowmask, *colmask;
     // learned kernel (learned != 0): the value at offset d is
     // exp(beta sqrt(1 - es_c d^2) + sum_k kq[k] s^k), s = 2 es_c d^2 - 1,
     // instead of the ES kernel's exp(beta sqrt(1 - es_c d^2))
@@ -131,9 +135,10 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
 // only those inputs.  A dir > 0 column pass leaves a split column axis in
 // split order (frequency ka + na kb at row ka nb + kb, see fft_col_split);
 // a dir < 0 column pass expects that order.
-// rowmask (nf2 bytes, may be null): rows with mask 0 are zero on input to a
-// dir > 0 pass and not needed on output of a dir < 0 pass, so they may be
-// skipped (the row pass and the natural-order column step use it).
+// rowmask (nf2 + nf1 bytes, may be null): rows with mask 0 are zero on
+// input to a dir > 0 pass and not needed on output of a dir < 0 pass, so
+// they may be skipped (the row pass and the natural-order column step use
+// it); the columns x with rowmask[nf2 + x] = 0 likewise for the row pass.
 // Frequencies k along n2 with spec_a <= k < spec_b are dead in the spectrum
 // (not needed after a dir > 0 column pass, zero before a dir < 0 one); the
 // split-order column step skips them when fft_spectrum_pruned(n2).
@@ -142,9 +147,14 @@ int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows
                     const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
                     int spec_a, int spec_b, hipStream_t s);
 // true when the column pass honours the dead spectrum rows (then a dir < 0
-// column pass does not read them and they need not be written)
+// column pass does not read them and they need not be written); it then
+// also honours the row mask (a dir > 0 pass does not read masked rows)
 template <typename T>
 bool fft_spectrum_pruned(int n2);
+// true when the row pass honours the row and column masks (a dir > 0 pass
+// reads neither masked rows nor masked columns)
+template <typename T>
+bool fft_rows_masked(int n1);
 // n = na * nb when columns of length n are transformed by the split
 bool fft_col_split(int n, int &na, int &nb);
 template <typename T>

@@ -294,18 +294,27 @@ __global__ __launch_bounds__(256) void k_fill_u8(uint8_t *p, int n)
     if (i < n) p[i] = 0;
 }
 
+// mask[y] (y < nf2) marks the grid rows and mask[nf2 + x] (x < nf1) the
+// grid columns that the footprints of the points of non-empty tiles reach
 __global__ __launch_bounds__(256) void k_row_mask(const int32_t *__restrict__ tile_start,
-                                                  int64_t ntiles, int ntx, int ty, int kw, int nf2,
-                                                  uint8_t *__restrict__ mask)
+                                                  int64_t ntiles, int ntx, int tx, int ty, int kw,
+                                                  int nf1, int nf2, uint8_t *__restrict__ mask)
 {
     const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t >= ntiles || tile_start[t + 1] == tile_start[t]) return;
     const int y0 = (int)(t / ntx) * ty;
-    const int span = min(ty + kw - 1, nf2);
-    for (int r = 0; r < span; r++) {
+    const int spany = min(ty + kw - 1, nf2);
+    for (int r = 0; r < spany; r++) {
         int y = y0 + r;
         if (y >= nf2) y -= nf2;
         mask[y] = 1;
+    }
+    const int x0 = (int)(t % ntx) * tx;
+    const int spanx = min(tx + kw - 1, nf1);
+    for (int r = 0; r < spanx; r++) {
+        int x = x0 + r;
+        if (x >= nf1) x -= nf1;
+        mask[nf2 + x] = 1;
     }
 }
 
@@ -323,13 +332,14 @@ int launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s)
     return CG_OK;
 }
 
-int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf2, uint8_t *mask,
-                    hipStream_t s)
+int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf1, int nf2,
+                    uint8_t *mask, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(nf2, 256)), dim3(256), 0, s, mask, nf2);
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)ceil_div(nf1 + nf2, 256)), dim3(256), 0, s, mask,
+                       nf1 + nf2);
     const int64_t ntiles = t.ntiles();
     hipLaunchKernelGGL(k_row_mask, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, tile_start,
-                       ntiles, t.ntx, t.ty, kw, nf2, mask);
+                       ntiles, t.ntx, t.tx, t.ty, kw, nf1, nf2, mask);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
