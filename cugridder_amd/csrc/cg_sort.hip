@@ -144,42 +144,61 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *key
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// One pass: the block ranks its 4096 items by digit in LDS (stable: round,
+// then wavefront, then lane order), then writes them out in that order, so
+// the items of one digit leave as one contiguous run at the digit's global
+// offset for this block (coalesced stores instead of one scattered store
+// per item).  offs is the scanned histogram (digit-major, block-minor): the
+// block's count of digit d is the difference of consecutive offsets.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, int64_t m, int shift,
     const int32_t *offs, int nblocks)
 {
     constexpr int NW = kSortThreads / 64;
-    __shared__ int s_base[kRadix];
+    __shared__ int s_gbase[kRadix];
+    __shared__ int s_lstart[kRadix];
+    __shared__ int s_lrun[kRadix];
     __shared__ int s_wcnt[NW][kRadix];
     __shared__ int s_rtot[kRadix];
+    __shared__ int s_wave[65];
+    __shared__ uint32_t s_key[kSortTile];
+    __shared__ int32_t s_val[kSortTile];
     const int tid = threadIdx.x, wave = tid >> 6;
-    s_base[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+    const int64_t hidx = (int64_t)tid * nblocks + blockIdx.x;
+    const int64_t hlen = (int64_t)kRadix * nblocks;
+    const int g0 = offs[hidx];
+    const int g1 = hidx + 1 < hlen ? offs[hidx + 1] : (int)m;
+    s_gbase[tid] = g0;
+    int tot;
+    const int ls = block_excl_scan(g1 - g0, s_wave, &tot);
+    s_lstart[tid] = ls;
+    s_lrun[tid] = ls;
     const uint64_t lt = __lanemask_lt();
-    int64_t base = blockIdx.x * (int64_t)kSortTile;
+    const int64_t base = blockIdx.x * (int64_t)kSortTile;
     for (int r = 0; r < kSortRounds; r++) {
-        int64_t i = base + r * kSortThreads + tid;
-        bool valid = i < m;
-        uint32_t key = valid ? kin[i] : 0u;
-        int32_t val = valid ? vin[i] : 0;
-        int d = (int)((key >> shift) & (kRadix - 1));
+        const int64_t i = base + r * kSortThreads + tid;
+        const bool valid = i < m;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const int32_t val = valid ? vin[i] : 0;
+        const int d = (int)((key >> shift) & (kRadix - 1));
 #pragma unroll
         for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
         __syncthreads();
         uint64_t mask = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; b++) {
-            int bit = (d >> b) & 1;
-            uint64_t bb = __ballot(bit);
+            const int bit = (d >> b) & 1;
+            const uint64_t bb = __ballot(bit);
             mask &= bit ? bb : ~bb;
         }
-        int rank = __popcll(mask & lt);
+        const int rank = __popcll(mask & lt);
         if (valid && rank == 0) s_wcnt[wave][d] = __popcll(mask);
         __syncthreads();
         {
             int acc = 0;
 #pragma unroll
             for (int w = 0; w < NW; w++) {
-                int t = s_wcnt[w][tid];
+                const int t = s_wcnt[w][tid];
                 s_wcnt[w][tid] = acc;
                 acc += t;
             }
@@ -187,15 +206,25 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
         }
         __syncthreads();
         if (valid) {
-            int pos = s_base[d] + s_wcnt[wave][d] + rank;
-            if (pos >= 0 && pos < m) {  // always true for a consistent histogram
-                kout[pos] = key;
-                vout[pos] = val;
+            const int lp = s_lrun[d] + s_wcnt[wave][d] + rank;
+            if (lp >= 0 && lp < kSortTile) {  // always true for a consistent histogram
+                s_key[lp] = key;
+                s_val[lp] = val;
             }
         }
         __syncthreads();
-        s_base[tid] += s_rtot[tid];
+        s_lrun[tid] += s_rtot[tid];
         __syncthreads();
+    }
+    const int nvalid = (int)std::min<int64_t>(kSortTile, m - base);
+    for (int p = tid; p < nvalid; p += kSortThreads) {
+        const uint32_t key = s_key[p];
+        const int d = (int)((key >> shift) & (kRadix - 1));
+        const int64_t pos = (int64_t)s_gbase[d] + (p - s_lstart[d]);
+        if (pos >= 0 && pos < m) {  // always true for a consistent histogram
+            kout[pos] = key;
+            vout[pos] = s_val[p];
+        }
     }
 }
 
