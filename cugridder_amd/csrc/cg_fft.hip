@@ -693,6 +693,133 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
     }
 }
 
+// Two-pass form of the persistent step for fp32 and L = 16 R2 (R2 = 2..16):
+// thread t holds column b = t % CB at rows e0 + (L/16) u, u < 16
+// (e0 = t / CB), which are exactly the inputs of the first radix-16
+// Stockham butterfly (P = 1, no twiddles).  That butterfly runs in
+// registers on the loaded tile and goes to LDS once; the second (last)
+// pass, radix R2 with P = 16, reads it back and its outputs go straight
+// from registers to HBM: one LDS write and one read per element.
+template <int DIR, int LOGL, int CB>
+__global__ __launch_bounds__(CB * (1 << LOGL) / 16) void k_fft_cols_step_r2(
+    float2 *data, int n2, int n1, int se, int sj, int nsub, int batch, int twmode,
+    const float2 *__restrict__ tw2, Live live, const uint8_t *__restrict__ rowmask, int mode_in,
+    int mode_out, int spec_a, int spec_b)
+{
+    using T = float;
+    constexpr int L = 1 << LOGL;
+    constexpr int R2 = L / 16;
+    constexpr int E = CB * L;
+    constexpr int NT = E / 16;
+    constexpr int K2 = 16 * CB / NT;  // last-pass butterflies per thread
+    constexpr int SH2 = 7;
+    static_assert(R2 >= 2 && R2 <= 16 && NT % CB == 0 && (16 * CB) % NT == 0, "shape");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    const TwTab<T> tab = load_twtab<T>(buf + E, tw2, n2, SH2);
+    lds_barrier();
+    const int nlive = live.count(n1);
+    const int nx = (nlive + CB - 1) / CB;
+    const int64_t ntile = (int64_t)nx * nsub * batch;
+    const int ts = n2 >> LOGL;
+    const int b0 = threadIdx.x % CB, e0 = threadIdx.x / CB;
+    auto dead = [&](int j, size_t row, int e) -> bool {
+        if (sj == 1) return rowmask && !rowmask[row];
+        const int k = j + nsub * e;
+        return k >= spec_a && k < spec_b;
+    };
+    float2 pre[16];
+    auto load = [&](int64_t t) {
+        const int x = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % nsub);
+        const float2 *pl = data + (size_t)(r / nsub) * n2 * n1;
+        const size_t row0 = (size_t)j * sj;
+        const int c = x * CB + b0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = e0 + u * R2;
+            const size_t row = row0 + (size_t)e * se;
+            const bool rd = c < nlive && !(mode_in && dead(j, row, e));
+            pre[u] = rd ? pl[row * n1 + live.at(c)] : make_float2(0.f, 0.f);
+        }
+    };
+    int64_t t = blockIdx.x;
+    if (t < ntile) load(t);
+    for (; t < ntile; t += gridDim.x) {
+        const int x = (int)(t % nx);
+        const int64_t r = t / nx;
+        const int j = (int)(r % nsub);
+        float2 *pl = data + (size_t)(r / nsub) * n2 * n1;
+        const size_t row0 = (size_t)j * sj;
+        float2 u1[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            float2 v = pre[u];
+            if (twmode == 2) {
+                float2 w = tab.get(j * (e0 + u * R2));
+                if (DIR < 0) w.y = -w.y;
+                v = cmul<T>(v, w);
+            }
+            u1[u] = v;
+        }
+        if (t + gridDim.x < ntile) load(t + gridDim.x);
+        // first pass: butterfly i = e0, outputs at 16 e0 + r
+        dft_small<T, 16, DIR>(u1);
+#pragma unroll
+        for (int rr = 0; rr < 16; rr++) buf[(16 * e0 + rr) * CB + b0] = u1[rr];
+        lds_barrier();
+        // last pass: butterfly i < 16 of column b, inputs i + 16 r, twiddle
+        // w_L^(i r), outputs at rows i + 16 r
+#pragma unroll
+        for (int k = 0; k < K2; k++) {
+            const int q = threadIdx.x + k * NT;
+            const int b = q % CB, i = q / CB;
+            float2 v[R2];
+#pragma unroll
+            for (int rr = 0; rr < R2; rr++) v[rr] = buf[(i + 16 * rr) * CB + b];
+            float2 w1 = tab.get(i * ts);
+            if (DIR < 0) w1.y = -w1.y;
+            const float2 w2 = cmul<T>(w1, w1);
+            const float2 pw[4] = {make_float2(1.f, 0.f), w1, w2, cmul<T>(w2, w1)};
+            if constexpr (R2 <= 4) {
+#pragma unroll
+                for (int rr = 1; rr < R2; rr++) v[rr] = cmul<T>(v[rr], pw[rr]);
+            } else {
+                float2 w4 = tab.get(4 * i * ts);
+                if (DIR < 0) w4.y = -w4.y;
+                const float2 w8 = cmul<T>(w4, w4);
+                const float2 qw[4] = {make_float2(1.f, 0.f), w4, w8, cmul<T>(w8, w4)};
+#pragma unroll
+                for (int rr = 1; rr < R2; rr++) {
+                    float2 z = v[rr];
+                    if (rr & 3) z = cmul<T>(z, pw[rr & 3]);
+                    if (rr >> 2) z = cmul<T>(z, qw[rr >> 2]);
+                    v[rr] = z;
+                }
+            }
+            dft_small<T, R2, DIR>(v);
+            const int c = x * CB + b;
+            if (c < nlive) {
+#pragma unroll
+                for (int rr = 0; rr < R2; rr++) {
+                    const int e = i + 16 * rr;
+                    const size_t row = row0 + (size_t)e * se;
+                    if (mode_out && dead(j, row, e)) continue;
+                    float2 o = v[rr];
+                    if (twmode == 1) {
+                        float2 w = tab.get(j * e);
+                        if (DIR < 0) w.y = -w.y;
+                        o = cmul<T>(o, w);
+                    }
+                    pl[row * n1 + live.at(c)] = o;
+                }
+            }
+        }
+        lds_barrier();
+    }
+}
+
 // split order <-> natural order of the column axis, out of place:
 // to_natural: dst[k] = src[pos(k)], else dst[pos(k)] = src[k]
 template <typename T>
@@ -848,6 +975,24 @@ static int cols_step_ct_cb(cx_t<T> *data, int batch, int n2, int n1, int se, int
             CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_ct<T, DIR, LOGL, CB>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const int nlive = live.count(n1);
+        if constexpr (sizeof(T) == 4 && LOGL >= 5 && LOGL <= 8) {
+            if (fft_knob("CG_FFT_COL2", 1) && CB * L / 16 == NT) {
+                const size_t lds2 = (size_t)(CB * L + 128 + (n2 + 127) / 128) * ELEM;
+                if (lds2 > 65536)
+                    CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_r2<DIR, LOGL, CB>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds2));
+                const int64_t ntile = (int64_t)((nlive + CB - 1) / CB) * nsub * batch;
+                const int per_cu = std::max(1, std::min((int)(kLdsMax / lds2), 2048 / NT));
+                const int64_t grid = std::min<int64_t>(ntile, (int64_t)256 * per_cu);
+                hipLaunchKernelGGL((k_fft_cols_step_r2<DIR, LOGL, CB>), dim3((unsigned)grid),
+                                   dim3(NT), lds2, s, (float2 *)data, n2, n1, se, sj, nsub, batch,
+                                   twmode, (const float2 *)tw2, live, dr.rowmask, dr.mode_in,
+                                   dr.mode_out, dr.spec_a, dr.spec_b);
+                CG_HIP(hipGetLastError());
+                return CG_OK;
+            }
+        }
         if (fft_knob("CG_FFT_COLPF", 1)) {
             if (lds > 65536)
                 CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_pf<T, DIR, LOGL, CB>,
