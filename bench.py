@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 C_LIGHT = 299792458.0
 HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "Mvis/s ms2dirty+dirty2ms at eps 1e-5, plain vs batched/learned; HBM GB/s"
 
 
@@ -189,35 +190,62 @@ def setup_learned(torch, ctx, W, args, kw=6):
             "es_judge_Mvis_s": round(W["M"] / (es_ms * 1e-3) / 1e6, 3)}
 
 
-def stage_kernel(stage, direction, W, plan):
-    """(kernel name, launches, algorithmic HBM bytes per launch) of a pipeline
-    stage, complex64 grid.  Records are 24 B per row (FP32 offsets + packed
-    cell, complex value); C = nf1*nf2*nf3 cells of 8 B; f = N1/nf1 is the
-    fraction of fast-axis columns that hold image pixels (only those are
-    column-transformed, stored by the forward row FFT and read by the
-    inverse one)."""
+def grid_stats(ctx, plan, kw, dense_min):
+    """Per-call geometry the algorithmic byte counts need, from the last
+    call's tile starts: grid rows / columns that points reach (the FFT's
+    masks), gridding tiles that are written, points in dense bins."""
+    ts = ctx.tile_starts().cpu().numpy().astype(np.int64)
+    nf1, nf2 = plan["nf1"], plan["nf2"]
+    tx, ty = plan["tile_x"], plan["tile_y"]
+    ntx, nty = nf1 // tx, nf2 // ty
+    cnt = np.diff(ts)[: ntx * nty].reshape(nty, ntx)
+    nz = cnt > 0
+    rows = np.zeros(nf2, bool)
+    cols = np.zeros(nf1, bool)
+    for by in np.nonzero(nz.any(1))[0]:
+        rows[(by * ty + np.arange(ty + kw - 1)) % nf2] = True
+    for bx in np.nonzero(nz.any(0))[0]:
+        cols[(bx * tx + np.arange(tx + kw - 1)) % nf1] = True
+    row_t = rows.reshape(nty, ty).any(1)
+    col_t = cols.reshape(ntx, tx).any(1)
+    dense = int(cnt[cnt >= dense_min].sum()) if (dense_min > 0 and ctx.key_shift > 0) else 0
+    return {"rows": int(rows.sum()), "cols": int(cols.sum()),
+            "tiles_written": int(row_t.sum() * col_t.sum()), "tile": tx * ty,
+            "nonempty_tiles": int(nz.sum()), "dense_points": dense}
+
+
+def kernel_table(W, plan, fw, bw, stage_ms):
+    """(kernel, direction, launches per call, ms per call, bound, algorithmic
+    bytes or flops per launch) of every timed kernel of one ms2dirty and one
+    dirty2ms call, complex64 grid (8 B per cell), records 24 B per row.
+    Rm = grid rows points reach, N1 live columns of nf1, N2 live frequencies
+    of nf2, P planes."""
     M = W["M"]
-    nf1, nf2, nf3 = plan["nf1"], plan["nf2"], plan["nf3"]
-    C = nf1 * nf2 * nf3
-    n1 = W["ny"]
-    f = n1 / nf1
-    npix = W["nx"] * W["ny"]
-    p2 = lambda n: n & (n - 1) == 0
-    if stage == "grid" and direction == "ms2dirty":
-        return "k_grid + k_grid_dense", 1, 24 * M + 8 * C       # read records, write grid once
-    if stage == "grid" and direction == "dirty2ms":
-        return "k_degrid", 1, 8 * C + 24 * M + 4 * M + 8 * M    # grid + records + perm, write vis
-    if stage == "fft_rows":
-        return ("k_fft_rows_ct" if p2(nf1) else "k_fft_rows"), 1, int(8 * C * (1 + f))
-    if stage == "fft_cols":
-        if nf2 > 128:                                            # four-step: two launches
-            return "k_fft_cols_step_ct", 2, int(16 * C * f)
-        return "k_fft_cols", 1, int(16 * C * f)
-    if stage == "image" and direction == "ms2dirty":
-        return "k_ms2dirty_epi", 1, 8 * nf3 * npix + 8 * npix   # P values per pixel, write image
-    if stage == "image" and direction == "dirty2ms":
-        return "k_dirty2ms_pro", 1, 8 * npix + int(8 * C * f)   # read image, write live columns
-    return None, 1, None
+    nf1, nf2, P = plan["nf1"], plan["nf2"], plan["nf3"]
+    N1, N2 = W["ny"], W["nx"]
+    kw = plan["kw"]
+    Md = fw["dense_points"]
+    Rm, Cm = fw["rows"], fw["cols"]
+    sm, sb = stage_ms["ms2dirty"], stage_ms["dirty2ms"]
+    col_kernel = "k_fft_cols_step_r2" if (nf2 > 128 and nf2 & (nf2 - 1) == 0) else "k_fft_cols_step"
+    rows = []
+    rows.append(("k_grid", "ms2dirty", 1, sm["grid"] - sm["grid_dense"], "hbm",
+                 24 * (M - Md) + 8 * P * fw["tiles_written"] * fw["tile"]))
+    if Md:
+        # 4 MFMA 32x32x2 (4096 flop each) per point pair
+        rows.append(("k_grid_dense", "ms2dirty", 1, sm["grid_dense"], "mfma", 8192 * Md))
+    rows.append(("k_fft_rows_ct", "ms2dirty", 1, sm["fft_rows"], "hbm", 8 * P * Rm * (Cm + N1)))
+    rows.append((col_kernel, "ms2dirty", 2, sm["fft_cols"], "hbm",
+                 (8 * P * (Rm * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + N2 * N1)) // 2))
+    rows.append(("k_ms2dirty_epi", "ms2dirty", 1, sm["image"], "hbm", 8 * P * N1 * N2 + 8 * N1 * N2))
+    Rb, Cb = bw["rows"], bw["cols"]
+    rows.append(("k_dirty2ms_pro", "dirty2ms", 1, sb["image"], "hbm", 8 * N1 * N2 + 8 * P * N2 * N1))
+    rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"], "hbm",
+                 (8 * P * (N2 * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + Rb * N1)) // 2))
+    rows.append(("k_fft_rows_ct", "dirty2ms", 1, sb["fft_rows"], "hbm", 8 * P * Rb * (N1 + Cb)))
+    halo = bw["nonempty_tiles"] * (16 + kw - 1) ** 2 * P * 8
+    rows.append(("k_degrid", "dirty2ms", 1, sb["grid"], "hbm", halo + 24 * M + 4 * M + 8 * M))
+    return rows
 
 
 def run_cpu_baseline(W, workload):
@@ -384,27 +412,37 @@ def main():
                                               for d, s in stage_ms.items()}))
 
     roof = None
-    if args.workload == "batch":
-        pass
-    elif args.workload != "nufft2d":
-        # the stage with the most time; per-launch figures for its kernel
-        best = None
-        for d in ("ms2dirty", "dirty2ms"):
-            for k in ("grid", "fft_rows", "fft_cols", "image"):
-                if best is None or stage_ms[d][k] > best[2]:
-                    best = (d, k, stage_ms[d][k])
-        d, k, t_ms = best
-        kname, launches, nbytes = stage_kernel(k, d, W, plan)
+    kernels = None
+    if args.workload not in ("batch", "nufft2d"):
+        # per-call geometry of each direction (untimed calls after the steps)
+        ctx.ms2dirty(W["uvw"], W["vis"], W["wgt"], W["nx"], W["ny"], W["fov"], W["freq"], W["eps"],
+                     out=out_d)
+        fw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], 64)
+        ctx.dirty2ms(W["uvw"], W["img"], W["wgt"], W["fov"], W["freq"], W["eps"], out=out_v)
+        bw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], 0)
+        torch.cuda.synchronize()
+        kernels = kernel_table(W, plan, fw_stats, bw_stats, stage_ms)
+        # the dominant kernel: most time per step (a kernel name counts the
+        # launches of both directions only when they are the same template)
+        name, d, launches, t_ms, bound, work = max(kernels, key=lambda r: r[3])
         t_launch = t_ms / launches
-        achieved = nbytes / (t_launch * 1e-3) / 1e9
-        traffic = None
-        if " + " not in kname:
-            traffic = traffic_from_profiles(args.workload, kname)
-        roof = {"kernel": "%s (%s)" % (kname, d), "bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes": nbytes, "avg_ms": round(t_launch, 4),
-                "launches_per_call": launches}
-    else:
+        if bound == "mfma":
+            achieved = work / (t_launch * 1e-3) / 1e12
+            roof = {"kernel": "%s (%s)" % (name, d), "bound": "mfma", "achieved": round(achieved, 2),
+                    "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+                    "algorithmic_flops": work, "avg_ms": round(t_launch, 4),
+                    "launches_per_call": launches}
+        else:
+            achieved = work / (t_launch * 1e-3) / 1e9
+            roof = {"kernel": "%s (%s)" % (name, d), "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profiles(args.workload, "%s:%s" % (name, d)),
+                    "algorithmic_bytes": work, "avg_ms": round(t_launch, 4),
+                    "launches_per_call": launches}
+        log("kernels (name, direction, launches, ms per call, bound, work per launch): %s"
+            % json.dumps(kernels))
+    elif args.workload == "nufft2d":
         t_grid = stage_ms["ms2dirty"]["grid"]
         nbytes = 16 * W["M"] + 8 * (2 * W["N"]) ** 2
         achieved = nbytes / (t_grid * 1e-3) / 1e9

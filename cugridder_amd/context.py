@@ -141,13 +141,15 @@ class Context:
 
     # ------------------------------------------------------------ introspection
     def stage_times(self):
-        """{'ms2dirty': {stage: ms}, 'dirty2ms': {...}, 'calls': (n1, n2)} summed."""
+        """{'ms2dirty': {stage: ms}, 'dirty2ms': {...}, 'calls': (n1, n2)} summed;
+        ms2dirty's 'grid_dense' is the MFMA kernel's part of its 'grid'."""
         n = len(STAGES)
-        out = np.zeros(2 * n)
+        out = np.zeros(2 * n + 1)
         calls = (ctypes.c_longlong * 2)()
         check(lib().cg_ctx_stage_times(self._h, out, calls), "stage_times")
-        return {"ms2dirty": dict(zip(STAGES, out[:n].tolist())),
-                "dirty2ms": dict(zip(STAGES, out[n:].tolist())),
+        fw = dict(zip(STAGES, out[:n].tolist()))
+        fw["grid_dense"] = float(out[2 * n])
+        return {"ms2dirty": fw, "dirty2ms": dict(zip(STAGES, out[n:2 * n].tolist())),
                 "calls": (calls[0], calls[1])}
 
     def reset_times(self):
@@ -188,6 +190,18 @@ class Context:
         buf = torch.empty(n, dtype=dtype, device="cuda")
         _memcpy_d2d(buf.data_ptr(), p.value, n * itemsize)
         return buf.view(plan["nf3"], plan["nf2"], plan["nf1"])
+
+    def tile_starts(self):
+        """tile_start (int32, ntiles + 1) of the last call's sort."""
+        torch = self._torch
+        k, pm, ts = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        n = (ctypes.c_longlong * 3)()
+        check(lib().cg_ctx_sorted(self._h, ctypes.byref(k), ctypes.byref(pm), ctypes.byref(ts), n),
+              "ctx_sorted")
+        tst = torch.empty(n[1] + 1, dtype=torch.int32, device="cuda")
+        _memcpy_d2d(tst.data_ptr(), ts.value, 4 * (n[1] + 1))
+        self.key_shift = int(n[2])
+        return tst
 
     def sorted_view(self):
         """(keys uint32-as-int64, perm int32, tile_start int32) of the last call."""

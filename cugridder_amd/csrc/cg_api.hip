@@ -52,9 +52,12 @@ struct Ctx {
     int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
     // stage timing (HIP events on the context stream)
     bool timing = false;
-    hipEvent_t ev[ST_COUNT + 1] = {};
-    bool ev_ok = false;
+    // stage marks ev[0..ST_COUNT]; ev[ST_COUNT + 1, + 2] bracket the dense
+    // gridding kernel inside the ms2dirty grid stage
+    hipEvent_t ev[ST_COUNT + 3] = {};
+    bool ev_ok = false, dense_timed = false;
     double stage_ms[2][ST_COUNT] = {};
+    double dense_ms = 0;
     int64_t stage_calls[2] = {0, 0};
     WPlan last_plan;
     Tiling last_tiling;
@@ -133,6 +136,12 @@ struct StageClock {
             float ms = 0.f;
             CG_HIP(hipEventElapsedTime(&ms, C.ev[i], C.ev[i + 1]));
             C.stage_ms[dir][i] += ms;
+        }
+        if (C.dense_timed) {
+            float ms = 0.f;
+            CG_HIP(hipEventElapsedTime(&ms, C.ev[ST_COUNT + 1], C.ev[ST_COUNT + 2]));
+            C.dense_ms += ms;
+            C.dense_timed = false;
         }
         C.stage_calls[dir]++;
         return CG_OK;
@@ -386,9 +395,12 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
     if (cp.dense_min > 0) {
         CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
+        CG_TRY(clk.mark(ST_COUNT + 1));
         CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
                                  as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
                                  (cx_t<float> *)C.grid.p, g, P.conv.kw, C.stream));
+        CG_TRY(clk.mark(ST_COUNT + 2));
+        C.dense_timed = C.timing;
     }
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
@@ -1131,13 +1143,15 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     return CG_OK;
 }
 
-// stage times summed over calls: out[dir*7 + stage] (ms); returns calls
+// stage times summed over calls: out[dir*7 + stage] (ms), out[14] the dense
+// gridding kernel's part of the ms2dirty grid stage; returns calls
 int cg_ctx_stage_times(void *ctx, double *out, long long *calls)
 {
     Ctx *C = (Ctx *)ctx;
     if (!C || !out) return CG_ERR_ARG;
     for (int d = 0; d < 2; d++)
         for (int i = 0; i < ST_COUNT; i++) out[d * ST_COUNT + i] = C->stage_ms[d][i];
+    out[2 * ST_COUNT] = C->dense_ms;
     if (calls) {
         calls[0] = C->stage_calls[0];
         calls[1] = C->stage_calls[1];
@@ -1150,6 +1164,7 @@ int cg_ctx_reset_times(void *ctx)
     Ctx *C = (Ctx *)ctx;
     if (!C) return CG_ERR_ARG;
     std::memset(C->stage_ms, 0, sizeof(C->stage_ms));
+    C->dense_ms = 0;
     C->stage_calls[0] = C->stage_calls[1] = 0;
     return CG_OK;
 }

@@ -131,9 +131,10 @@ int cg_ctx_destroy(void *ctx);
  * skips grid rows that no point reaches; 0: transforms every row),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
-/* per-stage times summed over calls, out[14] = {ms2dirty[7], dirty2ms[7]}
- * in ms, stages: plan, sort, records, grid/degrid, fft rows, fft columns,
- * image (epilogue/prologue); calls[2] */
+/* per-stage times summed over calls, out[15] = {ms2dirty[7], dirty2ms[7],
+ * dense} in ms, stages: plan, sort, records, grid/degrid, fft rows, fft
+ * columns, image (epilogue/prologue); dense = the MFMA gridding kernel's
+ * part of the ms2dirty grid stage; calls[2] */
 int cg_ctx_stage_times(void *ctx, double *out, long long *calls);
 int cg_ctx_reset_times(void *ctx);
 /* parameters of the last call: nf1 nf2 nf3 kw beta es_c pix_x pix_y
