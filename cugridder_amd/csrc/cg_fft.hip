@@ -701,7 +701,7 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_st
 // pass, radix R2 with P = 16, reads it back and its outputs go straight
 // from registers to HBM: one LDS write and one read per element.
 template <int DIR, int LOGL, int CB>
-__global__ __launch_bounds__(CB * (1 << LOGL) / 16) void k_fft_cols_step_r2(
+__global__ __launch_bounds__(CB * (1 << LOGL) / 16, CB % 64 == 0 ? 4 : 2) void k_fft_cols_step_r2(
     float2 *data, int n2, int n1, int se, int sj, int nsub, int batch, int twmode,
     const float2 *__restrict__ tw2, Live live, const uint8_t *__restrict__ rowmask, int mode_in,
     int mode_out, int spec_a, int spec_b)
@@ -728,6 +728,9 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16) void k_fft_cols_step_r2(
         const int k = j + nsub * e;
         return k >= spec_a && k < spec_b;
     };
+    // with 64 columns per tile a wavefront covers one row per load, so the
+    // row (and its mask byte) is wave-uniform and stays in scalar registers
+    const int e0w = CB % 64 == 0 ? __builtin_amdgcn_readfirstlane(e0) : e0;
     float2 pre[16];
     auto load = [&](int64_t t) {
         const int x = (int)(t % nx);
@@ -736,12 +739,14 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16) void k_fft_cols_step_r2(
         const float2 *pl = data + (size_t)(r / nsub) * n2 * n1;
         const size_t row0 = (size_t)j * sj;
         const int c = x * CB + b0;
+        const bool cok = c < nlive;
+        const int lc = live.at(c);
 #pragma unroll
         for (int u = 0; u < 16; u++) {
-            const int e = e0 + u * R2;
+            const int e = e0w + u * R2;
             const size_t row = row0 + (size_t)e * se;
-            const bool rd = c < nlive && !(mode_in && dead(j, row, e));
-            pre[u] = rd ? pl[row * n1 + live.at(c)] : make_float2(0.f, 0.f);
+            const bool rd = cok && !(mode_in && dead(j, row, e));
+            pre[u] = rd ? pl[row * n1 + lc] : make_float2(0.f, 0.f);
         }
     };
     int64_t t = blockIdx.x;
@@ -774,7 +779,8 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16) void k_fft_cols_step_r2(
 #pragma unroll
         for (int k = 0; k < K2; k++) {
             const int q = threadIdx.x + k * NT;
-            const int b = q % CB, i = q / CB;
+            const int b = q % CB;
+            const int i = CB % 64 == 0 ? __builtin_amdgcn_readfirstlane(q / CB) : q / CB;
             float2 v[R2];
 #pragma unroll
             for (int rr = 0; rr < R2; rr++) v[rr] = buf[(i + 16 * rr) * CB + b];
