@@ -48,7 +48,7 @@ struct Ctx {
     int last_cshift = 0;
     // workspace (grow-only)
     DevBuf keys, keys2, vals, vals2, hist, scan, tstart, geo, val, grid, zq, phi3, phi1, phi2,
-        part, tw1, tw2, fftmp, dlist, rmask, chans;
+        part, tw1, tw2, fftmp, dlist, rmask, chans, tlist;
     int tw1_n = 0, tw2_n = 0, tw_bytes = 0;
     // stage timing (HIP events on the context stream)
     bool timing = false;
@@ -73,7 +73,7 @@ struct Ctx {
     {
         DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
                           &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist,
-                          &rmask, &chans};
+                          &rmask, &chans, &tlist};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
         if (ev_ok)
@@ -165,12 +165,15 @@ static int row_mask(Ctx &C, const Tiling &t, int kw, int nf1, int nf2, const uin
 // every pass of this FFT honours the masks (test hook stop_after = 1 keeps
 // the whole grid written)
 template <typename T>
-static void grid_masks(const Ctx &C, GridArgs &g, const uint8_t *mask, int nf1, int nf2)
+static int grid_masks(Ctx &C, GridArgs &g, const uint8_t *mask, int nf1, int nf2)
 {
     if (mask && C.stop_after != 1 && fft_rows_masked<T>(nf1) && fft_spectrum_pruned<T>(nf2)) {
+        CG_TRY(ensure(C.tlist, sizeof(int32_t) * ((size_t)g.ntx * g.nty + 1)));
         g.rowmask = mask;
         g.colmask = mask + nf2;
+        g.tile_list = as<int32_t>(C.tlist);
     }
+    return CG_OK;
 }
 
 // twiddle tables (cached per length and precision)
@@ -390,7 +393,7 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     CG_TRY(row_mask(C, t, P.conv.kw, P.nf1, P.nf2, rm));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
     g.dense_min = cp.dense_min;
-    grid_masks<T>(C, g, rm, P.nf1, P.nf2);
+    CG_TRY(grid_masks<T>(C, g, rm, P.nf1, P.nf2));
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                           as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
     if (cp.dense_min > 0) {
@@ -812,7 +815,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     const uint8_t *rm = nullptr;
     CG_TRY(row_mask(C, t, kw, P.nf1, P.nf2, rm));
     if (type == 1) {
-        grid_masks<T>(C, g, rm, P.nf1, P.nf2);
+        CG_TRY(grid_masks<T>(C, g, rm, P.nf1, P.nf2));
         CG_TRY(clk.mark(ST_GRID));
         CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                               as<cx_t<T>>(C.grid), g, kw, C.stream));

@@ -111,6 +111,8 @@ __device__ __forceinline__ Span tile_span(int r, int t, int n)
 // most source bins feeding one tile: (1 + ceil(15/2))^2 for tiles >= 2
 constexpr int kMaxBins = 81;
 
+__global__ void k_zero_i32(int32_t *p, int64_t second);
+
 template <typename T, int KW>
 __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
                                                const cx_t<T> *__restrict__ val,
@@ -121,17 +123,13 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
     double *acc = reinterpret_cast<double *>(smem_raw);
     __shared__ int s_beg[kMaxBins], s_cnt[kMaxBins], s_ox[kMaxBins], s_oy[kMaxBins];
     __shared__ int s_total;
-    const int chunk = blockIdx.x % g.nchunk;
-    const int tile = blockIdx.x / g.nchunk;
+    // work item = (tile, plane chunk): one per workgroup, or, with a tile
+    // list (the tiles some FFT pass reads), the workgroups walk the list
+    const int nwork = g.tile_list ? g.tile_list[0] * g.nchunk : (int)gridDim.x;
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const int chunk = wi % g.nchunk;
+    const int tile = g.tile_list ? g.tile_list[1 + wi / g.nchunk] : wi / g.nchunk;
     const int bx = tile % g.ntx, by = tile / g.ntx;
-    // a tile no FFT pass reads (no masked-in row or no masked-in column) is
-    // left unwritten; the decision is uniform across the workgroup
-    if (g.rowmask) {
-        bool need_r = false, need_c = false;
-        for (int r = 0; r < g.ty && !need_r; r++) need_r = g.rowmask[by * g.ty + r] != 0;
-        for (int c = 0; c < g.tx && !need_c; c++) need_c = g.colmask[bx * g.tx + c] != 0;
-        if (!need_r || !need_c) return;
-    }
     const int p0 = chunk * g.pc;
     const int np = min(g.pc, g.nf3 - p0);
     const int cells = np * g.ty * g.tx;
@@ -228,6 +226,24 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
         size_t gi = ((size_t)(p0 + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
         fw[gi] = mkc((T)acc[k], (T)acc[cells + k]);
     }
+    __syncthreads();  // acc and the bin table are reused by the next item
+    }
+}
+
+// list[0] = number of tiles with a masked-in row and a masked-in column,
+// list[1..] = those tiles (any order; each is gridded independently)
+__global__ __launch_bounds__(256) void k_need_tiles(const uint8_t *__restrict__ rowmask,
+                                                    const uint8_t *__restrict__ colmask, int ntx,
+                                                    int64_t ntiles, int tx, int ty,
+                                                    int32_t *__restrict__ list)
+{
+    const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (t >= ntiles) return;
+    const int bx = (int)(t % ntx), by = (int)(t / ntx);
+    bool need_r = false, need_c = false;
+    for (int r = 0; r < ty && !need_r; r++) need_r = rowmask[by * ty + r] != 0;
+    for (int c = 0; c < tx && !need_c; c++) need_c = colmask[bx * tx + c] != 0;
+    if (need_r && need_c) list[1 + atomicAdd(list, 1)] = (int32_t)t;
 }
 
 // threads per gridding workgroup: enough waves per CU when the tile's LDS
@@ -247,7 +263,20 @@ static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, c
     if (g.sbx * g.sby > kMaxBins) return CG_ERR_SIZE;
     int is2d = g.nf3 == 1 ? 1 : 0;
     size_t lds = (size_t)grid_lds_bytes(g, (int)sizeof(cx_t<T>));
-    unsigned nblk = (unsigned)((int64_t)g.ntx * g.nty * g.nchunk);
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
+    unsigned nblk = (unsigned)(ntiles * g.nchunk);
+    if (g.tile_list) {
+        // build the list of tiles to write; workgroups then walk it
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, const_cast<int32_t *>(g.tile_list),
+                           (int64_t)0);
+        hipLaunchKernelGGL(k_need_tiles, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s,
+                           g.rowmask, g.colmask, g.ntx, ntiles, g.tx, g.ty,
+                           const_cast<int32_t *>(g.tile_list));
+        const int threads = grid_threads(lds);
+        const int per_cu = std::max(1, std::min((int)(163840 / std::max<size_t>(lds, 1)),
+                                                2048 / threads));
+        nblk = (unsigned)std::min<int64_t>(ntiles * g.nchunk, (int64_t)256 * per_cu);
+    }
     if (lds > 65536)
         CG_HIP(hipFuncSetAttribute((const void *)k_grid<T, KW>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

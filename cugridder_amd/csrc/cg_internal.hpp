@@ -97,8 +97,10 @@ struct GridArgs {
     int dense_min;  // > 0: bins with at least this many points go to the dense path
     double beta, es_c;
     // tiles whose rows all have rowmask 0, or whose columns all have
-    // colmask 0, are not written (null: every tile is written)
+    // colmask 0, are not written (null: every tile is written); the sparse
+    // gridder then collects the others in tile_list (ntiles + 1 ints)
     const uint8_t *rowmask, *colmask;
+    int32_t *tile_list;
     // learned kernel (learned != 0): the value at offset d is
     // exp(beta sqrt(1 - es_c d^2) + sum_k kq[k] s^k), s = 2 es_c d^2 - 1,
     // instead of the ES kernel's exp(beta sqrt(1 - es_c d^2))
