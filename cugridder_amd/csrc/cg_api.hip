@@ -8,10 +8,12 @@
 //
 // Device-pointer entry points (cg_*_dev) run the same pipeline on inputs
 // already resident in HBM with a reusable workspace held by a context.
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -591,9 +593,17 @@ static int set_learned(Ctx &C, const LearnedKernel &K, double eps)
     return CG_OK;
 }
 
+// Items run on `lanes` independent branches, each with its own workspace
+// (the graph holds its pointers) and stream, so small channels overlap;
+// items are spread over the lanes by estimated cost.
 struct Batch {
-    Ctx C;                // private workspace: the graph holds its pointers
-    int dir = 0;          // 0 ms2dirty, 1 dirty2ms
+    Ctx C;                // planning and option template
+    std::vector<std::unique_ptr<Ctx>> lane_ctx;
+    std::vector<hipStream_t> lane_stream;
+    std::vector<hipEvent_t> lane_done;
+    hipEvent_t fork = nullptr;
+    std::vector<int> lane;  // lane of each item
+    int dir = 0;            // 0 ms2dirty, 1 dirty2ms
     std::vector<cg_batch_item> items;
     std::vector<CallPlan> plans;
     std::vector<Tune> tunes;
@@ -608,6 +618,9 @@ struct Batch {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (cap) (void)hipStreamDestroy(cap);
+        for (hipStream_t st : lane_stream) (void)hipStreamDestroy(st);
+        for (hipEvent_t e : lane_done) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
     }
 };
 
@@ -622,17 +635,81 @@ static int batch_plan(Batch &B, size_t i, const Tune &tn, CallPlan &cp)
                                         (const double *)it.d_uvw, B.eps, B.sigma, B.sign, true, tn);
 }
 
+// item i on its lane's workspace and current stream
 static int batch_exec(Batch &B, size_t i)
 {
     const cg_batch_item &it = B.items[i];
-    StageClock clk(B.C, B.dir);
+    Ctx &L = *B.lane_ctx[B.lane[i]];
+    StageClock clk(L, B.dir);
     if (B.dir == 0)
-        return exec_ms2dirty<float, double, float>(B.C, B.plans[i], (const double *)it.d_uvw,
+        return exec_ms2dirty<float, double, float>(L, B.plans[i], (const double *)it.d_uvw,
                                                    (const float2 *)it.d_vis, (const float *)it.d_wgt,
                                                    (float2 *)it.d_dirty, clk);
-    return exec_dirty2ms<float, double, float>(B.C, B.plans[i], (const double *)it.d_uvw,
+    return exec_dirty2ms<float, double, float>(L, B.plans[i], (const double *)it.d_uvw,
                                                (float2 *)it.d_vis, (const float *)it.d_wgt,
                                                (const float2 *)it.d_dirty, clk);
+}
+
+// every item, lanes forked from and joined back into stream s (captured
+// into the graph as parallel branches when s is capturing)
+static int batch_run_lanes(Batch &B, hipStream_t s)
+{
+    const size_t nl = B.lane_ctx.size();
+    CG_HIP(hipEventRecord(B.fork, s));
+    for (size_t k = 0; k < nl; k++) {
+        CG_HIP(hipStreamWaitEvent(B.lane_stream[k], B.fork, 0));
+        B.lane_ctx[k]->stream = B.lane_stream[k];
+    }
+    int r = CG_OK;
+    for (size_t i = 0; i < B.items.size() && !r; i++) r = batch_exec(B, i);
+    for (size_t k = 0; k < nl; k++) {
+        if (hipEventRecord(B.lane_done[k], B.lane_stream[k]) != hipSuccess ||
+            hipStreamWaitEvent(s, B.lane_done[k], 0) != hipSuccess)
+            r = r ? r : CG_ERR_HIP;
+    }
+    return r;
+}
+
+// lanes by longest-first greedy assignment of a cost estimate (rows plus
+// grid cells), each lane a copy of the template context's options
+static int batch_lanes(Batch &B, int nlanes)
+{
+    const size_t n = B.items.size();
+    nlanes = std::max(1, std::min<int>(nlanes, (int)n));
+    for (int k = 0; k < nlanes; k++) {
+        std::unique_ptr<Ctx> L(new Ctx());
+        L->lds_budget = B.C.lds_budget;
+        L->grid_lds_budget = B.C.grid_lds_budget;
+        L->tile_grid = B.C.tile_grid;
+        L->tile_degrid = B.C.tile_degrid;
+        L->dense_min = B.C.dense_min;
+        L->row_mask = B.C.row_mask;
+        L->use_learned = B.C.use_learned;
+        for (int kw = 0; kw <= LEARN_MAXKW; kw++) L->learned[kw] = B.C.learned[kw];
+        L->stream = B.user;
+        B.lane_ctx.push_back(std::move(L));
+        hipStream_t st = nullptr;
+        CG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        B.lane_stream.push_back(st);
+        hipEvent_t e = nullptr;
+        CG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B.lane_done.push_back(e);
+    }
+    CG_HIP(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+    std::vector<size_t> order(n);
+    std::vector<double> cost(n), load(nlanes, 0.0);
+    for (size_t i = 0; i < n; i++) {
+        order[i] = i;
+        cost[i] = (double)B.items[i].nrow + 0.25 * (double)B.plans[i].P.grid_cells();
+    }
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    B.lane.assign(n, 0);
+    for (size_t i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        B.lane[i] = k;
+        load[k] += cost[i];
+    }
+    return CG_OK;
 }
 
 // time one execution of item i under tuning tn (after one untimed run)
@@ -645,10 +722,11 @@ static int batch_time(Batch &B, size_t i, const Tune &tn, float &ms)
     hipEvent_t e0, e1;
     CG_HIP(hipEventCreate(&e0));
     CG_HIP(hipEventCreate(&e1));
+    const hipStream_t st = B.lane_ctx[B.lane[i]]->stream;
     int r = batch_exec(B, i);
-    if (!r && hipEventRecord(e0, B.C.stream) != hipSuccess) r = CG_ERR_HIP;
+    if (!r && hipEventRecord(e0, st) != hipSuccess) r = CG_ERR_HIP;
     if (!r) r = batch_exec(B, i);
-    if (!r && hipEventRecord(e1, B.C.stream) != hipSuccess) r = CG_ERR_HIP;
+    if (!r && hipEventRecord(e1, st) != hipSuccess) r = CG_ERR_HIP;
     if (!r && hipEventSynchronize(e1) != hipSuccess) r = CG_ERR_HIP;
     if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = CG_ERR_HIP;
     (void)hipEventDestroy(e0);
@@ -699,13 +777,14 @@ static int batch_autotune(Batch &B)
     return CG_OK;
 }
 
-static int batch_build(Batch &B, int flags)
+static int batch_build(Batch &B, int flags, int nlanes)
 {
     const size_t n = B.items.size();
     B.plans.assign(n, CallPlan());
     B.tunes.assign(n, ctx_tune(B.C));
     B.tuned_ms.assign(n, 0.f);
     for (size_t i = 0; i < n; i++) CG_TRY(batch_plan(B, i, B.tunes[i], B.plans[i]));
+    CG_TRY(batch_lanes(B, nlanes));
     if (flags & CG_BATCH_AUTOTUNE) CG_TRY(batch_autotune(B));
     int ier = 0;
     for (size_t i = 0; i < n; i++) {
@@ -713,19 +792,16 @@ static int batch_build(Batch &B, int flags)
         if (r >= 2) return r;
         ier = std::max(ier, r);
     }
-    // one eager pass: the workspace reaches its final size
+    // one eager pass: every lane's workspace reaches its final size
     for (size_t i = 0; i < n; i++) CG_TRY(batch_exec(B, i));
-    CG_HIP(hipStreamSynchronize(B.C.stream));
+    CG_HIP(hipStreamSynchronize(B.user));
     if (flags & CG_BATCH_GRAPH) {
         // capture on a private stream (the context's may be the null stream)
         CG_HIP(hipStreamCreateWithFlags(&B.cap, hipStreamNonBlocking));
-        B.C.stream = B.cap;
         CG_HIP(hipStreamBeginCapture(B.cap, hipStreamCaptureModeThreadLocal));
-        int r = CG_OK;
-        for (size_t i = 0; i < n && !r; i++) r = batch_exec(B, i);
+        const int r = batch_run_lanes(B, B.cap);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(B.cap, &g);
-        B.C.stream = B.user;
         if (r) {
             if (g) (void)hipGraphDestroy(g);
             return r;
@@ -1295,7 +1371,9 @@ int cg_batch_create(void **batch, void *ctx, int direction, int n, const cg_batc
         B->eps = epsilon;
         B->sigma = sigma;
         B->sign = sign;
-        const int ier = batch_build(*B, flags);
+        int lanes = 4;
+        if (const char *e = std::getenv("CG_BATCH_LANES")) lanes = std::max(1, std::atoi(e));
+        const int ier = batch_build(*B, flags, lanes);
         if (ier >= 2) {
             delete B;
             return ier;
@@ -1315,8 +1393,7 @@ int cg_batch_run(void *batch)
         CG_HIP(hipGraphLaunch(B->exec, B->user));
         return CG_OK;
     }
-    for (size_t i = 0; i < B->items.size(); i++) CG_TRY(batch_exec(*B, i));
-    return CG_OK;
+    return batch_run_lanes(*B, B->user);
 }
 
 int cg_batch_info(void *batch, double *out)
