@@ -178,6 +178,21 @@ static int grid_masks(Ctx &C, GridArgs &g, const uint8_t *mask, int nf1, int nf2
     return CG_OK;
 }
 
+// scratch for the two-pass records (complex64 ms2dirty, whose records read
+// uvw, vis and wgt; dirty2ms reads less and keeps the one-pass gather): the
+// grid buffer, which is not in use until after the records are built
+template <typename T>
+static int records_scratch(Ctx &C, const WPlan &P, int64_t M, int dir, float4 *&rec)
+{
+    rec = nullptr;
+    if (sizeof(T) != 4 || M <= 0 || dir != 0) return CG_OK;
+    const char *e = std::getenv("CG_PREREC");
+    if (e && std::atoi(e) == 0) return CG_OK;
+    CG_TRY(ensure(C.grid, std::max((size_t)P.grid_cells() * sizeof(cx_t<T>), (size_t)32 * M)));
+    rec = as<float4>(C.grid);
+    return CG_OK;
+}
+
 // twiddle tables (cached per length and precision)
 template <typename T>
 static int twiddles(Ctx &C, int n1, int n2)
@@ -386,8 +401,10 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
     CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
+    float4 *rec = nullptr;
+    CG_TRY(records_scratch<T>(C, P, M, 0, rec));
     CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, vis, wgt, as<int32_t>(C.vals), M, a, 0,
-                                                as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
+                                                as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), rec,
                                                 C.stream)));
     CG_TRY(clk.mark(ST_GRID));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
@@ -493,9 +510,11 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
     CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
+    float4 *rec = nullptr;
+    CG_TRY(records_scratch<T>(C, P, M, 1, rec));
     CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, (const cx_t<T> *)nullptr, wgt,
                                                 as<int32_t>(C.vals), M, a, 1, as<geo_t<T>>(C.geo),
-                                                as<cx_t<T>>(C.val), C.stream)));
+                                                as<cx_t<T>>(C.val), rec, C.stream)));
     CG_TRY(clk.mark(ST_IMAGE));
     const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
     CG_TRY(ensure(C.zq, sizeof(double) * nq));

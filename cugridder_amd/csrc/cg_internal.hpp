@@ -62,10 +62,12 @@ template <> struct Geo<float> { using type = float4; };
 template <> struct Geo<double> { using type = double4; };
 template <typename T> using geo_t = typename Geo<T>::type;
 
+// sorted records geo / val of the points; with rec (32 B per point of
+// scratch, complex64 only) in two coalescing passes, else one gather pass
 template <typename T, typename TU, typename TV, typename TW>
 int launch_records(const TU *uvw, const TV *vis, const TW *wgt, const int32_t *perm,
                    int64_t nrow, const CoordArgs &a, int adjoint, geo_t<T> *geo,
-                   cx_t<T> *val, hipStream_t s);
+                   cx_t<T> *val, float4 *rec, hipStream_t s);
 template <typename T, typename TU, typename TV>
 int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm,
                       int64_t m, const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val,

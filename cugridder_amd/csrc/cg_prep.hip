@@ -398,11 +398,71 @@ __global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, c
     val[i] = mkc((T)c.x, (T)c.y);
 }
 
+// The same records for complex64 in two passes: every point's record is
+// computed in the caller's order (coalesced reads of uvw, vis, wgt) into a
+// 32-byte slot {geo, val, pad}, then one aligned 32-byte slot is gathered
+// per sorted position (instead of three scattered reads).  Bit-identical to
+// k_records.
+template <typename TU, typename TV, typename TW>
+__global__ __launch_bounds__(256) void k_prerecords(const TU *uvw, const TV *vis, const TW *wgt,
+                                                    int64_t nrow, CoordArgs a, int adjoint,
+                                                    float4 *__restrict__ rec)
+{
+    const int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= nrow) return;
+    const PointScale ps = point_scale(j, a);
+    const int64_t r = ps.row;
+    double u = (double)uvw[3 * r], v = (double)uvw[3 * r + 1], w = (double)uvw[3 * r + 2];
+    Foot f = make_foot(u, v, w, a, ps.suv, ps.sw);
+    uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8) | ((uint32_t)f.z0 << 16);
+    double wt = wgt ? (double)wgt[j] : 1.0;
+    double ws = w * ps.sw;
+    double2 c;
+    if (!adjoint) {
+        double2 x = ld_c(vis, j);
+        double sn, co;
+        sincos(a.o_center * ws, &sn, &co);
+        c = make_double2(wt * (x.x * co - x.y * sn), wt * (x.x * sn + x.y * co));
+    } else {
+        double wp = (ws - a.i_center) / a.gamma;
+        double sn, co;
+        sincos(-(wp * a.gamma + a.i_center) * a.o_center, &sn, &co);
+        c = make_double2(wt * co, wt * sn);
+    }
+    rec[2 * j] = make_geo<float>(f.d1, f.d2, f.d3, pk);
+    rec[2 * j + 1] = make_float4((float)c.x, (float)c.y, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_gather_records(const float4 *__restrict__ rec,
+                                                        const int32_t *__restrict__ perm,
+                                                        int64_t nrow, float4 *__restrict__ geo,
+                                                        float2 *__restrict__ val)
+{
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= nrow) return;
+    int64_t j = perm[i];
+    if (j < 0 || j >= nrow) j = 0;  // a valid permutation never takes this
+    geo[i] = rec[2 * j];
+    const float4 t = rec[2 * j + 1];
+    val[i] = make_float2(t.x, t.y);
+}
+
 template <typename T, typename TU, typename TV, typename TW>
 int launch_records(const TU *uvw, const TV *vis, const TW *wgt, const int32_t *perm, int64_t nrow,
-                   const CoordArgs &a, int adjoint, geo_t<T> *geo, cx_t<T> *val, hipStream_t s)
+                   const CoordArgs &a, int adjoint, geo_t<T> *geo, cx_t<T> *val, float4 *rec,
+                   hipStream_t s)
 {
     if (nrow == 0) return CG_OK;
+    if constexpr (sizeof(T) == 4) {
+        if (rec) {
+            hipLaunchKernelGGL((k_prerecords<TU, TV, TW>), dim3((unsigned)ceil_div(nrow, 256)),
+                               dim3(256), 0, s, uvw, vis, wgt, nrow, a, adjoint, rec);
+            hipLaunchKernelGGL(k_gather_records, dim3((unsigned)ceil_div(nrow, 256)), dim3(256), 0, s,
+                               (const float4 *)rec, perm, nrow, geo, val);
+            CG_HIP(hipGetLastError());
+            return CG_OK;
+        }
+    }
     hipLaunchKernelGGL((k_records<T, TU, TV, TW>), dim3((unsigned)ceil_div(nrow, 256)), dim3(256), 0,
                        s, uvw, vis, wgt, perm, nrow, a, adjoint, geo, val);
     CG_HIP(hipGetLastError());
@@ -447,9 +507,9 @@ template int launch_keys<float>(const float *, int64_t, const CoordArgs &, uint3
 template int launch_keys<double>(const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<float>(const float *, const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<double>(const double *, const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
-template int launch_records<float, float, float2, float>(const float *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, hipStream_t);
-template int launch_records<float, double, float2, float>(const double *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, hipStream_t);
-template int launch_records<double, double, double2, double>(const double *, const double2 *, const double *, const int32_t *, int64_t, const CoordArgs &, int, double4 *, double2 *, hipStream_t);
+template int launch_records<float, float, float2, float>(const float *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, float4 *, hipStream_t);
+template int launch_records<float, double, float2, float>(const double *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, float4 *, hipStream_t);
+template int launch_records<double, double, double2, double>(const double *, const double2 *, const double *, const int32_t *, int64_t, const CoordArgs &, int, double4 *, double2 *, float4 *, hipStream_t);
 template int launch_records_xy<float, float, float2>(const float *, const float *, const float2 *, const int32_t *, int64_t, const CoordArgs &, float4 *, float2 *, hipStream_t);
 template int launch_records_xy<double, double, double2>(const double *, const double *, const double2 *, const int32_t *, int64_t, const CoordArgs &, double4 *, double2 *, hipStream_t);
 
