@@ -133,12 +133,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *key
 {
     __shared__ int cnt[kRadix];
     cnt[threadIdx.x] = 0;
-    __syncthreads();
     int64_t base = blockIdx.x * (int64_t)kSortTile;
-#pragma unroll 4
+    // all loads first: one memory latency per block, not one per round
+    uint32_t kk[kSortRounds];
+#pragma unroll
     for (int k = 0; k < kSortRounds; k++) {
-        int64_t i = base + k * kSortThreads + threadIdx.x;
-        if (i < m) atomicAdd(&cnt[(keys[i] >> shift) & (kRadix - 1)], 1);
+        const int64_t i = base + k * kSortThreads + threadIdx.x;
+        kk[k] = i < m ? keys[i] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortRounds; k++) {
+        const int64_t i = base + k * kSortThreads + threadIdx.x;
+        if (i < m) atomicAdd(&cnt[(kk[k] >> shift) & (kRadix - 1)], 1);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
@@ -175,11 +182,21 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     s_lrun[tid] = ls;
     const uint64_t lt = __lanemask_lt();
     const int64_t base = blockIdx.x * (int64_t)kSortTile;
+    // all loads first: one memory latency per block, not one per round
+    uint32_t kk[kSortRounds];
+    int32_t vv[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; r++) {
+        const int64_t i = base + r * kSortThreads + tid;
+        kk[r] = i < m ? kin[i] : 0u;
+        vv[r] = i < m ? vin[i] : 0;
+    }
+#pragma unroll
     for (int r = 0; r < kSortRounds; r++) {
         const int64_t i = base + r * kSortThreads + tid;
         const bool valid = i < m;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const int32_t val = valid ? vin[i] : 0;
+        const uint32_t key = kk[r];
+        const int32_t val = vv[r];
         const int d = (int)((key >> shift) & (kRadix - 1));
 #pragma unroll
         for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
