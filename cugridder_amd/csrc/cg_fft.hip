@@ -512,8 +512,16 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
     const TwTab<T> tab = load_twtab<T>(buf + E, tw, N, SH);
     // rowmask[n2 + x] marks the columns points reach: the others are zero on
     // input (dir > 0; they need not have been written) or unread on output
-    // (dir < 0; they are not stored)
-    const uint8_t *colmask = rowmask ? rowmask + n2 : nullptr;
+    // (dir < 0; they are not stored).  EPT <= 32 columns per thread.
+    // this thread's columns k = threadIdx.x + u NT are the same in every row:
+    // their mask bits are read once
+    uint32_t cbits = 0xffffffffu;
+    if (rowmask) {
+        cbits = 0;
+#pragma unroll
+        for (int u = 0; u < EPT; u++)
+            if (rowmask[n2 + threadIdx.x + u * NT]) cbits |= 1u << u;
+    }
     for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
         // a masked row is zero (dir > 0) or unread (dir < 0): skip it
         if (rowmask && !rowmask[g % n2]) continue;
@@ -521,7 +529,7 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
-            const bool ld = DIR > 0 ? !(colmask && !colmask[k]) : live(k);
+            const bool ld = DIR > 0 ? ((cbits >> u) & 1u) != 0 : live(k);
             buf[k + (k >> 5)] = ld ? src[k] : mkc((T)0, (T)0);
         }
         lds_barrier();
@@ -530,7 +538,7 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
-            const bool st = DIR < 0 ? !(colmask && !colmask[k]) : live(k);
+            const bool st = DIR < 0 ? ((cbits >> u) & 1u) != 0 : live(k);
             if (st) dst[k] = buf[k + (k >> 5)];
         }
         lds_barrier();
