@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "cg_internal.hpp"
@@ -42,9 +43,12 @@ struct Ctx {
     int lds_budget = 155648;      // degridding tile + halo staging
     int grid_lds_budget = 65536;  // gridding accumulators
     int tile_grid = 8;
+    int tile_grid2d = 32;  // 2-D (one plane) gridding tiles
+    int mfma2d = 1;        // 2-D complex64 gridding as per-quadrant MFMA products
     int tile_degrid = 16;
     int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
     int row_mask = 1;        // let the FFT skip grid rows no point reaches
+    int prerec = 1;          // complex64 records in two coalescing passes
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
     int last_cshift = 0;
@@ -113,6 +117,9 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_GRID_LDS")) C.grid_lds_budget = std::atoi(e);
     if (const char *e = std::getenv("CG_DENSE_MIN")) C.dense_min = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_GRID")) C.tile_grid = std::atoi(e);
+    if (const char *e = std::getenv("CG_TILE_GRID2D")) C.tile_grid2d = std::atoi(e);
+    if (const char *e = std::getenv("CG_PREREC")) C.prerec = std::atoi(e);
+    if (const char *e = std::getenv("CG_MFMA2D")) C.mfma2d = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
 }
 
@@ -185,9 +192,7 @@ template <typename T>
 static int records_scratch(Ctx &C, const WPlan &P, int64_t M, int dir, float4 *&rec)
 {
     rec = nullptr;
-    if (sizeof(T) != 4 || M <= 0 || dir != 0) return CG_OK;
-    const char *e = std::getenv("CG_PREREC");
-    if (e && std::atoi(e) == 0) return CG_OK;
+    if (sizeof(T) != 4 || M <= 0 || dir != 0 || !C.prerec) return CG_OK;
     CG_TRY(ensure(C.grid, std::max((size_t)P.grid_cells() * sizeof(cx_t<T>), (size_t)32 * M)));
     rec = as<float4>(C.grid);
     return CG_OK;
@@ -258,6 +263,11 @@ static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, in
     g.nchunk = t.nchunk;
     g.beta = P.conv.beta;
     g.es_c = P.conv.es_c;
+    g.f_nb = (float)(-g.beta * 1.4426950408889634);
+    g.f_ec = (float)g.es_c;
+    g.f_e1 = (float)std::exp(g.beta);
+    g.f_e2 = (float)std::exp(2.0 * g.beta);
+    g.e3 = std::exp(3.0 * g.beta);
     return g;
 }
 
@@ -855,7 +865,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = conv.kw;
-    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.grid_lds_budget, C.tile_grid, 0)
+    Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.grid_lds_budget, C.tile_grid2d, 0)
                          : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
     C.last_plan = P;
     C.last_tiling = t;
@@ -872,6 +882,16 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     const int64_t ntiles = t.ntiles();
     CG_TRY(clk.mark(ST_SORT));
     CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
+    // complex64 with FP32 coordinates: the key pass also writes the caller-order
+    // records into the grid buffer, which is idle until gridding / deconvolution
+    float4 *rec = nullptr;
+    if constexpr (sizeof(T) == 4 && std::is_same<TU, float>::value) {
+        if (M > 0 && C.prerec) {
+            CG_TRY(ensure(C.grid, std::max((size_t)P.nf1 * P.nf2 * eb,
+                                           (size_t)(type == 1 ? 32 : 16) * M)));
+            rec = as<float4>(C.grid);
+        }
+    }
     if (M > 0) {
         CG_TRY(ensure(C.keys, sizeof(uint32_t) * M));
         CG_TRY(ensure(C.keys2, sizeof(uint32_t) * M));
@@ -880,7 +900,14 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         const int64_t hs = radix_hist_size(M);
         CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
         CG_TRY(ensure(C.scan, sizeof(int32_t) * scan_tmp_size(hs)));
-        CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
+        if constexpr (sizeof(T) == 4 && std::is_same<TU, float>::value) {
+            if (rec)
+                CG_TRY(launch_keys_prerecords_xy(x, y, type == 1 ? (const float2 *)c : nullptr, M, a,
+                                                 as<uint32_t>(C.keys), as<int32_t>(C.vals), rec,
+                                                 C.stream));
+        }
+        if (!rec)
+            CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
         CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
                                  as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles),
                                  as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream));
@@ -892,9 +919,13 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
     CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
-    CG_TRY((launch_records_xy<T, TU, cx_t<T>>(x, y, type == 1 ? c : (const cx_t<T> *)nullptr,
-                                              as<int32_t>(C.vals), M, a, as<geo_t<T>>(C.geo),
-                                              as<cx_t<T>>(C.val), C.stream)));
+    if (rec)
+        CG_TRY(launch_gather_records_xy(rec, as<int32_t>(C.vals), M, type == 1 ? 1 : 0,
+                                        (float4 *)C.geo.p, (float2 *)C.val.p, C.stream));
+    else
+        CG_TRY((launch_records_xy<T, TU, cx_t<T>>(x, y, type == 1 ? c : (const cx_t<T> *)nullptr,
+                                                  as<int32_t>(C.vals), M, a, as<geo_t<T>>(C.geo),
+                                                  as<cx_t<T>>(C.val), C.stream)));
     CG_TRY(ensure(C.grid, (size_t)P.nf1 * P.nf2 * eb));
     CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
     CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
@@ -903,6 +934,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.mfma2d = C.mfma2d;
     // fast-axis columns that hold modes; grid rows that points reach; mode
     // frequencies along nf2
     const int la = N1 - N1 / 2, lb = P.nf1 - N1 / 2;
@@ -1232,6 +1264,9 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "lds_budget")) C->lds_budget = value;
     else if (!std::strcmp(name, "grid_lds_budget")) C->grid_lds_budget = value;
     else if (!std::strcmp(name, "tile_grid")) C->tile_grid = value;
+    else if (!std::strcmp(name, "tile_grid2d")) C->tile_grid2d = value;
+    else if (!std::strcmp(name, "prerec")) C->prerec = value;
+    else if (!std::strcmp(name, "mfma2d")) C->mfma2d = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;

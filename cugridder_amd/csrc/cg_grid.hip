@@ -31,24 +31,51 @@ namespace cg {
 __device__ __forceinline__ uint32_t geo_bits(float w) { return __float_as_uint(w); }
 __device__ __forceinline__ uint32_t geo_bits(double w) { return (uint32_t)__double_as_longlong(w); }
 
-template <typename T>
-__device__ __forceinline__ T es_val(T d, T beta, T c, T hw);
-template <>
-__device__ __forceinline__ float es_val<float>(float d, float beta, float c, float hw)
-{
-    return fabsf(d) < hw ? expf(beta * sqrtf(1.0f - c * d * d)) : 0.0f;
-}
-template <>
-__device__ __forceinline__ double es_val<double>(double d, double beta, double c, double hw)
+__device__ __forceinline__ double es_val(double d, double beta, double c, double hw)
 {
     return fabs(d) < hw ? exp(beta * sqrt(1.0 - c * d * d)) : 0.0;
 }
+
+// FP32 kernel values divided by e^beta (the MFMA gridders scale their sums
+// back once, the other FP32 kernels every value):
+//   psi(d) / e^beta = exp(beta (sqrt(1 - u) - 1) + y) = 2^(-beta log2(e) u / (1 + sqrt(1 - u)) + log2(e) y),
+// u = c d^2, y the learned kernel's polynomial in s = 2 u - 1 (0 for the ES
+// kernel).  The rewritten exponent has no cancellation near the centre, so
+// the hardware square root, reciprocal and exp2 (about ten instructions
+// instead of the library's correctly rounded expf and sqrtf) keep the large
+// central taps within a few ulp; only the taps near the support edge, whose
+// values are ~e^-beta of the centre, see errors up to ~beta * 1e-7.
+struct KernelEvalN {
+    float nb, ec, hw;
+    int learned;
+    const float *q;  // the launch argument's polynomial coefficients
+    __device__ __forceinline__ explicit KernelEvalN(const GridArgs &g, int kw)
+        : nb(g.f_nb), ec(g.f_ec), hw(0.5f * (float)kw), learned(g.learned), q(g.kq)
+    {
+    }
+    __device__ __forceinline__ float operator()(float off0, int t) const
+    {
+        const float d = off0 + (float)t;
+        const float u = ec * d * d;
+        float y = nb * u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - u));
+        if (learned) {
+            const float s = 2.0f * u - 1.0f;
+            float p = q[LEARN_MAXDEG];
+#pragma unroll
+            for (int k = LEARN_MAXDEG - 1; k >= 0; k--) p = p * s + q[k];
+            y += 1.4426950408889634f * p;
+        }
+        const float v = __builtin_amdgcn_exp2f(y);
+        return fabsf(d) < hw ? v : 0.0f;
+    }
+};
 
 // Kernel values of a point: the ES kernel exp(beta sqrt(1 - c d^2)) at the
 // tap offsets d, or the learned kernel, whose exponent adds the network's
 // polynomial sum_k q_k s^k in s = 2 c d^2 - 1 (c = (2/kw)^2).  The choice is
 // uniform across a launch and the coefficients are launch arguments, so
-// they stay in scalar registers.
+// they stay in scalar registers.  FP64 evaluates the formula directly; FP32
+// is KernelEvalN times e^beta.
 template <typename T>
 struct KernelEval {
     T beta, ec, hw;
@@ -64,7 +91,7 @@ struct KernelEval {
     __device__ __forceinline__ T operator()(T off0, int t) const
     {
         const T d = off0 + (T)t;
-        if (!learned) return es_val<T>(d, beta, ec, hw);
+        if (!learned) return es_val(d, beta, ec, hw);
         if (!(fabs(d) < hw)) return (T)0;
         const T r = (T)1 - ec * d * d;
         const T s = (T)1 - (T)2 * r;
@@ -72,6 +99,18 @@ struct KernelEval {
 #pragma unroll
         for (int k = LEARN_MAXDEG - 1; k >= 0; k--) y = y * s + q[k];
         return exp(beta * sqrt(r) + y);
+    }
+};
+template <>
+struct KernelEval<float> : KernelEvalN {
+    float eb;  // e^beta
+    __device__ __forceinline__ explicit KernelEval(const GridArgs &g, int kw)
+        : KernelEvalN(g, kw), eb(g.f_e1)
+    {
+    }
+    __device__ __forceinline__ float operator()(float off0, int t) const
+    {
+        return KernelEvalN::operator()(off0, t) * eb;
     }
 };
 
@@ -230,6 +269,192 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
     }
 }
 
+// ------------------------------------------------------------ 2-D on MFMA
+//
+// One plane (nf3 == 1), complex64: LDS FP64 atomics are what bounds the
+// sparse kernel (about 2.5 random lane-ops per clock per CU on gfx950, 72 per
+// point at kw 6).  Here a wave owns a 16 x 16 quadrant of a tile and
+// accumulates it in registers as a matrix product over the points that
+// touch it:
+//   C[i][j] = sum_k psi_y,k(i) * (c_k psi_x,k(j)),
+// four points per v_mfma_f32_16x16x4f32 (A[i][k] from lane i + 16 k, B[k][j]
+// from lane j + 16 k, D[4 (l / 16) + v][l % 16] in lane l, register v), real
+// and imaginary parts as two products.  The wave reads the points of the
+// bins that can reach its quadrant as one list, 64 at a time with the next
+// 64 already in flight, compacts the touching ones into an LDS queue (ballot
+// + prefix count, so the order and the sums are reproducible) and runs one
+// MFMA step per four queued points, carrying the rest (< 4) over.  The
+// quadrant is written once with plain stores.  The four waves of a
+// workgroup work on independent quadrants (no barriers).  Sums over points
+// are FP32 (MFMA accumulation).
+
+constexpr int kQ2 = 16;      // quadrant edge
+constexpr int kQueue = 68;   // queued points: < 4 carried + 64 new
+constexpr int kQWaves = 4;   // independent waves per workgroup
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct Queue2d {
+    int xs[kQueue], ys[kQueue];     // first tap relative to the quadrant origin
+    float gx[kQueue], gy[kQueue];   // offset of the first tap from the point
+    float vr[kQueue], vi[kQueue];   // strength
+    int beg[4], cnt[4], ox[4], oy[4];  // source bins
+};
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int KW>
+__global__ __launch_bounds__(64 * kQWaves) void k_grid2d_mfma(const float4 *__restrict__ geo,
+                                                             const float2 *__restrict__ val,
+                                                             const int32_t *__restrict__ tile_start,
+                                                             float2 *__restrict__ fw, GridArgs g)
+{
+    __shared__ Queue2d Qs[kQWaves];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    Queue2d &Q = Qs[wave];
+    const int nqx = g.tx / kQ2, nq = nqx * (g.ty / kQ2);
+    const int work = blockIdx.x * kQWaves + wave;
+    const int item = work / nq, quad = work - item * nq;
+    const int nitems = g.tile_list ? g.tile_list[0] : g.ntx * g.nty;
+    if (item >= nitems) return;  // the waves share no barrier
+    const int tile = g.tile_list ? g.tile_list[1 + item] : item;
+    const int qx = quad % nqx, qy = quad / nqx;
+    const int bx = tile % g.ntx, by = tile / g.ntx;
+    // source bins (sx, sy) in {0, 1}^2: this tile's, the left, upper and
+    // upper-left neighbours' (the tile edge is >= kw - 1).  Left bins reach
+    // only the quadrants of the tile's first 16 columns, upper bins only
+    // those of its first 16 rows (kw - 1 <= 16): others get count 0.
+    if (lane < 4) {
+        const int b = lane, sx = b & 1, sy = b >> 1;
+        int sby_ = by - sy, sbx_ = bx - sx;
+        if (sby_ < 0) sby_ += g.nty;
+        if (sbx_ < 0) sbx_ += g.ntx;
+        int oy = (sby_ - by) * g.ty, ox = (sbx_ - bx) * g.tx;
+        if (oy < 0) oy += g.nf2;
+        if (ox < 0) ox += g.nf1;
+        const int sidx = sby_ * g.ntx + sbx_;
+        const bool reach = !((sx && qx != 0) || (sy && qy != 0));
+        Q.beg[b] = tile_start[sidx];
+        Q.cnt[b] = reach ? tile_start[sidx + 1] - Q.beg[b] : 0;
+        Q.ox[b] = ox;
+        Q.oy[b] = oy;
+    }
+    wave_lds_sync();
+    const int n0 = Q.cnt[0], n1 = n0 + Q.cnt[1], n2 = n1 + Q.cnt[2], total = n2 + Q.cnt[3];
+    // point i of the list: its bin and position
+    auto fetch = [&](int i, float4 &G, float2 &v, int &b) {
+        b = i < n0 ? 0 : i < n1 ? 1 : i < n2 ? 2 : 3;
+        if (i < total) {
+            const int idx = Q.beg[b] + i - (b == 0 ? 0 : b == 1 ? n0 : b == 2 ? n1 : n2);
+            G = geo[idx];
+            v = val[idx];
+        }
+    };
+
+    const KernelEvalN ker(g, KW);  // values / e^beta: the sums are scaled by e^(2 beta)
+    const int k = lane >> 4, r = lane & 15;
+    v4f cre = {0.f, 0.f, 0.f, 0.f}, cim = {0.f, 0.f, 0.f, 0.f};
+    // one MFMA step over queued points e0 .. e0 + 3
+    auto step = [&](int e0) {
+        const int e = e0 + k;
+        const int dy = r - Q.ys[e], dx = r - Q.xs[e];
+        const float a = (dy >= 0 && dy < KW) ? ker(Q.gy[e], dy) : 0.f;
+        const float kx = (dx >= 0 && dx < KW) ? ker(Q.gx[e], dx) : 0.f;
+        cre = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Q.vr[e] * kx, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Q.vi[e] * kx, cim, 0, 0, 0);
+    };
+    int queued = 0;  // wave-uniform, < 4 between chunks
+    float4 G = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 v = make_float2(0.f, 0.f);
+    int b = 0;
+    fetch(lane, G, v, b);
+    for (int base = 0; base < total; base += 64) {
+        // the next 64 points are loaded while these are queued and stepped
+        float4 Gn = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 vn = make_float2(0.f, 0.f);
+        int bn = 0;
+        fetch(base + 64 + lane, Gn, vn, bn);
+        bool touch = false;
+        int xq = 0, yq = 0;
+        if (base + lane < total) {
+            const uint32_t pk = __float_as_uint(G.w);
+            int rx = Q.ox[b] + (int)(pk & 0xffu);
+            if (rx >= g.nf1) rx -= g.nf1;
+            int ry = Q.oy[b] + (int)((pk >> 8) & 0xffu);
+            if (ry >= g.nf2) ry -= g.nf2;
+            // first tap relative to the tile (negative: an earlier tile),
+            // then to the quadrant
+            xq = (rx < g.tx ? rx : rx - g.nf1) - qx * kQ2;
+            yq = (ry < g.ty ? ry : ry - g.nf2) - qy * kQ2;
+            touch = xq > -KW && xq < kQ2 && yq > -KW && yq < kQ2;
+        }
+        const uint64_t mask = __ballot(touch);
+        if (touch) {
+            const int pos = queued + (int)__popcll(mask & ((1ull << lane) - 1ull));
+            Q.xs[pos] = xq;
+            Q.ys[pos] = yq;
+            Q.gx[pos] = G.x;
+            Q.gy[pos] = G.y;
+            Q.vr[pos] = v.x;
+            Q.vi[pos] = v.y;
+        }
+        wave_lds_sync();
+        queued += (int)__popcll(mask);
+        const int full = queued & ~3;
+#pragma unroll 2
+        for (int e0 = 0; e0 < full; e0 += 4) step(e0);
+        const int rest = queued - full;
+        if (rest && full && lane < rest) {
+            // carry the last (< 4) queued points to the front (source
+            // entries >= 4, destinations < 3: the ranges are disjoint)
+            const int src = full + lane;
+            Q.xs[lane] = Q.xs[src];
+            Q.ys[lane] = Q.ys[src];
+            Q.gx[lane] = Q.gx[src];
+            Q.gy[lane] = Q.gy[src];
+            Q.vr[lane] = Q.vr[src];
+            Q.vi[lane] = Q.vi[src];
+        }
+        queued = rest;
+        wave_lds_sync();
+        G = Gn;
+        v = vn;
+        b = bn;
+    }
+    if (queued) {
+        // pad the last step with zero-strength points
+        if (lane >= queued && lane < 4) {
+            Q.xs[lane] = 0;
+            Q.ys[lane] = 0;
+            Q.gx[lane] = 0.f;
+            Q.gy[lane] = 0.f;
+            Q.vr[lane] = 0.f;
+            Q.vi[lane] = 0.f;
+        }
+        wave_lds_sync();
+        step(0);
+    }
+    // lane l holds rows 4 (l / 16) + v of column l % 16
+    const int gx0 = bx * g.tx + qx * kQ2 + r;
+    const int gy0 = by * g.ty + qy * kQ2 + 4 * k;
+    const float sc = g.f_e2;
+#pragma unroll
+    for (int v4 = 0; v4 < 4; v4++)
+        fw[(size_t)(gy0 + v4) * g.nf1 + gx0] = make_float2(cre[v4] * sc, cim[v4] * sc);
+}
+
+bool grid2d_mfma_supported(const GridArgs &g, int kw)
+{
+    return g.mfma2d && g.nf3 == 1 && kw >= 2 && kw <= kQ2 && (g.tx == 16 || g.tx == 32) &&
+           (g.ty == 16 || g.ty == 32) && g.ntx >= 2 && g.nty >= 2 && g.sbx == 2 && g.sby == 2 &&
+           g.nchunk == 1;
+}
+
 // list[0] = number of tiles with a masked-in row and a masked-in column,
 // list[1..] = those tiles (any order; each is gridded independently)
 __global__ __launch_bounds__(256) void k_need_tiles(const uint8_t *__restrict__ rowmask,
@@ -276,6 +501,18 @@ static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, c
         const int per_cu = std::max(1, std::min((int)(163840 / std::max<size_t>(lds, 1)),
                                                 2048 / threads));
         nblk = (unsigned)std::min<int64_t>(ntiles * g.nchunk, (int64_t)256 * per_cu);
+    }
+    if constexpr (sizeof(T) == 4 && KW <= kQ2) {
+        if (grid2d_mfma_supported(g, KW)) {
+            // one wave per quadrant of every tile (the hardware balances
+            // clustered tiles); with a tile list, waves past it return
+            const int64_t nwork = ntiles * (g.tx / kQ2) * (g.ty / kQ2);
+            const dim3 grid((unsigned)ceil_div(nwork, kQWaves)), block(64 * kQWaves);
+            hipLaunchKernelGGL(k_grid2d_mfma<KW>, grid, block, 0, s, (const float4 *)geo,
+                               (const float2 *)val, ts, (float2 *)fw, g);
+            CG_HIP(hipGetLastError());
+            return CG_OK;
+        }
     }
     if (lds > 65536)
         CG_HIP(hipFuncSetAttribute((const void *)k_grid<T, KW>,
@@ -344,7 +581,8 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
     const int nlist = list[0];
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5, i = lane & 31;
-    const KernelEval<float> ker(g, KW);
+    const KernelEvalN ker(g, KW);  // values / e^beta: the sums are scaled by e^(3 beta)
+    const double sc = g.e3;
     // lane roles: A[i][half] with tap i = t1 + 8 t2 (two 32-tap blocks),
     // B[half][j] with plane j = i.  Each lane evaluates one kernel value of
     // its point (i < 8: x tap i, 8 <= i < 16: y tap i - 8, 16 <= i < 24: z tap
@@ -457,13 +695,13 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
                     const int u1 = tap & 7, u2 = tap >> 3;  // block 0: t2 < 4; block 1: t2 + 4
                     if (u1 < KW && u2 < KW) {
                         const int idx = ((ly + u2) * RX + lx + u1) * NP + i;
-                        atomicAdd(are + idx, (double)r0[v]);
-                        atomicAdd(aim + idx, (double)q0[v]);
+                        atomicAdd(are + idx, (double)r0[v] * sc);
+                        atomicAdd(aim + idx, (double)q0[v] * sc);
                     }
                     if (u1 < KW && u2 + 4 < KW) {
                         const int idx = ((ly + u2 + 4) * RX + lx + u1) * NP + i;
-                        atomicAdd(are + idx, (double)r1[v]);
-                        atomicAdd(aim + idx, (double)q1[v]);
+                        atomicAdd(are + idx, (double)r1[v] * sc);
+                        atomicAdd(aim + idx, (double)q1[v] * sc);
                     }
                 }
             }

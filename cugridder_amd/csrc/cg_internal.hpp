@@ -72,6 +72,13 @@ template <typename T, typename TU, typename TV>
 int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm,
                       int64_t m, const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val,
                       hipStream_t s);
+// complex64 2-D: keys, identity vals and the caller-order record slots in one
+// pass (rec holds 32 B per point with c, 16 B without), then the gather
+int launch_keys_prerecords_xy(const float *x, const float *y, const float2 *c, int64_t m,
+                              const CoordArgs &a, uint32_t *keys, int32_t *vals, float4 *rec,
+                              hipStream_t s);
+int launch_gather_records_xy(const float4 *rec, const int32_t *perm, int64_t m, int has_c,
+                             float4 *geo, float2 *val, hipStream_t s);
 
 // ---- sort (cg_sort.hip): stable LSD radix sort of (key, val) pairs by the
 // low `key_bits` bits.  tmp_keys/tmp_vals have m entries; hist needs
@@ -108,7 +115,16 @@ struct GridArgs {
     // instead of the ES kernel's exp(beta sqrt(1 - es_c d^2))
     int learned;
     float kq[LEARN_MAXDEG + 1];
+    // FP32 kernel constants (set with beta, es_c): -beta log2(e), es_c, and
+    // e^beta, e^(2 beta), e^(3 beta) that scale values / sums of products of
+    // values divided by e^beta back
+    float f_nb, f_ec, f_e1, f_e2;
+    double e3;
+    // 1: complex64 one-plane gridding may run as per-quadrant MFMA products
+    // (grid2d_mfma_supported decides); 0: always the LDS-atomic kernel
+    int mfma2d;
 };
+bool grid2d_mfma_supported(const GridArgs &g, int kw);
 template <typename T>
 int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
                 cx_t<T> *fw, const GridArgs &g, int kw, hipStream_t s);

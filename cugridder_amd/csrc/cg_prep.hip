@@ -489,6 +489,75 @@ __global__ __launch_bounds__(256) void k_records_xy(const TU *x, const TU *y, co
     }
 }
 
+// 2-D complex64 records in two coalescing passes: the key pass also writes
+// every point's record in the caller's order (a 16-byte geo slot, plus a
+// 16-byte strength slot when c is given), the record pass gathers the slots
+// by the sorted permutation.  Bit-identical to k_keys_xy + k_records_xy.
+__global__ __launch_bounds__(256) void k_keys_prerecords_xy(const float *__restrict__ x,
+                                                            const float *__restrict__ y,
+                                                            const float2 *__restrict__ c,
+                                                            int64_t m, CoordArgs a,
+                                                            uint32_t *__restrict__ keys,
+                                                            int32_t *__restrict__ vals,
+                                                            float4 *__restrict__ rec)
+{
+    const int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= m) return;
+    const Foot f = make_foot_xy((double)x[j], (double)y[j], a);
+    keys[j] = tile_key(f, a);
+    vals[j] = (int32_t)j;
+    const uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8);
+    const float4 G = make_geo<float>(f.d1, f.d2, 0.0, pk);
+    if (c) {
+        const float2 v = c[j];
+        rec[2 * j] = G;
+        rec[2 * j + 1] = make_float4(v.x, v.y, 0.f, 0.f);
+    } else {
+        rec[j] = G;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_records_xy(const float4 *__restrict__ rec,
+                                                           const int32_t *__restrict__ perm,
+                                                           int64_t m, int has_c,
+                                                           float4 *__restrict__ geo,
+                                                           float2 *__restrict__ val)
+{
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i >= m) return;
+    int64_t j = perm[i];
+    if (j < 0 || j >= m) j = 0;  // a valid permutation never takes this
+    if (has_c) {
+        geo[i] = rec[2 * j];
+        const float4 t = rec[2 * j + 1];
+        val[i] = make_float2(t.x, t.y);
+    } else {
+        geo[i] = rec[j];
+        val[i] = make_float2(1.f, 0.f);
+    }
+}
+
+int launch_keys_prerecords_xy(const float *x, const float *y, const float2 *c, int64_t m,
+                              const CoordArgs &a, uint32_t *keys, int32_t *vals, float4 *rec,
+                              hipStream_t s)
+{
+    if (m == 0) return CG_OK;
+    hipLaunchKernelGGL(k_keys_prerecords_xy, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, x, y,
+                       c, m, a, keys, vals, rec);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+int launch_gather_records_xy(const float4 *rec, const int32_t *perm, int64_t m, int has_c,
+                             float4 *geo, float2 *val, hipStream_t s)
+{
+    if (m == 0) return CG_OK;
+    hipLaunchKernelGGL(k_gather_records_xy, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, rec,
+                       perm, m, has_c, geo, val);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
 template <typename T, typename TU, typename TV>
 int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm, int64_t m,
                       const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val, hipStream_t s)

@@ -126,9 +126,14 @@ int cg_ctx_destroy(void *ctx);
 /* options: "timing" (0/1, per-stage HIP events), "lds_budget" (degridding
  * staging bytes), "grid_lds_budget" (gridding accumulator bytes),
  * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
+ * "tile_grid2d" (the same for 2-D NUFFT type 1, <= 64; default 32),
+ * "mfma2d" (1: complex64 2-D NUFFT type 1 grids 16 x 16 quadrants as MFMA
+ * products when the tiling allows; 0: LDS-atomic gridding),
  * "dense_min" (points per gridding bin from which the complex64 gridder
  * uses per-start-cell MFMA products; 0 disables), "row_mask" (1: the FFT
  * skips grid rows that no point reaches; 0: transforms every row),
+ * "prerec" (1: complex64 records are built in the caller's order and
+ * gathered as aligned slots; 0: one gather pass; results are identical),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
 /* per-stage times summed over calls, out[15] = {ms2dirty[7], dirty2ms[7],

@@ -115,6 +115,54 @@ def test_golden_nufft2d(gpu, prec):
     assert rel_l2(f, ref1) <= max(eps, _floor(prec != "double"))
 
 
+def _clustered_xy(rng, m):
+    """80% of the points from 4 wrapped Gaussians (std 0.3), 20% uniform,
+    in [-pi, pi)^2 (the shape of the 2-D NUFFT benchmark workload)."""
+    nb = int(0.8 * m)
+    means = rng.uniform(-np.pi, np.pi, (4, 2))
+    p = np.concatenate([means[rng.integers(0, 4, nb)] + 0.3 * rng.standard_normal((nb, 2)),
+                        rng.uniform(-np.pi, np.pi, (m - nb, 2))])
+    p = np.remainder(p + np.pi, 2 * np.pi) - np.pi
+    return p[:, 0].astype(np.float32), p[:, 1].astype(np.float32)
+
+
+@pytest.mark.parametrize("tile,N1,N2,eps", [(8, 96, 64, 1e-5), (16, 96, 64, 1e-5), (32, 96, 64, 1e-5),
+                                             (64, 96, 64, 1e-5), (32, 32, 48, 2e-6), (16, 40, 24, 1e-3)])
+def test_nufft2d_tiles_and_records(gpu, tile, N1, N2, eps):
+    """complex64 2-D NUFFT with clustered points: every type-1 gridding tile
+    size against the FP32-ABI oracle; the MFMA quadrant gridder (tiles of 16
+    or 32) against the LDS-atomic one; the two record paths bit-identical;
+    type 2 (whose tiling the options leave alone) unchanged.  32 x 48 modes
+    give a grid of two tiles along x at tile 32."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(40 + tile + N1)
+    M = 60000
+    x, y = _clustered_xy(rng, M)
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    fk = (rng.standard_normal((N2, N1)) + 1j * rng.standard_normal((N2, N1))).astype(np.complex64)
+    tx, ty, tc, tf = (torch.from_numpy(a).cuda() for a in (x, y, c, fk))
+    out = {}
+    for prerec, mfma in ((1, 1), (0, 1), (1, 0)):
+        ctx = Context(tile_grid2d=tile, prerec=prerec, mfma2d=mfma)
+        try:
+            f1 = ctx.nufft2d1(tx, ty, tc, N1, N2, eps).cpu().numpy()
+            c2 = ctx.nufft2d2(tx, ty, tf, eps).cpu().numpy()
+        finally:
+            ctx.close()
+        out[prerec, mfma] = (f1, c2)
+    assert np.array_equal(out[1, 1][0], out[0, 1][0])
+    assert np.array_equal(out[1, 1][1], out[0, 1][1])
+    assert np.array_equal(out[1, 1][1], out[1, 0][1])
+    # FP32 MFMA sums against FP64 LDS sums of the same FP32 terms
+    assert rel_l2(out[1, 1][0], out[1, 0][0]) <= 1e-6
+    out = {1: out[1, 1]}
+    ref1 = O.nufft2d1(x, y, c, N1, N2, as_float_abi(eps), single=True)
+    ref2 = O.nufft2d2(x, y, fk, as_float_abi(eps), single=True)
+    assert rel_l2(out[1][0], ref1) <= max(eps, _floor(True))
+    assert rel_l2(out[1][1], ref2) <= max(eps, _floor(True))
+
+
 def test_cfg1_plain_full_size(gpu):
     """The plain one-channel workload (131072 rows, 1024^2, fov 2 deg,
     freq 1e8, eps 1e-5, mixed precision) against the FP64 restatement on the
