@@ -218,9 +218,11 @@ static int twiddles(Ctx &C, int n1, int n2)
 }
 
 // sort rows by the tile of their footprint start; leaves the permutation in
-// C.vals and the tile offsets in C.tstart
+// C.vals and the tile offsets in C.tstart.  With rec (complex64 ms2dirty),
+// the key pass also writes the caller-order record slots of vis and wgt.
 template <typename TU>
-static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const Tiling &t)
+static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const Tiling &t,
+                     const float2 *vis = nullptr, const float *wgt = nullptr, float4 *rec = nullptr)
 {
     const int64_t ntiles = t.ntiles();
     CG_TRY(ensure(C.keys, sizeof(uint32_t) * M));
@@ -231,7 +233,11 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
     CG_TRY(ensure(C.scan, sizeof(int32_t) * std::max(scan_tmp_size(hs), scan_tmp_size(ntiles + 1))));
     CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
-    CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
+    if (rec)
+        CG_TRY((launch_keys_prerecords<TU, float2, float>(uvw, vis, wgt, M, a, 0, as<uint32_t>(C.keys),
+                                                           as<int32_t>(C.vals), rec, C.stream)));
+    else
+        CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
     CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
                              as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles), as<int32_t>(C.hist),
                              as<int32_t>(C.scan), C.stream));
@@ -403,19 +409,29 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     C.last_m = M;
     C.last_cshift = a.cshift;
     CG_TRY(clk.mark(ST_SORT));
-    if (M > 0) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
-    else {
+    // complex64: the key pass writes the caller-order records too, and the
+    // record stage only gathers them
+    float4 *rec = nullptr;
+    if (M > 0) {
+        if constexpr (sizeof(T) == 4 && std::is_same<TW, float>::value) {
+            CG_TRY(records_scratch<T>(C, P, M, 0, rec));
+            if (rec) CG_TRY(sort_rows<TU>(C, uvw, M, a, t, (const float2 *)vis, wgt, rec));
+        }
+        if (!rec) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
+    } else {
         CG_TRY(ensure(C.tstart, sizeof(int32_t) * (t.ntiles() + 1)));
         CG_TRY(launch_fill_i32(as<int32_t>(C.tstart), t.ntiles() + 1, 0, C.stream));
     }
     CG_TRY(clk.mark(ST_RECORDS));
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
     CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
-    float4 *rec = nullptr;
-    CG_TRY(records_scratch<T>(C, P, M, 0, rec));
-    CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, vis, wgt, as<int32_t>(C.vals), M, a, 0,
-                                                as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), rec,
-                                                C.stream)));
+    if (rec)
+        CG_TRY(launch_gather_records(rec, as<int32_t>(C.vals), M, (float4 *)C.geo.p, (float2 *)C.val.p,
+                                     C.stream));
+    else
+        CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, vis, wgt, as<int32_t>(C.vals), M, a, 0,
+                                                    as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), nullptr,
+                                                    C.stream)));
     CG_TRY(clk.mark(ST_GRID));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
     const uint8_t *rm = nullptr;

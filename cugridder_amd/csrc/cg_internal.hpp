@@ -72,6 +72,15 @@ template <typename T, typename TU, typename TV>
 int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm,
                       int64_t m, const CoordArgs &a, geo_t<T> *geo, cx_t<T> *val,
                       hipStream_t s);
+// complex64: the key pass (launch_keys) and the caller-order record slots of
+// launch_records' two-pass path in one pass over uvw; launch_gather_records
+// is then the records' second pass
+template <typename TU, typename TV, typename TW>
+int launch_keys_prerecords(const TU *uvw, const TV *vis, const TW *wgt, int64_t nrow,
+                           const CoordArgs &a, int adjoint, uint32_t *keys, int32_t *vals,
+                           float4 *rec, hipStream_t s);
+int launch_gather_records(const float4 *rec, const int32_t *perm, int64_t nrow, float4 *geo,
+                          float2 *val, hipStream_t s);
 // complex64 2-D: keys, identity vals and the caller-order record slots in one
 // pass (rec holds 32 B per point with c, 16 B without), then the gather
 int launch_keys_prerecords_xy(const float *x, const float *y, const float2 *c, int64_t m,

@@ -404,12 +404,9 @@ __global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, c
 // per sorted position (instead of three scattered reads).  Bit-identical to
 // k_records.
 template <typename TU, typename TV, typename TW>
-__global__ __launch_bounds__(256) void k_prerecords(const TU *uvw, const TV *vis, const TW *wgt,
-                                                    int64_t nrow, CoordArgs a, int adjoint,
-                                                    float4 *__restrict__ rec)
+__device__ __forceinline__ void prerecord(const TU *uvw, const TV *vis, const TW *wgt, int64_t j,
+                                          const CoordArgs &a, int adjoint, float4 *__restrict__ rec)
 {
-    const int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
-    if (j >= nrow) return;
     const PointScale ps = point_scale(j, a);
     const int64_t r = ps.row;
     double u = (double)uvw[3 * r], v = (double)uvw[3 * r + 1], w = (double)uvw[3 * r + 2];
@@ -433,6 +430,50 @@ __global__ __launch_bounds__(256) void k_prerecords(const TU *uvw, const TV *vis
     rec[2 * j + 1] = make_float4((float)c.x, (float)c.y, 0.f, 0.f);
 }
 
+template <typename TU, typename TV, typename TW>
+__global__ __launch_bounds__(256) void k_prerecords(const TU *uvw, const TV *vis, const TW *wgt,
+                                                    int64_t nrow, CoordArgs a, int adjoint,
+                                                    float4 *__restrict__ rec)
+{
+    const int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= nrow) return;
+    prerecord(uvw, vis, wgt, j, a, adjoint, rec);
+}
+
+// the key pass (k_keys) and k_prerecords in one pass over uvw
+template <typename TU, typename TV, typename TW>
+__global__ __launch_bounds__(256) void k_keys_prerecords(const TU *uvw, const TV *vis,
+                                                         const TW *wgt, int64_t nrow, CoordArgs a,
+                                                         int adjoint, uint32_t *__restrict__ keys,
+                                                         int32_t *__restrict__ vals,
+                                                         float4 *__restrict__ rec)
+{
+    const int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (j >= nrow) return;
+    CoordArgs b = a;
+    b.has_w = 0;
+    const PointScale ps = point_scale(j, a);
+    const Foot f = make_foot((double)uvw[3 * ps.row], (double)uvw[3 * ps.row + 1], 0.0, b, ps.suv, ps.sw);
+    keys[j] = tile_key(f, a);
+    vals[j] = (int32_t)j;
+    prerecord(uvw, vis, wgt, j, a, adjoint, rec);
+}
+
+template <typename TU, typename TV, typename TW>
+int launch_keys_prerecords(const TU *uvw, const TV *vis, const TW *wgt, int64_t nrow,
+                           const CoordArgs &a, int adjoint, uint32_t *keys, int32_t *vals,
+                           float4 *rec, hipStream_t s)
+{
+    if (nrow == 0) return CG_OK;
+    hipLaunchKernelGGL((k_keys_prerecords<TU, TV, TW>), dim3((unsigned)ceil_div(nrow, 256)), dim3(256),
+                       0, s, uvw, vis, wgt, nrow, a, adjoint, keys, vals, rec);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
+int launch_gather_records(const float4 *rec, const int32_t *perm, int64_t nrow, float4 *geo,
+                          float2 *val, hipStream_t s);
+
 __global__ __launch_bounds__(256) void k_gather_records(const float4 *__restrict__ rec,
                                                         const int32_t *__restrict__ perm,
                                                         int64_t nrow, float4 *__restrict__ geo,
@@ -445,6 +486,16 @@ __global__ __launch_bounds__(256) void k_gather_records(const float4 *__restrict
     geo[i] = rec[2 * j];
     const float4 t = rec[2 * j + 1];
     val[i] = make_float2(t.x, t.y);
+}
+
+int launch_gather_records(const float4 *rec, const int32_t *perm, int64_t nrow, float4 *geo,
+                          float2 *val, hipStream_t s)
+{
+    if (nrow == 0) return CG_OK;
+    hipLaunchKernelGGL(k_gather_records, dim3((unsigned)ceil_div(nrow, 256)), dim3(256), 0, s, rec,
+                       perm, nrow, geo, val);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
 }
 
 template <typename T, typename TU, typename TV, typename TW>
@@ -576,6 +627,8 @@ template int launch_keys<float>(const float *, int64_t, const CoordArgs &, uint3
 template int launch_keys<double>(const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<float>(const float *, const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<double>(const double *, const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
+template int launch_keys_prerecords<float, float2, float>(const float *, const float2 *, const float *, int64_t, const CoordArgs &, int, uint32_t *, int32_t *, float4 *, hipStream_t);
+template int launch_keys_prerecords<double, float2, float>(const double *, const float2 *, const float *, int64_t, const CoordArgs &, int, uint32_t *, int32_t *, float4 *, hipStream_t);
 template int launch_records<float, float, float2, float>(const float *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, float4 *, hipStream_t);
 template int launch_records<float, double, float2, float>(const double *, const float2 *, const float *, const int32_t *, int64_t, const CoordArgs &, int, float4 *, float2 *, float4 *, hipStream_t);
 template int launch_records<double, double, double2, double>(const double *, const double2 *, const double *, const int32_t *, int64_t, const CoordArgs &, int, double4 *, double2 *, float4 *, hipStream_t);
