@@ -111,6 +111,7 @@ def lib():
     L.cg_ctx_stage_times.argtypes = [c_void_p, f64, ctypes.POINTER(ctypes.c_longlong)]
     L.cg_ctx_reset_times.argtypes = [c_void_p]
     L.cg_ctx_last_plan.argtypes = [c_void_p, f64]
+    L.cg_ctx_last_band.argtypes = [c_void_p, np.ctypeslib.ndpointer(np.int32, flags="C")]
     L.cg_ms2dirty_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                   c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                   c_double, c_double, c_int]
@@ -146,7 +147,7 @@ def lib():
     L.cg_learn_tabulate.argtypes = [c_int, c_int, f32, f32, f64]
     L.cg_ctx_set_kernel.argtypes = [c_void_p, c_int, c_int, f32n, c_double]
     for name in ("cg_ctx_create", "cg_ctx_destroy", "cg_ctx_set_option", "cg_ctx_stage_times",
-                 "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
+                 "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ctx_last_band", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
                  "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_batch_create", "cg_batch_run",
                  "cg_batch_info", "cg_batch_destroy",
                  "cg_nufft2d_dev", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",

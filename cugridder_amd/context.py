@@ -165,6 +165,13 @@ class Context:
             d[k] = int(d[k])
         return d
 
+    def last_band(self):
+        """(first row, rows) of the grid rows the last W-stacking call's points
+        reach along the column axis; rows = 0 when there is no band."""
+        out = np.zeros(2, np.int32)
+        lib().cg_ctx_last_band(self._h, out)
+        return int(out[0]), int(out[1])
+
     # ------------------------------------------------------------ test hooks
     @staticmethod
     def spectrum_rows(n2):

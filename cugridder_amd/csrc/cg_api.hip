@@ -49,6 +49,7 @@ struct Ctx {
     int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
     int row_mask = 1;        // let the FFT skip grid rows no point reaches
     int prerec = 1;          // complex64 records in two coalescing passes
+    int band_fft = 1;        // band-limited column passes when the rows points reach allow
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
     int last_cshift = 0;
@@ -120,6 +121,7 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_TILE_GRID2D")) C.tile_grid2d = std::atoi(e);
     if (const char *e = std::getenv("CG_PREREC")) C.prerec = std::atoi(e);
     if (const char *e = std::getenv("CG_MFMA2D")) C.mfma2d = std::atoi(e);
+    if (const char *e = std::getenv("CG_BAND_FFT")) C.band_fft = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
 }
 
@@ -289,9 +291,9 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
     if (nrow * (int64_t)nchan > (int64_t)INT32_MAX - 1) return CG_ERR_SIZE;
     for (int c = 0; c < nchan; c++)
         if (!(freq[c] > 0)) return CG_ERR_ARG;
-    CG_TRY(ensure(C.part, sizeof(double) * 2056));
-    double wlo = 0, whi = 0, wmin = 0, wmax = 0;
-    CG_TRY(launch_wminmax<TU>(uvw, nrow, 1.0, as<double>(C.part), wlo, whi, C.stream));
+    CG_TRY(ensure(C.part, sizeof(double) * 4104));
+    double wlo = 0, whi = 0, wmin = 0, wmax = 0, ulo = 0, uhi = 0;
+    CG_TRY(launch_wminmax<TU>(uvw, nrow, 1.0, as<double>(C.part), wlo, whi, C.stream, &ulo, &uhi));
     if (nrow > 0) {
         wmin = INFINITY;
         wmax = -INFINITY;
@@ -326,6 +328,17 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
         }
     }
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
+    // grid rows the points can reach: from the u range over all channels
+    if (nrow > 0 && C.band_fft) {
+        double smin = INFINITY, smax = 0;
+        for (int c = 0; c < nchan; c++) {
+            const double sc = 2.0 * kPi * (freq[c] / kC);
+            smin = std::min(smin, sc);
+            smax = std::max(smax, sc);
+        }
+        const double xlo = ulo < 0 ? ulo * smax : ulo * smin, xhi = uhi > 0 ? uhi * smax : uhi * smin;
+        plan_row_band(P, xlo, xhi);
+    }
     return ier;
 }
 
@@ -462,7 +475,8 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
                               as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
+                              C.band_fft ? P.band_len : 0));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_IMAGE));
     const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
@@ -566,7 +580,8 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
     const int sa = P.N2 - P.N2 / 2, sb = P.nf2 - P.N2 / 2;
     CG_TRY(clk.mark(ST_FFT2));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream));
+                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
+                              C.band_fft ? P.band_len : 0));
     CG_TRY(clk.mark(ST_FFT));
     CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, true,
                               as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
@@ -1283,6 +1298,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "tile_grid2d")) C->tile_grid2d = value;
     else if (!std::strcmp(name, "prerec")) C->prerec = value;
     else if (!std::strcmp(name, "mfma2d")) C->mfma2d = value;
+    else if (!std::strcmp(name, "band_fft")) C->band_fft = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
@@ -1330,6 +1346,15 @@ int cg_ctx_last_plan(void *ctx, double *out)
                     P.o_half_width, P.gamma, P.h, (double)C->last_tiling.tx,
                     (double)C->last_tiling.ty};
     std::memcpy(out, v, sizeof(v));
+    return CG_OK;
+}
+
+int cg_ctx_last_band(void *ctx, int *out)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !out) return CG_ERR_ARG;
+    out[0] = C->last_plan.band_s;
+    out[1] = C->band_fft ? C->last_plan.band_len : 0;
     return CG_OK;
 }
 

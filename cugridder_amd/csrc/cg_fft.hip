@@ -834,6 +834,156 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16, CB % 64 == 0 ? 4 : 2) void k
     }
 }
 
+// Band-limited columns, fp32.  When the non-zero rows of the input (forward)
+// or the rows wanted from the output (inverse) lie in one cyclic band of
+// B <= M rows (M = 1024 or 2048), a length-n2 column transform with
+// P = n2 / M is P transforms of length M:
+//   forward  X[ka + P kb] = FFT_M over r of  y_ka[r] = x[n_r] w^(n_r ka)
+//   inverse  x[n] = w^(n ka) summed over ka of IFFT_M(X[ka + P kb] over kb)[n mod M]
+// (w = e^(dir 2 pi i / n2), n_r the band row of residue r).  A workgroup
+// holds CB adjacent live columns (16 for M = 1024, 8 for M = 2048, so the
+// transform buffer is 128 KB of LDS); thread (e, b) keeps the band rows of
+// residues e + S j (S = 1024 / CB, j < 16) of column b in registers.  Those
+// residues are the inputs of the first radix-16 Stockham butterfly, which
+// therefore runs in registers; the other two passes run in LDS.  The
+// spectrum is read / written in the split order of the four-step column
+// pass (frequency k = ka + na kb at row ka nb + kb), live frequencies only.
+// No intermediate spectrum goes through HBM: the input is read once and the
+// output written once.
+constexpr int kBandTwSh = 7;  // two-level twiddle table: 128 low + n2 / 128 high entries
+
+// one Stockham pass of radix R (stage p = PP) over the CB columns of length
+// M in LDS, 1024 threads
+template <int R, int PP, int DIR, int M, int CB>
+__device__ __forceinline__ void band_pass(float2 *buf, const TwTab<float> &tab, int tws)
+{
+    constexpr int Tn = M / R, K = CB * Tn / 1024, LCB = CB == 16 ? 4 : 3;
+    static_assert(K >= 1 && CB * Tn % 1024 == 0 && (1 << LCB) == CB, "shape");
+    // addresses are recomputed per call rather than held across the caller's
+    // loop (hidden from loop-invariant code motion)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    float2 u[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = tid + 1024 * k, b = q & (CB - 1), i = q >> LCB;
+#pragma unroll
+        for (int r = 0; r < R; r++) u[k][r] = buf[(i + r * Tn) * CB + b];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = tid + 1024 * k, b = q & (CB - 1), i = q >> LCB;
+        const int kk = i % PP;
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = tab.get(r * kk * (M / (PP * R)) * tws);
+            if (DIR < 0) w.y = -w.y;
+            u[k][r] = cmul<float>(u[k][r], w);
+        }
+        dft_small<float, R, DIR>(u[k]);
+        const int j = (i - kk) * R + kk;
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[(j + r * PP) * CB + b] = u[k][r];
+    }
+    lds_barrier();
+}
+
+template <int DIR, int M>
+__global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, int n1, int np,
+                                                        int bs, int blen, Live live,
+                                                        const uint8_t *__restrict__ rowmask,
+                                                        int nash, int nb, int spec_a, int spec_b,
+                                                        const float2 *__restrict__ tw2)
+{
+    using T = float;
+    constexpr int CB = M == 1024 ? 16 : 8, S = 1024 / CB, LCB = CB == 16 ? 4 : 3;
+    static_assert(M == 1024 || M == 2048, "band length");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    const TwTab<T> tab = load_twtab<T>(buf + M * CB, tw2, n2, kBandTwSh);
+    const int tid = threadIdx.x, b = tid & (CB - 1), e = tid >> LCB;
+    const int nlive = live.count(n1);
+    const int c = blockIdx.x * CB + b;
+    const bool cok = c < nlive;
+    const int lc = cok ? live.at(c) : 0;
+    float2 *pl = data + (size_t)blockIdx.y * n2 * n1;
+    const int mask = n2 - 1, tws = n2 / M;
+    auto dead = [&](int k) { return k >= spec_a && k < spec_b; };
+    // split-order row of frequency k (na = 2^nash)
+    auto srow = [&](int k) { return (k & ((1 << nash) - 1)) * nb + (k >> nash); };
+    auto twid = [&](int n, int ka) {
+        float2 w = tab.get((n * ka) & mask);
+        if (DIR < 0) w.y = -w.y;
+        return w;
+    };
+    // residue e + S j of this thread: band offset o = (e + S j - bs) mod M,
+    // row (bs + o) mod n2 when o < blen
+    float2 xb[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int o = (e + S * j - bs) & (M - 1), n = (bs + o) & mask;
+        xb[j] = make_float2(0.f, 0.f);
+        if (DIR > 0 && o < blen && cok && (!rowmask || rowmask[n])) xb[j] = pl[(size_t)n * n1 + lc];
+    }
+    lds_barrier();  // twiddle table
+    for (int ka = 0; ka < np; ka++) {
+        // recomputing the band rows every iteration is cheaper than keeping
+        // their indices live across the loop: hide e from loop-invariant
+        // code motion
+        int eo = e;
+        asm volatile("" : "+v"(eo));
+        // v[j]: element e + S j of the length-M input, the inputs of the
+        // first radix-16 Stockham butterfly (no twiddles), run in registers
+        float2 v[16];
+        if (DIR > 0) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = cmul<T>(xb[j], twid(bs + ((eo + S * j - bs) & (M - 1)), ka));
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                // branch-free: a dead frequency loads row 0 (frequency 0, live
+                // and cached) and is selected away, so only live rows are read
+                // from HBM; lc < n1 for idle columns too
+                const int k = ka + np * (eo + S * m);
+                const bool dk = dead(k);
+                const float2 t = pl[(size_t)(dk ? 0 : srow(k)) * n1 + lc];
+                v[m] = (cok && !dk) ? t : make_float2(0.f, 0.f);
+            }
+        }
+        dft_small<T, 16, DIR>(v);
+#pragma unroll
+        for (int r = 0; r < 16; r++) buf[(16 * eo + r) * CB + b] = v[r];
+        lds_barrier();
+        band_pass<16, 16, DIR, M, CB>(buf, tab, tws);
+        if constexpr (M == 1024) band_pass<4, 256, DIR, M, CB>(buf, tab, tws);
+        else band_pass<8, 256, DIR, M, CB>(buf, tab, tws);
+        if (DIR > 0) {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int kb = eo + S * m, k = ka + np * kb;
+                if (cok && !dead(k)) pl[(size_t)srow(k) * n1 + lc] = buf[kb * CB + b];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int r = eo + S * j;
+                const float2 z = cmul<T>(buf[r * CB + b], twid(bs + ((r - bs) & (M - 1)), ka));
+                xb[j].x += z.x;
+                xb[j].y += z.y;
+            }
+        }
+        lds_barrier();
+    }
+    if (DIR < 0 && cok) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int o = (e + S * j - bs) & (M - 1);
+            if (o < blen) pl[(size_t)((bs + o) & mask) * n1 + lc] = xb[j];
+        }
+    }
+}
+
 // split order <-> natural order of the column axis, out of place:
 // to_natural: dst[k] = src[pos(k)], else dst[pos(k)] = src[k]
 template <typename T>
@@ -1113,12 +1263,35 @@ static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw,
 
 template <typename T, int DIR>
 static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2, Live live,
-                    const uint8_t *rowmask, int spec_a, int spec_b, hipStream_t s)
+                    const uint8_t *rowmask, int spec_a, int spec_b, int band_s, int band_len,
+                    hipStream_t s)
 {
     const int elem = (int)sizeof(cx_t<T>);
     const int nlive = live.count(n1);
     if (nlive <= 0 || batch <= 0) return CG_OK;
     int na, nb;
+    if constexpr (sizeof(T) == 4) {
+        // M = 1024 (16 columns per workgroup) for bands of <= 1024 rows, else
+        // M = 2048 (8 columns); the transform buffer is 128 KB either way
+        const int M = band_len <= 1024 ? 1024 : 2048, CB = M == 1024 ? 16 : 8;
+        const size_t lds = sizeof(float2) * ((size_t)M * CB + (1 << kBandTwSh) + (n2 >> kBandTwSh));
+        if (band_len > 0 && band_len <= M && lds <= (size_t)kLdsMax && band_s >= 0 && band_s < n2 &&
+            fft_knob("CG_FFT_BAND", 1) && fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
+            n2 >= 2 * M && fft_col_split(n2, na, nb)) {
+            const dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)batch);
+            auto go = [&](const void *fn, auto kern) -> int {
+                CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(kern, grid, dim3(1024), lds, s, (float2 *)data, n2, n1, n2 / M,
+                                   band_s, band_len, live, rowmask, ilog2(na), nb, spec_a, spec_b,
+                                   (const float2 *)tw2);
+                return CG_OK;
+            };
+            if (M == 1024) CG_TRY(go((const void *)k_fft_cols_band<DIR, 1024>, k_fft_cols_band<DIR, 1024>));
+            else CG_TRY(go((const void *)k_fft_cols_band<DIR, 2048>, k_fft_cols_band<DIR, 2048>));
+            CG_HIP(hipGetLastError());
+            return CG_OK;
+        }
+    }
     if (!fft_col_split(n2, na, nb)) {
         FftShape S{};
         S.n = n2;
@@ -1204,15 +1377,17 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
                     const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
-                    int spec_a, int spec_b, hipStream_t s)
+                    int spec_a, int spec_b, hipStream_t s, int band_s, int band_len)
 {
     if (batch <= 0) return CG_OK;
     Live live{live_a, live_b};
     if (rows) return dir > 0 ? fft_rows<T, 1>(data, batch, n2, n1, tw, live, rowmask, s)
                              : fft_rows<T, -1>(data, batch, n2, n1, tw, live, rowmask, s);
     if (n2 <= 1) return CG_OK;
-    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, s)
-                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, s);
+    return dir > 0 ? fft_cols<T, 1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, band_s,
+                                    band_len, s)
+                   : fft_cols<T, -1>(data, batch, n2, n1, tw, live, rowmask, spec_a, spec_b, band_s,
+                                     band_len, s);
 }
 
 template <typename T>
@@ -1289,8 +1464,8 @@ int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s)
 
 template int launch_fft2d<float>(float2 *, int, int, int, int, const float2 *, const float2 *, float2 *, hipStream_t);
 template int launch_fft2d<double>(double2 *, int, int, int, int, const double2 *, const double2 *, double2 *, hipStream_t);
-template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, int, int, hipStream_t);
-template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, int, int, hipStream_t);
+template int launch_fft_pass<float>(float2 *, int, int, int, int, bool, const float2 *, int, int, const uint8_t *, int, int, hipStream_t, int, int);
+template int launch_fft_pass<double>(double2 *, int, int, int, int, bool, const double2 *, int, int, const uint8_t *, int, int, hipStream_t, int, int);
 template bool fft_rows_masked<float>(int);
 template bool fft_rows_masked<double>(int);
 template bool fft_spectrum_pruned<float>(int);

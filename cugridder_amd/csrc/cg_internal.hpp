@@ -30,11 +30,13 @@ void set_coord_tiling(CoordArgs &a, const Tiling &t);
 int sort_key_bits(const CoordArgs &a, int64_t ntiles);
 
 // ---- prep (cg_prep.hip)
-// min/max of w * scale_w (FP64) over nrow rows of uvw [nrow][3]; partial
-// buffer needs 2*1024 doubles.  Result copied to host (synchronises).
+// min/max of w * scale_w (FP64) over nrow rows of uvw [nrow][3], and of u
+// when umin / umax are given; partial buffer needs 4100 doubles.  Results
+// copied to host (synchronises).
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
-                   double &wmin, double &wmax, hipStream_t s);
+                   double &wmin, double &wmax, hipStream_t s, double *umin = nullptr,
+                   double *umax = nullptr);
 // keys[j] = (tile of the footprint start of row j) << cshift | start cell
 // inside the tile; vals[j] = j
 template <typename TU>
@@ -171,10 +173,14 @@ int launch_fft2d(cx_t<T> *data, int batch, int n2, int n1, int dir, const cx_t<T
 // Frequencies k along n2 with spec_a <= k < spec_b are dead in the spectrum
 // (not needed after a dir > 0 column pass, zero before a dir < 0 one); the
 // split-order column step skips them when fft_spectrum_pruned(n2).
+// A column pass given a band (band_len > 0: rows band_s .. band_s + band_len
+// - 1, cyclic, hold every non-zero input row of a dir > 0 pass / every row a
+// dir < 0 pass must produce; rows outside it are neither read nor written)
+// runs as one band-limited pass when the band is short enough (complex64).
 template <typename T>
 int launch_fft_pass(cx_t<T> *data, int batch, int n2, int n1, int dir, bool rows,
                     const cx_t<T> *tw, int live_a, int live_b, const uint8_t *rowmask,
-                    int spec_a, int spec_b, hipStream_t s);
+                    int spec_a, int spec_b, hipStream_t s, int band_s = 0, int band_len = 0);
 // true when the column pass honours the dead spectrum rows (then a dir < 0
 // column pass does not read them and they need not be written); it then
 // also honours the row mask (a dir > 0 pass does not read masked rows)

@@ -153,6 +153,23 @@ void fseries_grid(const ConvOpts &o, int nf, double *out)
     }
 }
 
+void plan_row_band(WPlan &p, double xlo, double xhi)
+{
+    p.band_s = p.band_len = 0;
+    const double a = xlo * p.pix_x, b = xhi * p.pix_x;
+    // grid coordinate (x / 2 pi + 1/2) nf2 of the shift-rescale, valid while
+    // x stays inside [-pi, pi) (no wrap)
+    if (!(a <= b) || !(a > -PI + 1e-9) || !(b < PI - 1e-9)) return;
+    const double glo = (a * (0.5 / PI) + 0.5) * p.nf2, ghi = (b * (0.5 / PI) + 0.5) * p.nf2;
+    const int kw = p.conv.kw;
+    const int64_t y0 = (int64_t)std::ceil(glo - 0.5 * kw) - 1;
+    const int64_t y1 = (int64_t)std::ceil(ghi - 0.5 * kw) + kw;  // one past the last row, + 1 margin
+    const int64_t len = y1 - y0 + 1;
+    if (len <= 0 || len >= p.nf2) return;
+    p.band_s = (int)(((y0 % p.nf2) + p.nf2) % p.nf2);
+    p.band_len = (int)len;
+}
+
 Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,
                      int lds_budget, int tile_pref, int halo)
 {

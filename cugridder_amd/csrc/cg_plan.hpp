@@ -65,8 +65,17 @@ struct WPlan {
     double gamma = 0, h = 0;
     int ier = 0;
     Tiling tile;
+    // grid rows (nf2 axis) the points' footprints can reach: band_len rows
+    // from band_s, cyclic (band_len = 0: unknown or the whole axis)
+    int band_s = 0, band_len = 0;
     int64_t grid_cells() const { return (int64_t)nf1 * nf2 * nf3; }
 };
+
+// Sets p.band_s / p.band_len from the range [xlo, xhi] of u * 2 pi f / c
+// (radians per unit pixel angle, before multiplying by pix_x) over all
+// points: the footprint rows of the lowest and highest grid coordinate,
+// plus one row of margin on each side.  No band when the range wraps.
+void plan_row_band(WPlan &p, double xlo, double xhi);
 
 // Fills every field from the call parameters and the w range (min/max of
 // the scaled w = w*f/c*sign*2pi over all rows).  Returns ier (>=2 is fatal).

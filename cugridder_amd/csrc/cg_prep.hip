@@ -129,70 +129,81 @@ __device__ __forceinline__ uint32_t tile_key(const Foot &f, const CoordArgs &a)
 
 // ------------------------------------------------------------ w min/max
 
+// min/max of w * scale_w and of u over the rows: part[0, 1024) / [1024,
+// 2048) / [2048, 3072) / [3072, 4096) per block, finals at part[4096..4099]
 template <typename TU>
 __global__ __launch_bounds__(256) void k_wminmax(const TU *uvw, int64_t nrow, double scale_w,
                                                  double *part)
 {
-    __shared__ double smin[256], smax[256];
-    double mn = INFINITY, mx = -INFINITY;
+    __shared__ double sv[4][256];
+    double mn = INFINITY, mx = -INFINITY, un = INFINITY, ux = -INFINITY;
     for (int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x; j < nrow; j += (int64_t)gridDim.x * 256) {
         double w = (double)uvw[3 * j + 2] * scale_w;
+        double u = (double)uvw[3 * j];
         mn = fmin(mn, w);
         mx = fmax(mx, w);
+        un = fmin(un, u);
+        ux = fmax(ux, u);
     }
-    smin[threadIdx.x] = mn;
-    smax[threadIdx.x] = mx;
+    sv[0][threadIdx.x] = mn;
+    sv[1][threadIdx.x] = mx;
+    sv[2][threadIdx.x] = un;
+    sv[3][threadIdx.x] = ux;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
-            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            sv[0][threadIdx.x] = fmin(sv[0][threadIdx.x], sv[0][threadIdx.x + s]);
+            sv[1][threadIdx.x] = fmax(sv[1][threadIdx.x], sv[1][threadIdx.x + s]);
+            sv[2][threadIdx.x] = fmin(sv[2][threadIdx.x], sv[2][threadIdx.x + s]);
+            sv[3][threadIdx.x] = fmax(sv[3][threadIdx.x], sv[3][threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = smin[0];
-        part[1024 + blockIdx.x] = smax[0];
-    }
+    if (threadIdx.x < 4) part[1024 * threadIdx.x + blockIdx.x] = sv[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(1024) void k_minmax_final(double *part, int nb)
 {
-    __shared__ double smin[1024], smax[1024];
+    __shared__ double sv[4][1024];
     int t = threadIdx.x;
-    smin[t] = t < nb ? part[t] : INFINITY;
-    smax[t] = t < nb ? part[1024 + t] : -INFINITY;
+    sv[0][t] = t < nb ? part[t] : INFINITY;
+    sv[1][t] = t < nb ? part[1024 + t] : -INFINITY;
+    sv[2][t] = t < nb ? part[2048 + t] : INFINITY;
+    sv[3][t] = t < nb ? part[3072 + t] : -INFINITY;
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
         if (t < s) {
-            smin[t] = fmin(smin[t], smin[t + s]);
-            smax[t] = fmax(smax[t], smax[t + s]);
+            sv[0][t] = fmin(sv[0][t], sv[0][t + s]);
+            sv[1][t] = fmax(sv[1][t], sv[1][t + s]);
+            sv[2][t] = fmin(sv[2][t], sv[2][t + s]);
+            sv[3][t] = fmax(sv[3][t], sv[3][t + s]);
         }
         __syncthreads();
     }
-    if (t == 0) {
-        part[2048] = smin[0];
-        part[2049] = smax[0];
-    }
+    if (t < 4) part[4096 + t] = sv[t][0];
 }
 
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part, double &wmin,
-                   double &wmax, hipStream_t s)
+                   double &wmax, hipStream_t s, double *umin, double *umax)
 {
     if (nrow == 0) {
         wmin = wmax = 0.0;
+        if (umin) *umin = 0.0;
+        if (umax) *umax = 0.0;
         return CG_OK;
     }
     int nb = (int)std::min<int64_t>(1024, ceil_div(nrow, 256));
     hipLaunchKernelGGL(k_wminmax<TU>, dim3(nb), dim3(256), 0, s, uvw, nrow, scale_w, d_part);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1024), 0, s, d_part, nb);
     CG_HIP(hipGetLastError());
-    double h[2];
-    CG_HIP(hipMemcpyAsync(h, d_part + 2048, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    double h[4];
+    CG_HIP(hipMemcpyAsync(h, d_part + 4096, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
     wmin = h[0];
     wmax = h[1];
+    if (umin) *umin = h[2];
+    if (umax) *umax = h[3];
     return CG_OK;
 }
 
@@ -621,8 +632,8 @@ int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm
 }
 
 // explicit instantiations
-template int launch_wminmax<float>(const float *, int64_t, double, double *, double &, double &, hipStream_t);
-template int launch_wminmax<double>(const double *, int64_t, double, double *, double &, double &, hipStream_t);
+template int launch_wminmax<float>(const float *, int64_t, double, double *, double &, double &, hipStream_t, double *, double *);
+template int launch_wminmax<double>(const double *, int64_t, double, double *, double &, double &, hipStream_t, double *, double *);
 template int launch_keys<float>(const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys<double>(const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<float>(const float *, const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);

@@ -145,6 +145,12 @@ int cg_ctx_reset_times(void *ctx);
 /* parameters of the last call: nf1 nf2 nf3 kw beta es_c pix_x pix_y
  * i_center i_half_width o_center o_half_width gamma h tile_x tile_y */
 int cg_ctx_last_plan(void *ctx, double *out);
+/* row band of the last W-stacking call: out[0] = first grid row (nf2 axis)
+ * that points can reach, out[1] = number of rows, cyclic (0: no band, the
+ * column passes run as the four-step split).  With a band of at most 1024
+ * or 2048 rows the complex64 column passes run band-limited: the input is
+ * read and the spectrum written once, no intermediate in HBM. */
+int cg_ctx_last_band(void *ctx, int *out);
 
 /* prec 0: complex64 vis/image, FP32 gridding and FFT; prec 1: complex128.
  * uvw_prec, wgt_prec: 0 float, 1 double (prec 1 requires both 1).  Supported

@@ -195,6 +195,59 @@ def test_ms2dirty_record_paths_identical(gpu, nchan, uvw32):
     assert np.array_equal(out[1][1], out[0][1])
 
 
+@pytest.mark.parametrize("spread,nchan,npix,band_max", [(150.0, 1, 1024, 1024), (2500.0, 1, 1024, 1024),
+                                                       (250.0, 3, 1024, 1024), (9000.0, 1, 1024, 0),
+                                                       (3500.0, 1, 2048, 2048), (3000.0, 2, 2048, 2048)])
+def test_band_column_fft_matches_four_step(gpu, spread, nchan, npix, band_max):
+    """complex64 images with several w-planes: the band-limited column pass
+    (the rows points reach inside one band of at most 1024 rows, or 2048 when
+    nf2 >= 4096) against the four-step pass (option band_fft = 0), both
+    directions, and adjointness.  band_max 0: the band is too wide and both
+    runs take the four-step pass.  1200 x 1024 (nf2 not a power of two)
+    always takes the four-step pass."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(int(spread) + nchan + npix)
+    nrow, fov = 60000, 2.0 * npix / 1024
+    freqs = np.linspace(1.0e8, 1.15e8, nchan)
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = rng.normal(0, spread, (nrow, 2))
+    uvw[:, 2] = rng.uniform(-60, 60, nrow)
+    shape = (nrow,) if nchan == 1 else (nrow, nchan)
+    vis = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    wgt = rng.uniform(0.5, 1.5, shape).astype(np.float32)
+    tu = torch.tensor(uvw, device="cuda")
+    tv, tw = torch.from_numpy(vis).cuda(), torch.from_numpy(wgt).cuda()
+    f = freqs[0] if nchan == 1 else freqs
+    shapes = [(npix, npix)] + ([(1200, 1024)] if npix == 1024 else [])
+    for nx, ny in shapes:
+        img = (rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))).astype(np.complex64)
+        ti = torch.from_numpy(img).cuda()
+        out = {}
+        for band in (1, 0):
+            ctx = Context(band_fft=band)
+            try:
+                out[band] = (ctx.ms2dirty(tu, tv, tw, nx, ny, fov, f, 1e-5).cpu().numpy(),
+                             ctx.dirty2ms(tu, ti, tw, fov, f, 1e-5).cpu().numpy())
+                nf2, nf3 = ctx.last_plan()["nf2"], ctx.last_plan()["nf3"]
+                bs, blen = ctx.last_band()
+            finally:
+                ctx.close()
+            if band:
+                # the planned band holds every row the points reach
+                assert 0 < blen < nf2 and 0 <= bs < nf2
+                if nx == npix and band_max:
+                    assert blen <= band_max and nf2 >= 2 * band_max, (blen, nf2)
+                elif nx == npix:
+                    assert blen > 1024 and nf2 < 4096
+        assert nf3 > 1
+        assert rel_l2(out[1][0], out[0][0]) <= 1e-6
+        assert rel_l2(out[1][1], out[0][1]) <= 1e-6
+        lhs = np.vdot(out[1][1].astype(np.complex128).ravel(), vis.astype(np.complex128).ravel())
+        rhs = np.vdot(img.astype(np.complex128).ravel(), out[1][0].astype(np.complex128).ravel())
+        assert abs(lhs - rhs) <= 5e-5 * abs(lhs)
+
+
 def test_cfg1_plain_full_size(gpu):
     """The plain one-channel workload (131072 rows, 1024^2, fov 2 deg,
     freq 1e8, eps 1e-5, mixed precision) against the FP64 restatement on the
