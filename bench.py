@@ -209,9 +209,19 @@ def grid_stats(ctx, plan, kw, dense_min):
     row_t = rows.reshape(nty, ty).any(1)
     col_t = cols.reshape(ntx, tx).any(1)
     dense = int(cnt[cnt >= dense_min].sum()) if (dense_min > 0 and ctx.key_shift > 0) else 0
+    band_s, band_len = ctx.last_band()
     return {"rows": int(rows.sum()), "cols": int(cols.sum()),
             "tiles_written": int(row_t.sum() * col_t.sum()), "tile": tx * ty,
-            "nonempty_tiles": int(nz.sum()), "dense_points": dense}
+            "nonempty_tiles": int(nz.sum()), "dense_points": dense, "band": band_len}
+
+
+def band_kernel(nf2, band):
+    """True when the column passes of a complex64 call with this row band run
+    as the one-pass band kernel (M = 1024 or 2048 rows, nf2 >= 2 M)."""
+    if band <= 0 or nf2 & (nf2 - 1):
+        return False
+    m = 1024 if band <= 1024 else 2048
+    return band <= m and nf2 >= 2 * m
 
 
 def kernel_table(W, plan, fw, bw, stage_ms):
@@ -235,13 +245,23 @@ def kernel_table(W, plan, fw, bw, stage_ms):
         # 4 MFMA 32x32x2 (4096 flop each) per point pair
         rows.append(("k_grid_dense", "ms2dirty", 1, sm["grid_dense"], "mfma", 8192 * Md))
     rows.append(("k_fft_rows_ct", "ms2dirty", 1, sm["fft_rows"], "hbm", 8 * P * Rm * (Cm + N1)))
-    rows.append((col_kernel, "ms2dirty", 2, sm["fft_cols"], "hbm",
-                 (8 * P * (Rm * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + N2 * N1)) // 2))
+    if band_kernel(nf2, fw["band"]):
+        # one pass: the masked rows of the band in, the live spectrum out
+        rows.append(("k_fft_cols_band", "ms2dirty", 1, sm["fft_cols"], "hbm",
+                     8 * P * Rm * N1 + 8 * P * N2 * N1))
+    else:
+        rows.append((col_kernel, "ms2dirty", 2, sm["fft_cols"], "hbm",
+                     (8 * P * (Rm * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + N2 * N1)) // 2))
     rows.append(("k_ms2dirty_epi", "ms2dirty", 1, sm["image"], "hbm", 8 * P * N1 * N2 + 8 * N1 * N2))
     Rb, Cb = bw["rows"], bw["cols"]
     rows.append(("k_dirty2ms_pro", "dirty2ms", 1, sb["image"], "hbm", 8 * N1 * N2 + 8 * P * N2 * N1))
-    rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"], "hbm",
-                 (8 * P * (N2 * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + Rb * N1)) // 2))
+    if band_kernel(nf2, bw["band"]):
+        # one pass: the live spectrum in, the band's rows out
+        rows.append(("k_fft_cols_band", "dirty2ms", 1, sb["fft_cols"], "hbm",
+                     8 * P * N2 * N1 + 8 * P * bw["band"] * N1))
+    else:
+        rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"], "hbm",
+                     (8 * P * (N2 * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + Rb * N1)) // 2))
     rows.append(("k_fft_rows_ct", "dirty2ms", 1, sb["fft_rows"], "hbm", 8 * P * Rb * (N1 + Cb)))
     halo = bw["nonempty_tiles"] * (16 + kw - 1) ** 2 * P * 8
     rows.append(("k_degrid", "dirty2ms", 1, sb["grid"], "hbm", halo + 24 * M + 4 * M + 8 * M))

@@ -901,10 +901,17 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
     static_assert(M == 1024 || M == 2048, "band length");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     float2 *buf = reinterpret_cast<float2 *>(smem_raw);
+    const int nlive = live.count(n1);
+    // column block: workgroups w and w + 8 (the same XCD when workgroups are
+    // dealt to the 8 XCDs round-robin) take adjacent blocks, so the two
+    // halves of a 128-byte row segment (8 columns each at M = 2048) meet in
+    // one L2
+    const int w = blockIdx.x;
+    const int cb = M == 2048 ? 2 * (8 * (w >> 4) + (w & 7)) + ((w >> 3) & 1) : w;
+    if (cb * CB >= nlive) return;  // padding workgroups (uniform, before any barrier)
     const TwTab<T> tab = load_twtab<T>(buf + M * CB, tw2, n2, kBandTwSh);
     const int tid = threadIdx.x, b = tid & (CB - 1), e = tid >> LCB;
-    const int nlive = live.count(n1);
-    const int c = blockIdx.x * CB + b;
+    const int c = cb * CB + b;
     const bool cok = c < nlive;
     const int lc = cok ? live.at(c) : 0;
     float2 *pl = data + (size_t)blockIdx.y * n2 * n1;
@@ -1278,7 +1285,9 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         if (band_len > 0 && band_len <= M && lds <= (size_t)kLdsMax && band_s >= 0 && band_s < n2 &&
             fft_knob("CG_FFT_BAND", 1) && fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
             n2 >= 2 * M && fft_col_split(n2, na, nb)) {
-            const dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)batch);
+            int nblk = (nlive + CB - 1) / CB;
+            if (M == 2048) nblk = (nblk + 15) / 16 * 16;  // whole groups of the block remap
+            const dim3 grid((unsigned)nblk, (unsigned)batch);
             auto go = [&](const void *fn, auto kern) -> int {
                 CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 hipLaunchKernelGGL(kern, grid, dim3(1024), lds, s, (float2 *)data, n2, n1, n2 / M,
