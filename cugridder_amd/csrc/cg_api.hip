@@ -50,6 +50,7 @@ struct Ctx {
     int row_mask = 1;        // let the FFT skip grid rows no point reaches
     int prerec = 1;          // complex64 records in two coalescing passes
     int band_fft = 1;        // band-limited column passes when the rows points reach allow
+    int compact_keys = 1;    // sort keys number only the tiles the u / v ranges allow
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     int64_t last_m = 0;
     int last_cshift = 0;
@@ -122,6 +123,7 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_PREREC")) C.prerec = std::atoi(e);
     if (const char *e = std::getenv("CG_MFMA2D")) C.mfma2d = std::atoi(e);
     if (const char *e = std::getenv("CG_BAND_FFT")) C.band_fft = std::atoi(e);
+    if (const char *e = std::getenv("CG_COMPACT_KEYS")) C.compact_keys = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
 }
 
@@ -243,9 +245,27 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
                              as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles), as<int32_t>(C.hist),
                              as<int32_t>(C.scan), C.stream));
-    CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, as<int32_t>(C.tstart),
+    CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a), as<int32_t>(C.tstart),
                               C.stream));
     return CG_OK;
+}
+
+// compact tile numbering of the sort key when both bands are known, do not
+// wrap, and the rectangle of their tiles needs fewer key bits than the grid
+static void set_key_rect(CoordArgs &a, const WPlan &P, const Tiling &t, bool on)
+{
+    a.cby0 = a.cnby = a.cbx0 = a.cnbx = 0;
+    if (!on || P.band_len <= 0 || P.cband_len <= 0 || P.band_s + P.band_len > P.nf2 ||
+        P.cband_s + P.cband_len > P.nf1)
+        return;
+    const int by0 = P.band_s / t.ty, by1 = (P.band_s + P.band_len - 1) / t.ty;
+    const int bx0 = P.cband_s / t.tx, bx1 = (P.cband_s + P.cband_len - 1) / t.tx;
+    const int64_t n = (int64_t)(by1 - by0 + 1) * (bx1 - bx0 + 1);
+    if (n * 2 > t.ntiles()) return;  // saves less than a key bit
+    a.cby0 = by0;
+    a.cnby = by1 - by0 + 1;
+    a.cbx0 = bx0;
+    a.cnbx = bx1 - bx0 + 1;
 }
 
 static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, int cshift)
@@ -291,9 +311,9 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
     if (nrow * (int64_t)nchan > (int64_t)INT32_MAX - 1) return CG_ERR_SIZE;
     for (int c = 0; c < nchan; c++)
         if (!(freq[c] > 0)) return CG_ERR_ARG;
-    CG_TRY(ensure(C.part, sizeof(double) * 4104));
-    double wlo = 0, whi = 0, wmin = 0, wmax = 0, ulo = 0, uhi = 0;
-    CG_TRY(launch_wminmax<TU>(uvw, nrow, 1.0, as<double>(C.part), wlo, whi, C.stream, &ulo, &uhi));
+    CG_TRY(ensure(C.part, sizeof(double) * 6152));
+    double wlo = 0, whi = 0, wmin = 0, wmax = 0, uv[4] = {0, 0, 0, 0};
+    CG_TRY(launch_wminmax<TU>(uvw, nrow, 1.0, as<double>(C.part), wlo, whi, C.stream, uv));
     if (nrow > 0) {
         wmin = INFINITY;
         wmax = -INFINITY;
@@ -328,16 +348,18 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
         }
     }
     if (!fft_supported<T>(P.nf1) || !fft_supported<T>(P.nf2)) return CG_ERR_SIZE;
-    // grid rows the points can reach: from the u range over all channels
-    if (nrow > 0 && C.band_fft) {
+    // grid rows and columns the points can reach: from the u and v ranges
+    // over all channels
+    if (nrow > 0) {
         double smin = INFINITY, smax = 0;
         for (int c = 0; c < nchan; c++) {
             const double sc = 2.0 * kPi * (freq[c] / kC);
             smin = std::min(smin, sc);
             smax = std::max(smax, sc);
         }
-        const double xlo = ulo < 0 ? ulo * smax : ulo * smin, xhi = uhi > 0 ? uhi * smax : uhi * smin;
-        plan_row_band(P, xlo, xhi);
+        auto lo = [&](double x) { return x < 0 ? x * smax : x * smin; };
+        auto hi = [&](double x) { return x > 0 ? x * smax : x * smin; };
+        plan_bands(P, lo(uv[0]), hi(uv[1]), lo(uv[2]), hi(uv[3]));
     }
     return ier;
 }
@@ -395,6 +417,7 @@ static int plan_ms2dirty(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const do
     // gridding accumulates in FP64 (16 bytes per cell) whatever the precision
     cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, tn.grid_lds, tn.tile_grid, 0);
     cp.a = make_coord_args(P, cp.t);
+    set_key_rect(cp.a, P, cp.t, C.compact_keys != 0);
     CG_TRY(channel_scales(C, nchan, freq, cp.a));
     cp.M = nrow * nchan;
     // bins with many points per start cell go through the MFMA path, added
@@ -450,6 +473,7 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     const uint8_t *rm = nullptr;
     CG_TRY(row_mask(C, t, P.conv.kw, P.nf1, P.nf2, rm));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.kr = key_rect(a);
     g.dense_min = cp.dense_min;
     CG_TRY(grid_masks<T>(C, g, rm, P.nf1, P.nf2));
     CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
@@ -525,6 +549,7 @@ static int plan_dirty2ms(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const do
     const int kw = P.conv.kw;
     cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, kw, eb, tn.lds, tn.tile_degrid, kw - 1);
     cp.a = make_coord_args(P, cp.t);
+    set_key_rect(cp.a, P, cp.t, C.compact_keys != 0);
     CG_TRY(channel_scales(C, nchan, freq, cp.a));
     cp.M = nrow * nchan;
     cp.dense_min = 0;
@@ -588,6 +613,7 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_GRID));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.kr = key_rect(a);
     if (M > 0)
         CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                           as<int32_t>(C.vals), as<int32_t>(C.tstart),
@@ -942,7 +968,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
                                  as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles),
                                  as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream));
-        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift,
+        CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a),
                                   as<int32_t>(C.tstart), C.stream));
     } else {
         CG_TRY(launch_fill_i32(as<int32_t>(C.tstart), ntiles + 1, 0, C.stream));
@@ -965,6 +991,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     CG_TRY(launch_fseries_grid(qd, P.nf2, as<double>(C.phi2), C.stream));
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.kr = key_rect(a);
     g.mfma2d = C.mfma2d;
     // fast-axis columns that hold modes; grid rows that points reach; mode
     // frequencies along nf2
@@ -1299,6 +1326,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "prerec")) C->prerec = value;
     else if (!std::strcmp(name, "mfma2d")) C->mfma2d = value;
     else if (!std::strcmp(name, "band_fft")) C->band_fft = value;
+    else if (!std::strcmp(name, "compact_keys")) C->compact_keys = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;

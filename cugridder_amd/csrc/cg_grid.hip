@@ -606,7 +606,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__re
         for (int k = threadIdx.x; k < 2 * rcells; k += blockDim.x) are[k] = 0.0;
         // first sorted position of every start cell of the bin
         for (int c = threadIdx.x; c <= ncell; c += blockDim.x) {
-            const uint32_t key = ((uint32_t)tile << g.cshift) + (uint32_t)c;
+            const uint32_t key = ((uint32_t)g.kr.lower(tile, ntiles) << g.cshift) + (uint32_t)c;
             int lo = beg, hi = end;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;

@@ -20,7 +20,30 @@ struct CoordArgs {
     int cshift;        // key = tile << cshift | cell of the start inside the tile
     int pirange;       // 1: coordinates already in [-pi,pi) radians (2-D NUFFT)
     int has_w;         // 0: 2-D (no w axis)
+    // compact tile numbering (cnbx > 0): every point's tile lies in the
+    // rectangle of tile rows cby0 .. cby0 + cnby - 1 and tile columns cbx0 ..
+    // cbx0 + cnbx - 1 (no wrap), and the key numbers the rectangle's tiles
+    // row-major, which sorts exactly as the full numbering with fewer bits
+    int cby0, cnby, cbx0, cnbx;
 };
+// the key map's tile number of grid tile (bx, by), and of the first
+// rectangle tile at or after grid tile t in row-major order
+struct KeyRect {
+    int ntx, cby0, cnby, cbx0, cnbx;
+    __host__ __device__ int64_t lower(int64_t t, int64_t ntiles) const
+    {
+        if (!cnbx) return t;
+        if (t >= ntiles) return (int64_t)cnby * cnbx;
+        const int by = (int)(t / ntx), bx = (int)(t - (int64_t)by * ntx);
+        if (by < cby0) return 0;
+        if (by >= cby0 + cnby) return (int64_t)cnby * cnbx;
+        const int ry = by - cby0;
+        if (bx < cbx0) return (int64_t)ry * cnbx;
+        if (bx >= cbx0 + cnbx) return (int64_t)(ry + 1) * cnbx;
+        return (int64_t)ry * cnbx + (bx - cbx0);
+    }
+};
+inline KeyRect key_rect(const CoordArgs &a) { return KeyRect{a.ntx, a.cby0, a.cnby, a.cbx0, a.cnbx}; }
 
 CoordArgs make_coord_args(const WPlan &p, const Tiling &t);
 // fill the tiling fields of a; the start cell joins the sort key when the
@@ -30,13 +53,12 @@ void set_coord_tiling(CoordArgs &a, const Tiling &t);
 int sort_key_bits(const CoordArgs &a, int64_t ntiles);
 
 // ---- prep (cg_prep.hip)
-// min/max of w * scale_w (FP64) over nrow rows of uvw [nrow][3], and of u
-// when umin / umax are given; partial buffer needs 4100 doubles.  Results
-// copied to host (synchronises).
+// min/max of w * scale_w (FP64) over nrow rows of uvw [nrow][3], and
+// uv[4] = {umin, umax, vmin, vmax} when uv is given; partial buffer needs
+// 6150 doubles.  Results copied to host (synchronises).
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
-                   double &wmin, double &wmax, hipStream_t s, double *umin = nullptr,
-                   double *umax = nullptr);
+                   double &wmin, double &wmax, hipStream_t s, double *uv = nullptr);
 // keys[j] = (tile of the footprint start of row j) << cshift | start cell
 // inside the tile; vals[j] = j
 template <typename TU>
@@ -55,6 +77,7 @@ int launch_row_mask(const int32_t *tile_start, const Tiling &t, int kw, int nf1,
                     uint8_t *mask, hipStream_t s);
 // tile_start[t] = first sorted position with key >> shift >= t, t in [0, ntiles]
 int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
+                       const KeyRect &kr,
                        int32_t *tile_start, hipStream_t s);
 
 // Sorted point records: geo = {d1, d2, d3, packed(lx | ly<<8 | z0<<16)} and a
@@ -131,6 +154,7 @@ struct GridArgs {
     // values divided by e^beta back
     float f_nb, f_ec, f_e1, f_e2;
     double e3;
+    KeyRect kr;  // the sort key's tile numbering
     // 1: complex64 one-plane gridding may run as per-quadrant MFMA products
     // (grid2d_mfma_supported decides); 0: always the LDS-atomic kernel
     int mfma2d;

@@ -153,21 +153,26 @@ void fseries_grid(const ConvOpts &o, int nf, double *out)
     }
 }
 
-void plan_row_band(WPlan &p, double xlo, double xhi)
+// footprint cells of grid coordinates (x pix / 2 pi + 1/2) nf, x in [xlo, xhi]
+static void axis_band(int nf, double pix, int kw, double xlo, double xhi, int &s, int &len)
 {
-    p.band_s = p.band_len = 0;
-    const double a = xlo * p.pix_x, b = xhi * p.pix_x;
-    // grid coordinate (x / 2 pi + 1/2) nf2 of the shift-rescale, valid while
-    // x stays inside [-pi, pi) (no wrap)
+    s = len = 0;
+    const double a = xlo * pix, b = xhi * pix;
+    // valid while x pix stays inside [-pi, pi) (the shift-rescale's wrap)
     if (!(a <= b) || !(a > -PI + 1e-9) || !(b < PI - 1e-9)) return;
-    const double glo = (a * (0.5 / PI) + 0.5) * p.nf2, ghi = (b * (0.5 / PI) + 0.5) * p.nf2;
-    const int kw = p.conv.kw;
+    const double glo = (a * (0.5 / PI) + 0.5) * nf, ghi = (b * (0.5 / PI) + 0.5) * nf;
     const int64_t y0 = (int64_t)std::ceil(glo - 0.5 * kw) - 1;
-    const int64_t y1 = (int64_t)std::ceil(ghi - 0.5 * kw) + kw;  // one past the last row, + 1 margin
-    const int64_t len = y1 - y0 + 1;
-    if (len <= 0 || len >= p.nf2) return;
-    p.band_s = (int)(((y0 % p.nf2) + p.nf2) % p.nf2);
-    p.band_len = (int)len;
+    const int64_t y1 = (int64_t)std::ceil(ghi - 0.5 * kw) + kw;  // one past the last cell, + 1 margin
+    const int64_t n = y1 - y0 + 1;
+    if (n <= 0 || n >= nf) return;
+    s = (int)(((y0 % nf) + nf) % nf);
+    len = (int)n;
+}
+
+void plan_bands(WPlan &p, double ulo, double uhi, double vlo, double vhi)
+{
+    axis_band(p.nf2, p.pix_x, p.conv.kw, ulo, uhi, p.band_s, p.band_len);
+    axis_band(p.nf1, p.pix_y, p.conv.kw, vlo, vhi, p.cband_s, p.cband_len);
 }
 
 Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,

@@ -66,16 +66,18 @@ struct WPlan {
     int ier = 0;
     Tiling tile;
     // grid rows (nf2 axis) the points' footprints can reach: band_len rows
-    // from band_s, cyclic (band_len = 0: unknown or the whole axis)
-    int band_s = 0, band_len = 0;
+    // from band_s, cyclic (band_len = 0: unknown or the whole axis); the
+    // same for the grid columns (nf1 axis)
+    int band_s = 0, band_len = 0, cband_s = 0, cband_len = 0;
     int64_t grid_cells() const { return (int64_t)nf1 * nf2 * nf3; }
 };
 
-// Sets p.band_s / p.band_len from the range [xlo, xhi] of u * 2 pi f / c
-// (radians per unit pixel angle, before multiplying by pix_x) over all
-// points: the footprint rows of the lowest and highest grid coordinate,
-// plus one row of margin on each side.  No band when the range wraps.
-void plan_row_band(WPlan &p, double xlo, double xhi);
+// Sets p.band_s / p.band_len from the range [ulo, uhi] of u * 2 pi f / c
+// over all points (before multiplying by pix_x), and p.cband_s / p.cband_len
+// from that of v (pix_y, nf1): the footprint rows (columns) of the lowest
+// and highest grid coordinate, plus one of margin on each side.  No band on
+// an axis whose range wraps.
+void plan_bands(WPlan &p, double ulo, double uhi, double vlo, double vhi);
 
 // Fills every field from the call parameters and the w range (min/max of
 // the scaled w = w*f/c*sign*2pi over all rows).  Returns ier (>=2 is fatal).

@@ -56,7 +56,10 @@ void set_coord_tiling(CoordArgs &a, const Tiling &t)
     a.cshift = bits_for(t.ntiles()) + cs <= 32 ? cs : 0;
 }
 
-int sort_key_bits(const CoordArgs &a, int64_t ntiles) { return bits_for(ntiles) + a.cshift; }
+int sort_key_bits(const CoordArgs &a, int64_t ntiles)
+{
+    return bits_for(a.cnbx ? (int64_t)a.cnby * a.cnbx : ntiles) + a.cshift;
+}
 
 // include/conv.h:24 SHIFT_RESCALE (p = pirange)
 __device__ __forceinline__ double shift_rescale(double x, int N, int pirange)
@@ -122,88 +125,87 @@ __device__ __forceinline__ Foot make_foot(double u, double v, double w, const Co
 
 __device__ __forceinline__ uint32_t tile_key(const Foot &f, const CoordArgs &a)
 {
-    const uint32_t tile = (uint32_t)((f.y0 / a.ty) * a.ntx + (f.x0 / a.tx));
+    const int by = f.y0 / a.ty, bx = f.x0 / a.tx;
+    const uint32_t tile = a.cnbx ? (uint32_t)((by - a.cby0) * a.cnbx + (bx - a.cbx0))
+                                 : (uint32_t)(by * a.ntx + bx);
     if (!a.cshift) return tile;
     return (tile << a.cshift) | (uint32_t)((f.y0 % a.ty) * a.tx + f.x0 % a.tx);
 }
 
 // ------------------------------------------------------------ w min/max
 
-// min/max of w * scale_w and of u over the rows: part[0, 1024) / [1024,
-// 2048) / [2048, 3072) / [3072, 4096) per block, finals at part[4096..4099]
+// min/max of w * scale_w, of u and of v over the rows: part[1024 q + block]
+// for q = wmin, wmax, umin, umax, vmin, vmax; finals at part[6144 + q]
 template <typename TU>
 __global__ __launch_bounds__(256) void k_wminmax(const TU *uvw, int64_t nrow, double scale_w,
                                                  double *part)
 {
-    __shared__ double sv[4][256];
-    double mn = INFINITY, mx = -INFINITY, un = INFINITY, ux = -INFINITY;
+    __shared__ double sv[6][256];
+    double r[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
     for (int64_t j = blockIdx.x * (int64_t)256 + threadIdx.x; j < nrow; j += (int64_t)gridDim.x * 256) {
-        double w = (double)uvw[3 * j + 2] * scale_w;
-        double u = (double)uvw[3 * j];
-        mn = fmin(mn, w);
-        mx = fmax(mx, w);
-        un = fmin(un, u);
-        ux = fmax(ux, u);
+        const double w = (double)uvw[3 * j + 2] * scale_w;
+        const double u = (double)uvw[3 * j], v = (double)uvw[3 * j + 1];
+        r[0] = fmin(r[0], w);
+        r[1] = fmax(r[1], w);
+        r[2] = fmin(r[2], u);
+        r[3] = fmax(r[3], u);
+        r[4] = fmin(r[4], v);
+        r[5] = fmax(r[5], v);
     }
-    sv[0][threadIdx.x] = mn;
-    sv[1][threadIdx.x] = mx;
-    sv[2][threadIdx.x] = un;
-    sv[3][threadIdx.x] = ux;
+#pragma unroll
+    for (int q = 0; q < 6; q++) sv[q][threadIdx.x] = r[q];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            sv[0][threadIdx.x] = fmin(sv[0][threadIdx.x], sv[0][threadIdx.x + s]);
-            sv[1][threadIdx.x] = fmax(sv[1][threadIdx.x], sv[1][threadIdx.x + s]);
-            sv[2][threadIdx.x] = fmin(sv[2][threadIdx.x], sv[2][threadIdx.x + s]);
-            sv[3][threadIdx.x] = fmax(sv[3][threadIdx.x], sv[3][threadIdx.x + s]);
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                sv[q][threadIdx.x] = (q & 1) ? fmax(sv[q][threadIdx.x], sv[q][threadIdx.x + s])
+                                             : fmin(sv[q][threadIdx.x], sv[q][threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x < 4) part[1024 * threadIdx.x + blockIdx.x] = sv[threadIdx.x][0];
+    if (threadIdx.x < 6) part[1024 * threadIdx.x + blockIdx.x] = sv[threadIdx.x][0];
 }
 
 __global__ __launch_bounds__(1024) void k_minmax_final(double *part, int nb)
 {
-    __shared__ double sv[4][1024];
+    __shared__ double sv[6][1024];
     int t = threadIdx.x;
-    sv[0][t] = t < nb ? part[t] : INFINITY;
-    sv[1][t] = t < nb ? part[1024 + t] : -INFINITY;
-    sv[2][t] = t < nb ? part[2048 + t] : INFINITY;
-    sv[3][t] = t < nb ? part[3072 + t] : -INFINITY;
+#pragma unroll
+    for (int q = 0; q < 6; q++) sv[q][t] = t < nb ? part[1024 * q + t] : ((q & 1) ? -INFINITY : INFINITY);
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
         if (t < s) {
-            sv[0][t] = fmin(sv[0][t], sv[0][t + s]);
-            sv[1][t] = fmax(sv[1][t], sv[1][t + s]);
-            sv[2][t] = fmin(sv[2][t], sv[2][t + s]);
-            sv[3][t] = fmax(sv[3][t], sv[3][t + s]);
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                sv[q][t] = (q & 1) ? fmax(sv[q][t], sv[q][t + s]) : fmin(sv[q][t], sv[q][t + s]);
         }
         __syncthreads();
     }
-    if (t < 4) part[4096 + t] = sv[t][0];
+    if (t < 6) part[6144 + t] = sv[t][0];
 }
 
 template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part, double &wmin,
-                   double &wmax, hipStream_t s, double *umin, double *umax)
+                   double &wmax, hipStream_t s, double *uv)
 {
     if (nrow == 0) {
         wmin = wmax = 0.0;
-        if (umin) *umin = 0.0;
-        if (umax) *umax = 0.0;
+        if (uv) uv[0] = uv[1] = uv[2] = uv[3] = 0.0;
         return CG_OK;
     }
     int nb = (int)std::min<int64_t>(1024, ceil_div(nrow, 256));
     hipLaunchKernelGGL(k_wminmax<TU>, dim3(nb), dim3(256), 0, s, uvw, nrow, scale_w, d_part);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(1024), 0, s, d_part, nb);
     CG_HIP(hipGetLastError());
-    double h[4];
-    CG_HIP(hipMemcpyAsync(h, d_part + 4096, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    double h[6];
+    CG_HIP(hipMemcpyAsync(h, d_part + 6144, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
     CG_HIP(hipStreamSynchronize(s));
     wmin = h[0];
     wmax = h[1];
-    if (umin) *umin = h[2];
-    if (umax) *umax = h[3];
+    if (uv) {
+        for (int q = 0; q < 4; q++) uv[q] = h[2 + q];
+    }
     return CG_OK;
 }
 
@@ -270,25 +272,29 @@ int launch_keys_xy(const TU *x, const TU *y, int64_t m, const CoordArgs &a, uint
 
 // ------------------------------------------------------------ tile bounds
 
+// start[t] = first sorted position whose tile comes at or after grid tile t
+// (with a compact key map: at or after the first rectangle tile >= t)
 __global__ __launch_bounds__(256) void k_tile_bounds(const uint32_t *keys, int64_t m,
-                                                     int64_t ntiles, int shift, int32_t *start)
+                                                     int64_t ntiles, int shift, KeyRect kr,
+                                                     int32_t *start)
 {
     int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t > ntiles) return;
-    int64_t lo = 0, hi = m;  // first index with keys[i] >> shift >= t
+    const int64_t c = kr.lower(t, ntiles);
+    int64_t lo = 0, hi = m;  // first index with keys[i] >> shift >= c
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)(keys[mid] >> shift) < t) lo = mid + 1;
+        if ((int64_t)(keys[mid] >> shift) < c) lo = mid + 1;
         else hi = mid;
     }
     start[t] = (int32_t)lo;
 }
 
 int launch_tile_bounds(const uint32_t *sorted_keys, int64_t m, int64_t ntiles, int shift,
-                       int32_t *tile_start, hipStream_t s)
+                       const KeyRect &kr, int32_t *tile_start, hipStream_t s)
 {
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div(ntiles + 1, 256)), dim3(256), 0, s,
-                       sorted_keys, m, ntiles, shift, tile_start);
+                       sorted_keys, m, ntiles, shift, kr, tile_start);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
@@ -632,8 +638,8 @@ int launch_records_xy(const TU *x, const TU *y, const TV *c, const int32_t *perm
 }
 
 // explicit instantiations
-template int launch_wminmax<float>(const float *, int64_t, double, double *, double &, double &, hipStream_t, double *, double *);
-template int launch_wminmax<double>(const double *, int64_t, double, double *, double &, double &, hipStream_t, double *, double *);
+template int launch_wminmax<float>(const float *, int64_t, double, double *, double &, double &, hipStream_t, double *);
+template int launch_wminmax<double>(const double *, int64_t, double, double *, double &, double &, hipStream_t, double *);
 template int launch_keys<float>(const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys<double>(const double *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);
 template int launch_keys_xy<float>(const float *, const float *, int64_t, const CoordArgs &, uint32_t *, int32_t *, hipStream_t);

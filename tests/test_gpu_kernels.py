@@ -63,9 +63,9 @@ def test_tile_keys_and_permutation_bit_exact(gpu, nrow, nx, ny, fov, eps, freq):
     uvw[:, :2] = rng.normal(0, 1500, (nrow, 2))
     uvw[:, 2] = rng.uniform(-300, 300, nrow)
     vis = np.ones(nrow, np.complex64)
-    ctx = Context(stop_after=1)
-    ctx.ms2dirty(torch.tensor(uvw, device="cuda"), torch.tensor(vis, device="cuda"), None, nx, ny,
-                 fov, freq, eps)
+    tu, tv = torch.tensor(uvw, device="cuda"), torch.tensor(vis, device="cuda")
+    ctx = Context(stop_after=1, compact_keys=0)
+    ctx.ms2dirty(tu, tv, None, nx, ny, fov, freq, eps)
     p = ctx.last_plan()
     keys, perm, ts = ctx.sorted_view()
     cs = ctx.key_shift
@@ -77,6 +77,14 @@ def test_tile_keys_and_permutation_bit_exact(gpu, nrow, nx, ny, fov, eps, freq):
     assert np.array_equal(keys.cpu().numpy(), hk[ref])
     ntiles = (p["nf1"] // p["tile_x"]) * (p["nf2"] // p["tile_y"])
     assert np.array_equal(ts.cpu().numpy(), np.searchsorted(hk[ref] >> cs, np.arange(ntiles + 1)))
+    # compact keys (tiles of the u / v bands numbered alone): the same
+    # permutation and tile starts, with keys of fewer bits
+    cc = Context(stop_after=1)
+    cc.ms2dirty(tu, tv, None, nx, ny, fov, freq, eps)
+    k2, perm2, ts2 = cc.sorted_view()
+    assert np.array_equal(perm2.cpu().numpy(), ref)
+    assert np.array_equal(ts2.cpu().numpy(), ts.cpu().numpy())
+    assert int(k2.max()) <= int(keys.max())
 
 
 @pytest.mark.parametrize("shape", [(1, 8, 8), (2, 30, 500), (3, 64, 2048), (1, 96, 16384),
