@@ -134,6 +134,10 @@ int cg_ctx_destroy(void *ctx);
  * skips grid rows that no point reaches; 0: transforms every row),
  * "prerec" (1: complex64 records are built in the caller's order and
  * gathered as aligned slots; 0: one gather pass; results are identical),
+ * "band_fft" (1: complex64 column passes run band-limited when the grid
+ * rows points reach fit in 1024 / 2048 rows, see cg_ctx_last_band),
+ * "compact_keys" (1: sort keys number only the tiles inside the u / v
+ * bands: the same permutation with fewer radix passes),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
 /* per-stage times summed over calls, out[15] = {ms2dirty[7], dirty2ms[7],
