@@ -889,6 +889,45 @@ __device__ __forceinline__ void band_pass(float2 *buf, const TwTab<float> &tab, 
     lds_barrier();
 }
 
+// the last Stockham pass (radix R, stage PP = M / R), handing each output
+// to out(slot, element index, value) instead of writing it back to LDS.
+// Output r of the thread's k-th butterfly is element i + PP r (i its
+// butterfly index) and slot k R + r; the column of a thread's butterflies is
+// its own (1024 is a multiple of CB).  The caller may rewrite buf after this
+// returns (every read precedes its barrier).
+template <int R, int PP, int DIR, int M, int CB, typename F>
+__device__ __forceinline__ void band_pass_out(const float2 *buf, const TwTab<float> &tab, int tws,
+                                              F &&out)
+{
+    constexpr int Tn = M / R, K = CB * Tn / 1024, LCB = CB == 16 ? 4 : 3;
+    static_assert(K >= 1 && CB * Tn % 1024 == 0 && (1 << LCB) == CB, "shape");
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    float2 u[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = tid + 1024 * k, b = q & (CB - 1), i = q >> LCB;
+#pragma unroll
+        for (int r = 0; r < R; r++) u[k][r] = buf[(i + r * Tn) * CB + b];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = tid + 1024 * k, i = q >> LCB;
+        const int kk = i % PP;
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = tab.get(r * kk * (M / (PP * R)) * tws);
+            if (DIR < 0) w.y = -w.y;
+            u[k][r] = cmul<float>(u[k][r], w);
+        }
+        dft_small<float, R, DIR>(u[k]);
+        const int j = (i - kk) * R + kk;
+#pragma unroll
+        for (int r = 0; r < R; r++) out(k * R + r, j + r * PP, u[k][r]);
+    }
+}
+
 template <int DIR, int M>
 __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, int n1, int np,
                                                         int bs, int blen, Live live,
@@ -963,30 +1002,41 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
         for (int r = 0; r < 16; r++) buf[(16 * eo + r) * CB + b] = v[r];
         lds_barrier();
         band_pass<16, 16, DIR, M, CB>(buf, tab, tws);
-        if constexpr (M == 1024) band_pass<4, 256, DIR, M, CB>(buf, tab, tws);
-        else band_pass<8, 256, DIR, M, CB>(buf, tab, tws);
-        if (DIR > 0) {
-#pragma unroll
-            for (int m = 0; m < 16; m++) {
-                const int kb = eo + S * m, k = ka + np * kb;
-                if (cok && !dead(k)) pl[(size_t)srow(k) * n1 + lc] = buf[kb * CB + b];
-            }
+        if constexpr (DIR > 0) {
+            // the last pass stores its live outputs straight to the spectrum
+            auto store = [&](int, int kb, float2 val) {
+                const int k = ka + np * kb;
+                if (cok && !dead(k)) pl[(size_t)srow(k) * n1 + lc] = val;
+            };
+            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, tab, tws, store);
+            else band_pass_out<8, 256, DIR, M, CB>(buf, tab, tws, store);
+            continue;  // buf was read before the pass's barrier
         } else {
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int r = eo + S * j;
-                const float2 z = cmul<T>(buf[r * CB + b], twid(bs + ((r - bs) & (M - 1)), ka));
-                xb[j].x += z.x;
-                xb[j].y += z.y;
-            }
+            // the last pass's outputs (residue i + 256 r of butterfly i, slot
+            // k R + r) are accumulated where they are computed: this thread
+            // owns the band rows of those residues
+            auto acc = [&](int slot, int rr, float2 val) {
+                const float2 z = cmul<T>(val, twid(bs + ((rr - bs) & (M - 1)), ka));
+                xb[slot].x += z.x;
+                xb[slot].y += z.y;
+            };
+            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, tab, tws, acc);
+            else band_pass_out<8, 256, DIR, M, CB>(buf, tab, tws, acc);
         }
-        lds_barrier();
     }
-    if (DIR < 0 && cok) {
+    if constexpr (DIR < 0) {
+        // rows of the residues owned in the last pass: i + 256 r for the
+        // thread's butterflies i = (tid + 1024 k) / CB
+        constexpr int R = M == 1024 ? 4 : 8, K = 16 / R;
+        if (cok) {
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int o = (e + S * j - bs) & (M - 1);
-            if (o < blen) pl[(size_t)((bs + o) & mask) * n1 + lc] = xb[j];
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int rr = ((tid + 1024 * k) >> LCB) + 256 * r;
+                    const int o = (rr - bs) & (M - 1);
+                    if (o < blen) pl[(size_t)((bs + o) & mask) * n1 + lc] = xb[k * R + r];
+                }
         }
     }
 }
