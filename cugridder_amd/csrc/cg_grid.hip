@@ -116,9 +116,10 @@ struct KernelEval<float> : KernelEvalN {
 
 // gridding accumulates in FP64 whatever the grid precision: 16 bytes per cell
 int grid_lds_bytes(const GridArgs &g, int) { return g.pc * g.tx * g.ty * 16; }
+__host__ __device__ inline int degrid_row_stride(int tx, int kw) { return (tx + kw - 1) | 1; }
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem)
 {
-    return g.pc * (g.tx + kw - 1) * (g.ty + kw - 1) * elem;
+    return g.pc * degrid_row_stride(g.tx, kw) * (g.ty + kw - 1) * elem;
 }
 
 // ------------------------------------------------------------ gridding
@@ -813,6 +814,9 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
     if (beg == end) return;
     const int bx = tile % g.ntx, by = tile / g.ntx;
     const int hx = g.tx + KW - 1, hy = g.ty + KW - 1;
+    // LDS row stride: odd, so that cells a power-of-two number of rows or
+    // columns apart do not share banks
+    const int hxs = degrid_row_stride(g.tx, KW);
     const int x0 = bx * g.tx, y0 = by * g.ty;
     const KernelEval<T> ker(g, KW);
     const bool single_stage = g.nchunk == 1;
@@ -829,7 +833,7 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
             if (gx >= g.nf1) gx -= g.nf1;
             int gy = y0 + y;
             if (gy >= g.nf2) gy -= g.nf2;
-            lds[k] = fw[((size_t)(p0 + lz) * g.nf2 + gy) * g.nf1 + gx];
+            lds[(lz * hy + y) * hxs + x] = fw[((size_t)(p0 + lz) * g.nf2 + gy) * g.nf1 + gx];
         }
     };
 
@@ -872,14 +876,14 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
                     if (zz >= g.nf3) zz -= g.nf3;
                     if (zz >= np) continue;
                     const T kzv = is2d ? (T)1 : ker(G.z, tz);
-                    const cx_t<T> *pl = lds + (size_t)zz * hy * hx + (size_t)ly * hx + lx;
+                    const cx_t<T> *pl = lds + ((size_t)zz * hy + ly) * hxs + lx;
                     T sr = 0, si = 0;
 #pragma unroll
                     for (int t2 = 0; t2 < KW; t2++) {
                         T rr = 0, ri = 0;
 #pragma unroll
                         for (int t1 = 0; t1 < KW; t1++) {
-                            const cx_t<T> f = pl[t2 * hx + t1];
+                            const cx_t<T> f = pl[t2 * hxs + t1];
                             rr += f.x * kx[t1];
                             ri += f.y * kx[t1];
                         }

@@ -188,9 +188,13 @@ Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,
     Tiling T;
     T.tx = pick(nf1);
     T.ty = pick(nf2);
+    // a staged plane with a halo (degridding) has an odd row stride in LDS
+    auto plane_bytes = [&]() {
+        const int64_t w = halo ? ((T.tx + halo) | 1) : T.tx;
+        return w * (T.ty + halo) * elem_bytes;
+    };
     // shrink the tile while even one plane does not fit the LDS budget
-    while ((int64_t)(T.tx + halo) * (T.ty + halo) * elem_bytes > lds_budget &&
-           (T.tx > 2 || T.ty > 2)) {
+    while (plane_bytes() > lds_budget && (T.tx > 2 || T.ty > 2)) {
         if (T.tx >= T.ty && T.tx > 2) T.tx >>= 1;
         else T.ty >>= 1;
     }
@@ -201,7 +205,7 @@ Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,
     T.sby = 1 + (kw - 1 + T.ty - 1) / T.ty;
     T.sbx = std::min(T.sbx, T.ntx);
     T.sby = std::min(T.sby, T.nty);
-    int64_t per_plane = (int64_t)(T.tx + halo) * (T.ty + halo) * elem_bytes;
+    int64_t per_plane = plane_bytes();
     int pc = (int)std::max<int64_t>(1, lds_budget / per_plane);
     pc = std::min(pc, nplanes);
     T.nchunk = (nplanes + pc - 1) / pc;
