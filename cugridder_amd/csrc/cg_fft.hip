@@ -496,8 +496,12 @@ template <typename T>
 constexpr int ct_rmax() { return sizeof(T) == 4 ? 16 : 8; }
 
 // rows of N = 2^LOGN points, one row per workgroup, N / EPT threads,
-// persistent over the nrows rows of all planes
-template <typename T, int DIR, int LOGN>
+// persistent over the nrows rows of all planes.  Rows no point reaches are
+// skipped 64 at a time: each lane of a wave reads the mask byte of one of
+// the workgroup's next 64 rows and a ballot names the first live one (every
+// wave finds the same row).  PF: the next live row is loaded into registers
+// while the current one is transformed.
+template <typename T, int DIR, int LOGN, bool PF>
 __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t<T> *data, int64_t nrows,
                                                       const cx_t<T> *__restrict__ tw, Live live,
                                                       const uint8_t *__restrict__ rowmask, int n2)
@@ -522,16 +526,28 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
         for (int u = 0; u < EPT; u++)
             if (rowmask[n2 + threadIdx.x + u * NT]) cbits |= 1u << u;
     }
-    for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
-        // a masked row is zero (dir > 0) or unread (dir < 0): skip it
-        if (rowmask && !rowmask[g % n2]) continue;
+    // live bits of the workgroup's 64 rows g + j gridDim.x, j < 64: each
+    // lane reads one mask byte and a ballot gathers them (a masked row is
+    // zero for dir > 0 and unread for dir < 0).  Rows are numbered in 32
+    // bits (the host launches this kernel for fewer than 2^30 rows); the lane
+    // and n2 are made opaque so that no per-lane value (the lane offset, the
+    // reciprocal of n2) is hoisted and kept live across the transform.
+    auto probe = [&](int64_t g) -> uint64_t {
+        int lane = threadIdx.x & 63, m = n2;
+        asm volatile("" : "+v"(lane), "+v"(m));
+        const int c = (int)g + lane * (int)gridDim.x;
+        return __ballot(c < (int)nrows && rowmask[(unsigned)c % (unsigned)m] != 0);
+    };
+    auto load = [&](int64_t g, cx_t<T> *dst, bool to_lds) {
         const cx_t<T> *src = data + g * (int64_t)N;
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
             const bool ld = DIR > 0 ? ((cbits >> u) & 1u) != 0 : live(k);
-            buf[k + (k >> 5)] = ld ? src[k] : mkc((T)0, (T)0);
+            dst[to_lds ? k + (k >> 5) : u] = ld ? src[k] : mkc((T)0, (T)0);
         }
+    };
+    auto transform_store = [&](int64_t g) {
         lds_barrier();
         ct_fft<T, DIR, N, 1, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
         cx_t<T> *dst = data + g * (int64_t)N;
@@ -542,6 +558,37 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
             if (st) dst[k] = buf[k + (k >> 5)];
         }
         lds_barrier();
+    };
+    if constexpr (PF) {
+        // the first live row at or after g in steps of gridDim.x (nrows if none)
+        auto next_live = [&](int64_t g) -> int64_t {
+            if (!rowmask) return g;
+            for (; g < nrows; g += 64 * (int64_t)gridDim.x) {
+                const uint64_t on = probe(g);
+                if (on) return g + (int64_t)__builtin_ctzll(on) * gridDim.x;
+            }
+            return nrows;
+        };
+        cx_t<T> pre[EPT];
+        int64_t g = next_live(blockIdx.x);
+        if (g < nrows) load(g, pre, false);
+        while (g < nrows) {
+#pragma unroll
+            for (int u = 0; u < EPT; u++) {
+                const int k = threadIdx.x + u * NT;
+                buf[k + (k >> 5)] = pre[u];
+            }
+            const int64_t gn = next_live(g + gridDim.x);
+            if (gn < nrows) load(gn, pre, false);
+            transform_store(g);
+            g = gn;
+        }
+    } else {
+        for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
+            if (rowmask && !rowmask[g % n2]) continue;
+            load(g, buf, true);
+            transform_store(g);
+        }
     }
 }
 
@@ -1132,6 +1179,23 @@ static int log2_exact(int n)
     return ilog2(n);
 }
 
+template <typename T, int DIR, int LOGN, bool PF>
+static int rows_ct_launch(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live,
+                          const uint8_t *rowmask, int n2, size_t lds, hipStream_t s)
+{
+    constexpr int NT = (1 << LOGN) / fft_ept<T>();
+    if (lds > 65536)
+        CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows_ct<T, DIR, LOGN, PF>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = std::max<int>(1, (int)(kLdsMax / lds));
+    per_cu = std::min(per_cu, std::max(1, 2048 / NT));
+    const int64_t grid = std::min<int64_t>(nrows, (int64_t)256 * per_cu);
+    hipLaunchKernelGGL((k_fft_rows_ct<T, DIR, LOGN, PF>), dim3((unsigned)grid), dim3(NT), lds, s, data,
+                       nrows, tw, live, rowmask, n2);
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
 // compile-time row kernel for n1 = 2^LOGN (returns 1 if LOGN is not covered)
 template <typename T, int DIR, int LOGN>
 static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live,
@@ -1145,16 +1209,12 @@ static int rows_ct(cx_t<T> *data, int64_t nrows, const cx_t<T> *tw, Live live,
         constexpr int SH = LOGN / 2;
         const size_t lds = (size_t)(N + (N >> 5) + (1 << SH) + (N >> SH)) * sizeof(cx_t<T>);
         if (lds > (size_t)kLdsMax) return 1;
-        if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows_ct<T, DIR, LOGN>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        int per_cu = std::max<int>(1, (int)(kLdsMax / lds));
-        per_cu = std::min(per_cu, std::max(1, 2048 / NT));
-        const int64_t grid = std::min<int64_t>(nrows, (int64_t)256 * per_cu);
-        hipLaunchKernelGGL((k_fft_rows_ct<T, DIR, LOGN>), dim3((unsigned)grid), dim3(NT), lds, s, data,
-                           nrows, tw, live, rowmask, n2);
-        CG_HIP(hipGetLastError());
-        return CG_OK;
+        // the prefetched row takes 2 EPT more VGPRs per thread: only where
+        // the workgroup is at most 512 threads (up to 256 VGPRs each)
+        constexpr bool pf_fits = NT <= 512;
+        if (pf_fits && fft_knob("CG_FFT_ROWPF", 1) && nrows < (int64_t)1 << 30)
+            return rows_ct_launch<T, DIR, LOGN, pf_fits>(data, nrows, tw, live, rowmask, n2, lds, s);
+        return rows_ct_launch<T, DIR, LOGN, false>(data, nrows, tw, live, rowmask, n2, lds, s);
     }
 }
 
