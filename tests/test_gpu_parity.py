@@ -163,6 +163,57 @@ def test_nufft2d_tiles_and_records(gpu, tile, N1, N2, eps):
     assert rel_l2(out[1][1], ref2) <= max(eps, _floor(True))
 
 
+@pytest.mark.parametrize("npix", [512, 1024, 2048])
+@pytest.mark.parametrize("double", [False, True])
+def test_row_fft_prefetch_bit_identical(gpu, monkeypatch, npix, double):
+    """The row FFT that loads the next live row while transforming the
+    current one (rows of at most 512 threads: 512- to 4096-point grids here)
+    gives the same ms2dirty image, dirty2ms visibilities and 2-D NUFFT
+    outputs as the one-row-at-a-time kernel (CG_FFT_ROWPF=0), with the row
+    mask skipping rows no point reaches."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(90 + npix)
+    nrow, fov = 20000, 2.0
+    uvw = np.empty((nrow, 3))
+    uvw[:, 0] = rng.normal(0, 2000, nrow)
+    uvw[:, 1] = rng.normal(0, 300, nrow)  # few grid rows are reached
+    uvw[:, 2] = rng.uniform(-50, 50, nrow)
+    cdt = np.complex128 if double else np.complex64
+    vis = (rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)).astype(cdt)
+    dirty = (rng.standard_normal((npix, npix)) + 1j * rng.standard_normal((npix, npix))).astype(cdt)
+    x, y = _clustered_xy(rng, nrow)
+    fk = (rng.standard_normal((npix // 2, npix // 2)) + 1j * rng.standard_normal((npix // 2, npix // 2))).astype(cdt)
+    c = vis
+    if double:
+        x, y = x.astype(np.float64), y.astype(np.float64)
+    tu, tv, td = (torch.from_numpy(a).cuda() for a in (uvw, vis, dirty))
+    tx, ty, tc, tf = (torch.from_numpy(a).cuda() for a in (x, y, c, fk))
+    eps = 1e-9 if double else 1e-5
+    out = {}
+    for pf in ("1", "0"):
+        monkeypatch.setenv("CG_FFT_ROWPF", pf)
+        ctx = Context()
+        try:
+            out[pf] = (ctx.ms2dirty(tu, tv, None, npix, npix, fov, 1.0e9, eps).cpu().numpy(),
+                       ctx.dirty2ms(tu, td, None, fov, 1.0e9, eps).cpu().numpy(),
+                       ctx.nufft2d1(tx, ty, tc, npix // 2, npix // 2, eps).cpu().numpy(),
+                       ctx.nufft2d2(tx, ty, tf, eps).cpu().numpy())
+        finally:
+            ctx.close()
+    # degridding (dirty2ms, type 2) has no atomics: the same bits.  FP64
+    # gridding sums in LDS atomics whose order varies from run to run, so the
+    # FP64 image and type-1 output agree to rounding (a wrongly placed row
+    # would differ at O(1)); FP32 gridding (MFMA, or FP64 sums rounded to
+    # FP32) gives the same bits.
+    for i, (a, b) in enumerate(zip(out["1"], out["0"])):
+        if double and i in (0, 2):
+            assert rel_l2(a, b) <= 1e-13
+        else:
+            assert np.array_equal(a, b)
+    assert np.abs(out["1"][0]).max() > 0 and np.abs(out["1"][1]).max() > 0
+
+
 @pytest.mark.parametrize("nchan", [1, 3])
 @pytest.mark.parametrize("uvw32", [False, True])
 def test_ms2dirty_record_paths_identical(gpu, nchan, uvw32):
