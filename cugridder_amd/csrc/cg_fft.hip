@@ -496,11 +496,11 @@ template <typename T>
 constexpr int ct_rmax() { return sizeof(T) == 4 ? 16 : 8; }
 
 // rows of N = 2^LOGN points, one row per workgroup, N / EPT threads,
-// persistent over the nrows rows of all planes.  Rows no point reaches are
-// skipped 64 at a time: each lane of a wave reads the mask byte of one of
-// the workgroup's next 64 rows and a ballot names the first live one (every
-// wave finds the same row).  PF: the next live row is loaded into registers
-// while the current one is transformed.
+// persistent over the nrows rows of all planes; rows no point reaches are
+// skipped.  PF: the next live row is loaded into registers while the current
+// one is transformed; it is found 64 candidate rows at a time (each lane of
+// a wave reads one mask byte and a ballot names the first live row, the same
+// in every wave).
 template <typename T, int DIR, int LOGN, bool PF>
 __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t<T> *data, int64_t nrows,
                                                       const cx_t<T> *__restrict__ tw, Live live,
@@ -526,12 +526,11 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
         for (int u = 0; u < EPT; u++)
             if (rowmask[n2 + threadIdx.x + u * NT]) cbits |= 1u << u;
     }
-    // live bits of the workgroup's 64 rows g + j gridDim.x, j < 64: each
-    // lane reads one mask byte and a ballot gathers them (a masked row is
-    // zero for dir > 0 and unread for dir < 0).  Rows are numbered in 32
-    // bits (the host launches this kernel for fewer than 2^30 rows); the lane
-    // and n2 are made opaque so that no per-lane value (the lane offset, the
-    // reciprocal of n2) is hoisted and kept live across the transform.
+    // live bits of the workgroup's 64 rows g + j gridDim.x, j < 64 (a masked
+    // row is zero for dir > 0 and unread for dir < 0).  Rows are numbered in
+    // 32 bits (the host launches the PF kernel for fewer than 2^30 rows); the
+    // lane and n2 are made opaque so that no per-lane value (the lane offset,
+    // the reciprocal of n2) is hoisted and kept live across the transform.
     auto probe = [&](int64_t g) -> uint64_t {
         int lane = threadIdx.x & 63, m = n2;
         asm volatile("" : "+v"(lane), "+v"(m));
