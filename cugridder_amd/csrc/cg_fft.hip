@@ -537,18 +537,30 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
         const int c = (int)g + lane * (int)gridDim.x;
         return __ballot(c < (int)nrows && rowmask[(unsigned)c % (unsigned)m] != 0);
     };
-    auto load = [&](int64_t g, cx_t<T> *dst, bool to_lds) {
+    auto load = [&](int64_t g, cx_t<T> *dst) {
         const cx_t<T> *src = data + g * (int64_t)N;
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * NT;
             const bool ld = DIR > 0 ? ((cbits >> u) & 1u) != 0 : live(k);
-            dst[to_lds ? k + (k >> 5) : u] = ld ? src[k] : mkc((T)0, (T)0);
+            dst[u] = ld ? src[k] : mkc((T)0, (T)0);
+        }
+    };
+    // the first Stockham pass (radix EPT, no twiddles) reads elements
+    // threadIdx.x + u NT, the ones this thread loaded: it runs in registers
+    // and writes its outputs EPT threadIdx.x + r to LDS
+    static_assert(EPT == ct_rmax<T>(), "first pass radix");
+    auto first_pass = [&](cx_t<T> *v) {
+        dft_small<T, EPT, DIR>(v);
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int x = (int)threadIdx.x * EPT + r;
+            buf[x + (x >> 5)] = v[r];
         }
     };
     auto transform_store = [&](int64_t g) {
         lds_barrier();
-        ct_fft<T, DIR, N, 1, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
+        ct_fft<T, DIR, N, EPT, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
         cx_t<T> *dst = data + g * (int64_t)N;
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
@@ -570,22 +582,20 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
         };
         cx_t<T> pre[EPT];
         int64_t g = next_live(blockIdx.x);
-        if (g < nrows) load(g, pre, false);
+        if (g < nrows) load(g, pre);
         while (g < nrows) {
-#pragma unroll
-            for (int u = 0; u < EPT; u++) {
-                const int k = threadIdx.x + u * NT;
-                buf[k + (k >> 5)] = pre[u];
-            }
+            first_pass(pre);
             const int64_t gn = next_live(g + gridDim.x);
-            if (gn < nrows) load(gn, pre, false);
+            if (gn < nrows) load(gn, pre);
             transform_store(g);
             g = gn;
         }
     } else {
         for (int64_t g = blockIdx.x; g < nrows; g += gridDim.x) {
             if (rowmask && !rowmask[g % n2]) continue;
-            load(g, buf, true);
+            cx_t<T> v[EPT];
+            load(g, v);
+            first_pass(v);
             transform_store(g);
         }
     }

@@ -169,8 +169,8 @@ def test_row_fft_prefetch_bit_identical(gpu, monkeypatch, npix, double):
     """The row FFT that loads the next live row while transforming the
     current one (rows of at most 512 threads: 512- to 4096-point grids here)
     gives the same ms2dirty image, dirty2ms visibilities and 2-D NUFFT
-    outputs as the one-row-at-a-time kernel (CG_FFT_ROWPF=0), with the row
-    mask skipping rows no point reaches."""
+    outputs, to rounding, as the one-row-at-a-time kernel (CG_FFT_ROWPF=0),
+    with the row mask skipping rows no point reaches."""
     torch = gpu
     from cugridder_amd import Context
     rng = np.random.default_rng(90 + npix)
@@ -201,16 +201,12 @@ def test_row_fft_prefetch_bit_identical(gpu, monkeypatch, npix, double):
                        ctx.nufft2d2(tx, ty, tf, eps).cpu().numpy())
         finally:
             ctx.close()
-    # degridding (dirty2ms, type 2) has no atomics: the same bits.  FP64
-    # gridding sums in LDS atomics whose order varies from run to run, so the
-    # FP64 image and type-1 output agree to rounding (a wrongly placed row
-    # would differ at O(1)); FP32 gridding (MFMA, or FP64 sums rounded to
-    # FP32) gives the same bits.
-    for i, (a, b) in enumerate(zip(out["1"], out["0"])):
-        if double and i in (0, 2):
-            assert rel_l2(a, b) <= 1e-13
-        else:
-            assert np.array_equal(a, b)
+    # the two kernels are separate instruction streams whose compiled sums may
+    # round differently (and FP64 gridding sums in LDS atomics whose order
+    # varies from run to run): they agree to rounding, where a wrongly placed
+    # or skipped row would differ at O(1)
+    for a, b in zip(out["1"], out["0"]):
+        assert rel_l2(a, b) <= (1e-13 if double else 1e-6)
     assert np.abs(out["1"][0]).max() > 0 and np.abs(out["1"][1]).max() > 0
 
 
