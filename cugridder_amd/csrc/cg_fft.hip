@@ -495,6 +495,61 @@ __device__ __forceinline__ void ct_fft(cx_t<T> *buf, const TwTab<T> &tab, int ts
 template <typename T>
 constexpr int ct_rmax() { return sizeof(T) == 4 ? 16 : 8; }
 
+// one row of N held in padded LDS (NB = 1): every pass but the last, then
+// the last pass from LDS into registers, its outputs x = i + r P handed to
+// out(x, value) instead of written back (P = N / R is a multiple of NT, so
+// the thread's outputs are its own columns x = threadIdx.x + u NT)
+template <typename T, int DIR, int N, int P, int NT, int RMAX, typename F>
+__device__ __forceinline__ void ct_fft_row_out(cx_t<T> *buf, const TwTab<T> &tab, F out)
+{
+    constexpr int R = (N / P) >= RMAX ? RMAX : (N / P);
+    if constexpr (P * R < N) {
+        ct_pass<T, DIR, N, R, P, 1, false, true, NT>(buf, tab, 1);
+        ct_fft_row_out<T, DIR, N, P * R, NT, RMAX>(buf, tab, out);
+    } else {
+        static_assert(P > 1 && P % NT == 0, "last pass layout");
+        constexpr int K = P / NT;
+        cx_t<T> u[K][R];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = (int)threadIdx.x + k * NT;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int x = i + r * P;
+                u[k][r] = buf[x + (x >> 5)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = (int)threadIdx.x + k * NT;
+            // W^r for r = 4 r1 + r2 as (W^4)^r1 W^r2, W = w_N^i
+            cx_t<T> w1 = tab.get(i);
+            if (DIR < 0) w1.y = -w1.y;
+            const cx_t<T> w2 = cmul<T>(w1, w1);
+            const cx_t<T> pw[4] = {mkc((T)1, (T)0), w1, w2, cmul<T>(w2, w1)};
+            if constexpr (R <= 4) {
+#pragma unroll
+                for (int r = 1; r < R; r++) u[k][r] = cmul<T>(u[k][r], pw[r]);
+            } else {
+                cx_t<T> w4 = tab.get(4 * i);
+                if (DIR < 0) w4.y = -w4.y;
+                const cx_t<T> w8 = cmul<T>(w4, w4);
+                const cx_t<T> qw[4] = {mkc((T)1, (T)0), w4, w8, cmul<T>(w8, w4)};
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    cx_t<T> v = u[k][r];
+                    if (r & 3) v = cmul<T>(v, pw[r & 3]);
+                    if (r >> 2) v = cmul<T>(v, qw[r >> 2]);
+                    u[k][r] = v;
+                }
+            }
+            dft_small<T, R, DIR>(u[k]);
+#pragma unroll
+            for (int r = 0; r < R; r++) out(i + r * P, u[k][r]);
+        }
+    }
+}
+
 // rows of N = 2^LOGN points, one row per workgroup, N / EPT threads,
 // persistent over the nrows rows of all planes; rows no point reaches are
 // skipped.  PF: the next live row is loaded into registers while the current
@@ -558,16 +613,15 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
             buf[x + (x >> 5)] = v[r];
         }
     };
+    // the remaining passes; the last one stores from registers (output x is
+    // column threadIdx.x + u NT of this thread, u = x / NT)
     auto transform_store = [&](int64_t g) {
         lds_barrier();
-        ct_fft<T, DIR, N, EPT, 1, false, true, NT, ct_rmax<T>()>(buf, tab, 1);
         cx_t<T> *dst = data + g * (int64_t)N;
-#pragma unroll
-        for (int u = 0; u < EPT; u++) {
-            const int k = threadIdx.x + u * NT;
-            const bool st = DIR < 0 ? ((cbits >> u) & 1u) != 0 : live(k);
-            if (st) dst[k] = buf[k + (k >> 5)];
-        }
+        ct_fft_row_out<T, DIR, N, EPT, NT, ct_rmax<T>()>(buf, tab, [&](int x, cx_t<T> v) {
+            const bool st = DIR < 0 ? ((cbits >> (x / NT)) & 1u) != 0 : live(x);
+            if (st) dst[x] = v;
+        });
         lds_barrier();
     };
     if constexpr (PF) {
