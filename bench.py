@@ -225,62 +225,98 @@ def band_kernel(nf2, band):
 
 
 def kernel_table(W, plan, fw, bw, stage_ms):
-    """(kernel, direction, launches per call, ms per call, bound, algorithmic
-    bytes or flops per launch) of every timed kernel of one ms2dirty and one
-    dirty2ms call, complex64 grid (8 B per cell), records 24 B per row.
-    Rm = grid rows points reach, N1 live columns of nf1, N2 live frequencies
-    of nf2, P planes."""
+    """(kernel, direction, launches per call, ms per call, algorithmic bytes
+    per launch) of every timed kernel of one ms2dirty and one dirty2ms call.
+
+    Algorithmic bytes follow the per-unit figures of the HBM model: 36 B per
+    visibility (FP64 uvw 24 + complex64 vis 8 + FP32 weight 4) for the stage
+    that consumes the visibilities (gridding / degridding), 8 B per grid cell
+    written or read once, 8 B per cell per FFT pass read and written, and
+    8 B per live pixel per plane for the w-(I)DFT plus 8 B per image pixel.
+    Cells are counted only where the masks keep them (the grid rows Rm and
+    columns Cm points reach, the N1 live columns, the N2 live frequencies of
+    nf2, the tiles the gridder writes); sort, record and w-range passes are
+    not credited.  P planes, complex64 (8 B) cells."""
     M = W["M"]
     nf1, nf2, P = plan["nf1"], plan["nf2"], plan["nf3"]
     N1, N2 = W["ny"], W["nx"]
-    kw = plan["kw"]
     Md = fw["dense_points"]
     Rm, Cm = fw["rows"], fw["cols"]
     sm, sb = stage_ms["ms2dirty"], stage_ms["dirty2ms"]
     col_kernel = "k_fft_cols_step_r2" if (nf2 > 128 and nf2 & (nf2 - 1) == 0) else "k_fft_cols_step"
+    cells_w = P * fw["tiles_written"] * fw["tile"]
     rows = []
-    rows.append(("k_grid", "ms2dirty", 1, sm["grid"] - sm["grid_dense"], "hbm",
-                 24 * (M - Md) + 8 * P * fw["tiles_written"] * fw["tile"]))
+    # the sparse kernel writes every kept cell once; the dense kernel adds
+    # the dense bins' points onto those cells
+    rows.append(("k_grid", "ms2dirty", 1, sm["grid"] - sm["grid_dense"], 36 * (M - Md) + 8 * cells_w))
     if Md:
-        # 4 MFMA 32x32x2 (4096 flop each) per point pair
-        rows.append(("k_grid_dense", "ms2dirty", 1, sm["grid_dense"], "mfma", 8192 * Md))
-    rows.append(("k_fft_rows_ct", "ms2dirty", 1, sm["fft_rows"], "hbm", 8 * P * Rm * (Cm + N1)))
+        rows.append(("k_grid_dense", "ms2dirty", 1, sm["grid_dense"], 36 * Md))
+    rows.append(("k_fft_rows_ct", "ms2dirty", 1, sm["fft_rows"], 8 * P * Rm * (Cm + N1)))
     if band_kernel(nf2, fw["band"]):
         # one pass: the masked rows of the band in, the live spectrum out
-        rows.append(("k_fft_cols_band", "ms2dirty", 1, sm["fft_cols"], "hbm",
-                     8 * P * Rm * N1 + 8 * P * N2 * N1))
+        rows.append(("k_fft_cols_band", "ms2dirty", 1, sm["fft_cols"], 8 * P * Rm * N1 + 8 * P * N2 * N1))
     else:
-        rows.append((col_kernel, "ms2dirty", 2, sm["fft_cols"], "hbm",
+        rows.append((col_kernel, "ms2dirty", 2, sm["fft_cols"],
                      (8 * P * (Rm * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + N2 * N1)) // 2))
-    rows.append(("k_ms2dirty_epi", "ms2dirty", 1, sm["image"], "hbm", 8 * P * N1 * N2 + 8 * N1 * N2))
+    rows.append(("k_ms2dirty_epi", "ms2dirty", 1, sm["image"], 8 * P * N1 * N2 + 8 * N1 * N2))
     Rb, Cb = bw["rows"], bw["cols"]
-    rows.append(("k_dirty2ms_pro", "dirty2ms", 1, sb["image"], "hbm", 8 * N1 * N2 + 8 * P * N2 * N1))
+    rows.append(("k_dirty2ms_pro", "dirty2ms", 1, sb["image"], 8 * N1 * N2 + 8 * P * N2 * N1))
     if band_kernel(nf2, bw["band"]):
         # one pass: the live spectrum in, the band's rows out
-        rows.append(("k_fft_cols_band", "dirty2ms", 1, sb["fft_cols"], "hbm",
-                     8 * P * N2 * N1 + 8 * P * bw["band"] * N1))
+        rows.append(("k_fft_cols_band", "dirty2ms", 1, sb["fft_cols"], 8 * P * N2 * N1 + 8 * P * bw["band"] * N1))
     else:
-        rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"], "hbm",
+        rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"],
                      (8 * P * (N2 * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + Rb * N1)) // 2))
-    rows.append(("k_fft_rows_ct", "dirty2ms", 1, sb["fft_rows"], "hbm", 8 * P * Rb * (N1 + Cb)))
-    halo = bw["nonempty_tiles"] * (16 + kw - 1) ** 2 * P * 8
-    rows.append(("k_degrid", "dirty2ms", 1, sb["grid"], "hbm", halo + 24 * M + 4 * M + 8 * M))
+    rows.append(("k_fft_rows_ct", "dirty2ms", 1, sb["fft_rows"], 8 * P * Rb * (N1 + Cb)))
+    rows.append(("k_degrid", "dirty2ms", 1, sb["grid"], 36 * M + 8 * P * bw["tiles_written"] * bw["tile"]))
     return rows
 
 
-def run_cpu_baseline(W, workload):
+def hbm_model(W, kernels, ms_per_step):
+    """Whole-step algorithmic bytes (the kernel table summed over launches),
+    the fraction of HBM peak the step reaches on them, and the Mvis/s a step
+    moving exactly those bytes at peak would reach (the mask-aware ceiling)."""
+    total = sum(r[2] * r[4] for r in kernels)
+    return {"step_algorithmic_GB": round(total / 1e9, 3),
+            "step_frac_of_hbm_peak": round(total / (ms_per_step * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+            "ceiling_Mvis_s_at_peak": round(W["M"] / (total / (HBM_PEAK_GBS * 1e9)) / 1e6, 1)}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_cpu_baseline(W, workload, runs=5):
     """FP64 OpenMP restatement of the reference algorithm (the oracle) on the
-    host cores, on a bounded sample of the workload."""
+    host cores: the median of `runs` timed ms2dirty + dirty2ms pairs after
+    one untimed pair, on a bounded sample of the workload."""
     from oracle import oracle as O
     threads = O.num_threads()
+
+    def timed(uvw, freq, vis, wgt, img, nx, ny, fov, eps):
+        ts = []
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            O.ms2dirty(uvw, freq, vis, wgt, nx, ny, fov, eps)
+            O.dirty2ms(uvw, freq, img, wgt, fov, eps)
+            if k:
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), ts
+
     if workload == "plain":
         uvw = W["uvw"].cpu().numpy()
         vis = W["vis"].cpu().numpy().astype(np.complex128)
         wgt = W["wgt"].cpu().numpy().astype(np.float64)
         img = W["img"].cpu().numpy().astype(np.complex128)
-        sample = "the whole workload (131072 rows, 1024^2)"
+        sample = "the whole workload (131072 rows, 1024^2, eps 1e-5)"
         nx, ny, fov, eps = W["nx"], W["ny"], W["fov"], W["eps"]
-    elif workload == "wstack":
+    elif workload in ("wstack", "learned"):
         # 2^20 rows of the same tracks, 1024^2 pixels of the same size
         # (fov 3/8 deg), same eps: the full 16384^2 x 22-plane FP64 grid does
         # not fit a host
@@ -291,18 +327,78 @@ def run_cpu_baseline(W, workload):
         nx = ny = 1024
         fov, eps = W["fov"] / 8.0, W["eps"]
         img = W["img"][:nx, :ny].cpu().numpy().astype(np.complex128)
-        sample = "2^20 rows of the same tracks, 1024^2 pixels of the same size (fov 0.375 deg), eps 1e-6"
+        sample = ("2^20 rows of the same tracks, 1024^2 pixels of the same size (fov 0.375 deg), "
+                  "eps %g" % eps)
     else:
         return None
-    t0 = time.perf_counter()
-    O.ms2dirty(uvw, W["freq"], vis, wgt, nx, ny, fov, eps)
-    t1 = time.perf_counter()
-    O.dirty2ms(uvw, W["freq"], img, wgt, fov, eps)
-    t2 = time.perf_counter()
-    mvis = uvw.shape[0] / (t2 - t0) / 1e6
-    log("cpu baseline: ms2dirty %.2f s, dirty2ms %.2f s on %d threads" % (t1 - t0, t2 - t1, threads))
+    t, ts = timed(uvw, W["freq"], vis, wgt, img, nx, ny, fov, eps)
+    mvis = uvw.shape[0] / t / 1e6
+    log("cpu baseline: median %.2f s per ms2dirty + dirty2ms over %d runs (%s) on %d threads"
+        % (t, runs, ", ".join("%.2f" % x for x in ts), threads))
     return {"value": round(mvis, 4), "unit": "Mvis/s", "cores": threads, "kind": "port",
-            "sample": sample + "; FP64 C/OpenMP restatement of the reference algorithm"}
+            "sample": sample + "; FP64 C/OpenMP restatement of the reference algorithm; median of "
+                      "%d runs after 1 warm-up" % runs,
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "runs_s": [round(x, 3) for x in ts]}
+
+
+def dft_accuracy(torch, W, d_img, v_out, npix_sample=64, nrow_sample=64, seed=0):
+    """Relative L2 error of the last ms2dirty image on sampled pixels and of
+    the last dirty2ms visibilities on sampled rows against an FP64 direct
+    sum evaluated on the GPU (untimed).  dirty(l, m) = sum_j vis_j wgt_j
+    exp(2 pi i f/c (u l + v m + sign w (n - 1))) / n with sign -1, and its
+    adjoint."""
+    dev = d_img.device
+    f64, c128 = torch.float64, torch.complex128
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    nx, ny = W["nx"], W["ny"]
+    foc = 2 * np.pi * W["freq"] / C_LIGHT
+    pix_x = W["fov"] * np.pi / 180 / nx
+    pix_y = W["fov"] * np.pi / 180 / ny
+    uvw = W["uvw"].to(f64)
+    M = uvw.shape[0]
+    wgt = None if W["wgt"] is None else W["wgt"].to(f64)
+
+    def lmn(flat):
+        r, c = flat // ny, flat % ny
+        l_ = (r - nx // 2).to(f64) * pix_x
+        m_ = (c - ny // 2).to(f64) * pix_y
+        r2 = l_ * l_ + m_ * m_
+        return l_, m_, -r2 / (torch.sqrt(1 - r2) + 1)
+
+    pix = torch.randperm(nx * ny, generator=g)[:npix_sample].to(dev)
+    l_, m_, nm1 = lmn(pix)
+    acc = torch.zeros(npix_sample, dtype=c128, device=dev)
+    step = 1 << 22
+    for j0 in range(0, M, step):
+        u = uvw[j0:j0 + step]
+        ph = foc * (u[:, 0:1] * l_[None] + u[:, 1:2] * m_[None] - u[:, 2:3] * nm1[None])
+        cv = W["vis"][j0:j0 + step].to(c128)
+        if wgt is not None:
+            cv = cv * wgt[j0:j0 + step]
+        acc += (cv[:, None] * torch.polar(torch.ones_like(ph), ph)).sum(0)
+    ref = acc / (nm1 + 1)
+    got = d_img.reshape(-1)[pix].to(c128)
+    e_img = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+    rows = torch.randperm(M, generator=g)[:nrow_sample].to(dev)
+    ur = uvw[rows]
+    img = W["img"].reshape(-1)
+    accv = torch.zeros(nrow_sample, dtype=c128, device=dev)
+    pstep = 1 << 22
+    for p0 in range(0, nx * ny, pstep):
+        flat = torch.arange(p0, min(p0 + pstep, nx * ny), device=dev)
+        l_, m_, nm1 = lmn(flat)
+        ph = -foc * (ur[:, 0:1] * l_[None] + ur[:, 1:2] * m_[None] - ur[:, 2:3] * nm1[None])
+        val = img[p0:p0 + pstep].to(c128) / (nm1 + 1)
+        accv += (val[None] * torch.polar(torch.ones_like(ph), ph)).sum(1)
+    if wgt is not None:
+        accv = accv * wgt[rows]
+    gotv = v_out[rows].to(c128)
+    e_vis = float(torch.linalg.norm(gotv - accv) / torch.linalg.norm(accv))
+    return {"ms2dirty_rel_l2_vs_dft": e_img, "dirty2ms_rel_l2_vs_dft": e_vis,
+            "pixels_sampled": npix_sample, "rows_sampled": nrow_sample,
+            "reference": "FP64 direct sum on the GPU (torch), sampled pixels / rows"}
 
 
 def traffic_from_profiles(workload, kernel):
@@ -327,6 +423,8 @@ def main():
     ap.add_argument("--workload", choices=("wstack", "plain", "nufft2d", "batch", "learned"),
                     default="wstack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the direct-sum accuracy check of the last step's outputs")
     ap.add_argument("--nrow", type=int, default=1 << 27, help="rows of the wstack workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 1 GiB device-to-device copy before the steps, so that a "
@@ -444,23 +542,35 @@ def main():
         kernels = kernel_table(W, plan, fw_stats, bw_stats, stage_ms)
         # the dominant kernel: most time per step (a kernel name counts the
         # launches of both directions only when they are the same template)
-        name, d, launches, t_ms, bound, work = max(kernels, key=lambda r: r[3])
+        name, d, launches, t_ms, work = max(kernels, key=lambda r: r[3])
         t_launch = t_ms / launches
-        if bound == "mfma":
-            achieved = work / (t_launch * 1e-3) / 1e12
-            roof = {"kernel": "%s (%s)" % (name, d), "bound": "mfma", "achieved": round(achieved, 2),
-                    "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4), "traffic": None,
-                    "algorithmic_flops": work, "avg_ms": round(t_launch, 4),
-                    "launches_per_call": launches}
-        else:
-            achieved = work / (t_launch * 1e-3) / 1e9
-            roof = {"kernel": "%s (%s)" % (name, d), "bound": "hbm", "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles(args.workload, "%s:%s" % (name, d)),
-                    "algorithmic_bytes": work, "avg_ms": round(t_launch, 4),
-                    "launches_per_call": launches}
-        log("kernels (name, direction, launches, ms per call, bound, work per launch): %s"
+        achieved = work / (t_launch * 1e-3) / 1e9
+        traffic = traffic_from_profiles(args.workload, "%s:%s" % (name, d))
+        roof = {"kernel": "%s (%s)" % (name, d), "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes": work, "avg_ms": round(t_launch, 4),
+                "launches_per_call": launches}
+        if traffic:
+            roof["traffic_GBs"] = round(traffic / (t_launch * 1e-3) / 1e9, 1)
+        if name in ("k_grid_dense", "k_grid", "k_degrid"):
+            # useful arithmetic: kw^3 taps per point, a complex value times a
+            # real kernel product (4 flop) per tap
+            pts = fw_stats["dense_points"] if name == "k_grid_dense" else W["M"]
+            if name == "k_grid":
+                pts = W["M"] - fw_stats["dense_points"]
+            kw = plan["kw"]
+            useful = 4.0 * kw ** 3 * pts
+            roof["useful_TFLOP_s"] = round(useful / (t_launch * 1e-3) / 1e12, 2)
+            roof["useful_frac_of_fp32_mfma_peak"] = round(useful / (t_launch * 1e-3) / 1e12 /
+                                                          FP32_MFMA_PEAK_TFLOPS, 4)
+        roof.update(hbm_model(W, kernels, ms_per_step))
+        step_bytes = traffic_from_profiles(args.workload, "step_total")
+        if step_bytes:
+            # every kernel of one step by the counters (profiles/), over the
+            # measured step time
+            roof["step_counter_GB"] = round(step_bytes / 1e9, 2)
+            roof["step_counter_GBs"] = round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1)
+        log("kernels (name, direction, launches, ms per call, algorithmic bytes per launch): %s"
             % json.dumps(kernels))
     elif args.workload == "nufft2d":
         # type-1 gridding: 16 B geometry + 8 B strength read per point, one
@@ -471,6 +581,15 @@ def main():
         roof = {"kernel": "k_grid2d_mfma (type 1)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None, "algorithmic_bytes": nbytes, "avg_ms": round(t_grid, 4)}
+
+    accuracy = None
+    if args.workload in ("wstack", "plain", "learned") and not args.no_accuracy:
+        # the timed steps' outputs against an FP64 direct sum (untimed)
+        try:
+            accuracy = dft_accuracy(torch, W, out_d, out_v)
+            log("accuracy: %s" % json.dumps(accuracy))
+        except Exception as e:
+            log("accuracy check failed: %r" % (e,))
 
     cpu = None
     if not args.no_cpu:
@@ -499,7 +618,7 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
            "data": "synthetic", "config": cfg, "stage_ms": {d: {k: round(v, 3) for k, v in s.items()}
                                                             for d, s in stage_ms.items()},
-           "roofline": roof, "cpu_baseline": cpu, "wall_s": round(wall, 2)}
+           "roofline": roof, "cpu_baseline": cpu, "accuracy": accuracy, "wall_s": round(wall, 2)}
     print(json.dumps(out), flush=True)
 
 

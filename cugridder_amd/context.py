@@ -17,6 +17,47 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _check(t, name, dtypes, shape=None, numel=None):
+    """Raises unless t is a contiguous GPU tensor of one of `dtypes` with
+    the given shape (None entries match anything) or element count; the
+    kernels read and write raw pointers, so a host tensor, a strided view or
+    a short buffer must not reach them."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor, got %s" % (name, type(t).__name__))
+    if not t.is_cuda:
+        raise TypeError("%s must be on the GPU (got device %s)" % (name, t.device))
+    if t.dtype not in dtypes:
+        raise TypeError("%s must have dtype %s, got %s"
+                        % (name, " or ".join(str(d) for d in dtypes), t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if shape is not None and (t.dim() != len(shape) or
+                              any(s is not None and s != n for s, n in zip(shape, t.shape))):
+        raise ValueError("%s must have shape %s, got %s"
+                         % (name, tuple("*" if s is None else s for s in shape), tuple(t.shape)))
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s must have %d elements, got %d" % (name, numel, t.numel()))
+    return t
+
+
+def _check_imaging(torch, uvw, data, wgt, nchan):
+    """Argument checks shared by ms2dirty / dirty2ms: (prec, uvw_prec,
+    wgt_prec) of the combinations the library implements: complex64 data
+    with FP32 weights and FP64 or FP32 uvw, or complex128 data with FP64 uvw
+    and weights."""
+    prec = {torch.complex64: 0, torch.complex128: 1}.get(data.dtype)
+    if prec is None:
+        raise TypeError("vis / image must be complex64 or complex128, got %s" % data.dtype)
+    uvw_types = (torch.float64, torch.float32) if prec == 0 else (torch.float64,)
+    _check(uvw, "uvw", uvw_types, shape=(None, 3))
+    nrow = uvw.shape[0]
+    rdt = torch.float32 if prec == 0 else torch.float64
+    if wgt is not None:
+        _check(wgt, "wgt", (rdt,), numel=nrow * nchan)
+    return prec, (1 if uvw.dtype == torch.float64 else 0), prec, nrow
+
+
 class Context:
     """Wraps cg_ctx_create; `stream` is a torch.cuda.Stream (None = torch's
     current stream at construction)."""
@@ -63,19 +104,6 @@ class Context:
 
     # ------------------------------------------------------------ imaging
     @staticmethod
-    def _prec(vis_dtype, uvw_dtype, wgt):
-        import torch
-        if vis_dtype == torch.complex64:
-            prec = 0
-        elif vis_dtype == torch.complex128:
-            prec = 1
-        else:
-            raise TypeError("vis/image must be complex64 or complex128")
-        up = 1 if uvw_dtype == torch.float64 else 0
-        wp = prec if wgt is None else (1 if wgt.dtype == torch.float64 else 0)
-        return prec, up, wp
-
-    @staticmethod
     def _freqs(freq):
         f = np.ascontiguousarray(np.atleast_1d(np.asarray(freq, dtype=np.float64)).ravel())
         if f.size == 0:
@@ -88,10 +116,14 @@ class Context:
         frequency (vis [nrow]) or nchan of them (vis, wgt [nrow, nchan]); all
         channels are summed into the image."""
         torch = self._torch
-        prec, up, wp = self._prec(vis.dtype, uvw.dtype, wgt)
         f = self._freqs(freq)
+        if not isinstance(vis, torch.Tensor):
+            raise TypeError("vis must be a torch tensor")
+        prec, up, wp, nrow = _check_imaging(torch, uvw, vis, wgt, f.size)
+        _check(vis, "vis", (vis.dtype,), numel=nrow * f.size)
         if out is None:
             out = torch.empty((nxdirty, nydirty), dtype=vis.dtype, device=vis.device)
+        _check(out, "out", (vis.dtype,), shape=(nxdirty, nydirty))
         ier = lib().cg_ms2dirty_mc_dev(self._h, prec, uvw.shape[0], f.size, nxdirty, nydirty,
                                        float(fov), f, _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
                                        _ptr(out), float(epsilon), float(sigma), int(sign))
@@ -102,12 +134,16 @@ class Context:
         """Complex visibilities from a device image [nx, ny]: [nrow] for one
         frequency, [nrow, nchan] for nchan of them."""
         torch = self._torch
-        prec, up, wp = self._prec(dirty.dtype, uvw.dtype, wgt)
         f = self._freqs(freq)
+        if not isinstance(dirty, torch.Tensor):
+            raise TypeError("dirty must be a torch tensor")
+        prec, up, wp, nrow = _check_imaging(torch, uvw, dirty, wgt, f.size)
+        _check(dirty, "dirty", (dirty.dtype,), shape=(None, None))
         nx, ny = dirty.shape
         if out is None:
-            shape = (uvw.shape[0],) if f.size == 1 else (uvw.shape[0], f.size)
+            shape = (nrow,) if f.size == 1 else (nrow, f.size)
             out = torch.empty(shape, dtype=dirty.dtype, device=dirty.device)
+        _check(out, "out", (dirty.dtype,), numel=nrow * f.size)
         ier = lib().cg_dirty2ms_mc_dev(self._h, prec, uvw.shape[0], f.size, nx, ny, float(fov), f,
                                        _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
                                        float(epsilon), float(sigma), int(sign))
@@ -118,22 +154,42 @@ class Context:
         """A Batch of independent channels on this context's stream."""
         return Batch(self, direction, items, epsilon, sigma, sign, autotune, graph)
 
-    def nufft2d1(self, x, y, c, N1, N2, epsilon, sigma=2.0, out=None):
+    def _nufft_prec(self, x, y, data):
         torch = self._torch
-        prec = 0 if c.dtype == torch.complex64 else 1
+        if not isinstance(data, torch.Tensor):
+            raise TypeError("strengths / modes must be a torch tensor")
+        prec = {torch.complex64: 0, torch.complex128: 1}.get(data.dtype)
+        if prec is None:
+            raise TypeError("strengths / modes must be complex64 or complex128, got %s" % data.dtype)
+        rdt = torch.float32 if prec == 0 else torch.float64
+        _check(x, "x", (rdt,), shape=(None,))
+        _check(y, "y", (rdt,), numel=x.shape[0])
+        return prec
+
+    def nufft2d1(self, x, y, c, N1, N2, epsilon, sigma=2.0, out=None):
+        """Type 1: modes [N2, N1] from strengths c at points (x, y) in
+        [-pi, pi); x, y are float32 for complex64 c and float64 for
+        complex128 c."""
+        torch = self._torch
+        prec = self._nufft_prec(x, y, c)
+        _check(c, "c", (c.dtype,), numel=x.shape[0])
         if out is None:
             out = torch.empty((N2, N1), dtype=c.dtype, device=c.device)
+        _check(out, "out", (c.dtype,), shape=(N2, N1))
         ier = lib().cg_nufft2d_dev(self._h, 1, prec, x.shape[0], _ptr(x), _ptr(y), _ptr(c), N1, N2,
                                    _ptr(out), float(epsilon), float(sigma))
         check(ier, "nufft2d1")
         return out
 
     def nufft2d2(self, x, y, fk, epsilon, sigma=2.0, out=None):
+        """Type 2: strengths at points (x, y) from modes fk [N2, N1]."""
         torch = self._torch
-        prec = 0 if fk.dtype == torch.complex64 else 1
+        prec = self._nufft_prec(x, y, fk)
+        _check(fk, "fk", (fk.dtype,), shape=(None, None))
         N2, N1 = fk.shape
         if out is None:
             out = torch.empty(x.shape[0], dtype=fk.dtype, device=fk.device)
+        _check(out, "out", (fk.dtype,), numel=x.shape[0])
         ier = lib().cg_nufft2d_dev(self._h, 2, prec, x.shape[0], _ptr(x), _ptr(y), _ptr(out), N1,
                                    N2, _ptr(fk), float(epsilon), float(sigma))
         check(ier, "nufft2d2")
@@ -260,7 +316,10 @@ class Batch:
     or None), dirty (complex64 [nx, ny]), fov (deg) and freq (Hz) as device
     tensors / numbers.  direction "ms2dirty" writes every dirty, "dirty2ms"
     every vis.  The tensors must stay alive and unchanged in shape while the
-    batch exists."""
+    batch exists, and uvw must keep its contents: every item's plan (w-planes,
+    the u / v ranges the FFT passes keep and the sort keys number) is fixed
+    from the uvw present at creation.  vis, wgt and dirty contents may change
+    between runs; rebuild the batch for new uvw."""
 
     def __init__(self, ctx, direction, items, epsilon, sigma=2.0, sign=-1, autotune=True,
                  graph=True):
@@ -270,11 +329,12 @@ class Batch:
         self.items = list(items)
         arr = (BatchItem * len(self.items))()
         for k, it in enumerate(self.items):
-            if it["uvw"].dtype != torch.float64 or it["vis"].dtype != torch.complex64 or \
-                    it["dirty"].dtype != torch.complex64:
-                raise TypeError("batch items are float64 uvw with complex64 vis and dirty")
-            if it.get("wgt") is not None and it["wgt"].dtype != torch.float32:
-                raise TypeError("batch weights are float32")
+            _check(it["uvw"], "items[%d].uvw" % k, (torch.float64,), shape=(None, 3))
+            m = it["uvw"].shape[0]
+            _check(it["vis"], "items[%d].vis" % k, (torch.complex64,), numel=m)
+            _check(it["dirty"], "items[%d].dirty" % k, (torch.complex64,), shape=(None, None))
+            if it.get("wgt") is not None:
+                _check(it["wgt"], "items[%d].wgt" % k, (torch.float32,), numel=m)
             nx, ny = it["dirty"].shape
             arr[k] = BatchItem(it["uvw"].shape[0], nx, ny, float(it["fov"]), float(it["freq"]),
                                _ptr(it["uvw"]), _ptr(it["vis"]), _ptr(it.get("wgt")),

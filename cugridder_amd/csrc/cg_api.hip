@@ -73,6 +73,10 @@ struct Ctx {
     // certified at an eps no larger than the call's and narrower than ES's
     struct Learned {
         double eps = 0;  // 0: none
+        // the oversampling factor the kernel was trained and certified for:
+        // training (learn_beta, the training samples) and certification run
+        // at sigma = 2, so calls at another sigma keep the ES kernel
+        double sigma = 2.0;
         LearnedKernel K;
     } learned[LEARN_MAXKW + 1];
     int use_learned = 1;
@@ -331,7 +335,7 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
     if (sizeof(T) == 4 && C.use_learned) {
         for (int kw = 2; kw <= P.conv.kw; kw++) {
             const Ctx::Learned &L = C.learned[kw];
-            if (!(L.eps > 0) || L.eps > eps) continue;
+            if (!(L.eps > 0) || L.eps > eps || sigma != L.sigma) continue;
             // the gridders evaluate exp(beta sqrt(1 - x^2) + y): the node
             // values take the same normalisation (e^beta times psi)
             ConvOpts o = P.conv;

@@ -188,7 +188,13 @@ int cg_dirty2ms_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
  * captures the whole batch into one HIP graph), runs it once, and keeps a
  * private workspace; cg_batch_run replays it on the context's stream
  * (direction 0: vis -> dirty, 1: dirty -> vis).  The device buffers must
- * stay valid and keep their w-range while the batch exists. */
+ * stay valid while the batch exists.  Each item's plan is fixed from the uvw
+ * contents at creation: the w-range (planes), the u and v ranges (the grid
+ * rows and columns the FFT passes keep, and the rectangle of tiles the sort
+ * keys number).  A replay must therefore see the same uvw; vis, wgt and
+ * dirty contents may change between replays.  Points outside the planned
+ * ranges would be gridded into the wrong tiles: rebuild the batch when uvw
+ * changes. */
 typedef struct {
     int nrow, nxdirty, nydirty;
     double fov, freq;

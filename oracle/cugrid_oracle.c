@@ -801,6 +801,35 @@ void orc_dft_dirty2ms(int nrow, int nxdirty, int nydirty, double pix_x,
     }
 }
 
+/* the same sum for an image that is zero except at npx pixels (flat indices
+ * pixels[], complex values vals[]): cost nsel * npx, so that images of
+ * 8192^2 pixels can be checked on sampled rows */
+void orc_dft_dirty2ms_sparse(int nxdirty, int nydirty, double pix_x,
+                             double pix_y, double freq, const double *uvw,
+                             int npx, const int64_t *pixels, const double *vals,
+                             const double *wgt, int sign, int nsel,
+                             const int64_t *rows, double *out)
+{
+    const cplx *d = (const cplx *)vals;
+    cplx *o = (cplx *)out;
+    double foc = 2.0 * ORC_PI * freq / ORC_C;
+    int N1 = nydirty, N2 = nxdirty;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int q = 0; q < nsel; q++) {
+        int64_t j = rows[q];
+        cplx acc = 0;
+        for (int k = 0; k < npx; k++) {
+            int row = (int)(pixels[k] / N1), col = (int)(pixels[k] % N1);
+            double l = (row - N2 / 2) * pix_x, m = (col - N1 / 2) * pix_y;
+            double nm1 = orc_nm1(l, m);
+            double ph = -foc * (uvw[3 * j] * l + uvw[3 * j + 1] * m +
+                                sign * uvw[3 * j + 2] * nm1);
+            acc += d[k] / (nm1 + 1.0) * (cos(ph) + I * sin(ph));
+        }
+        o[q] = acc * (wgt ? wgt[j] : 1.0);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* 2-D NUFFT type 1 / type 2 (test/nufft_2d_1_test.cu:126-258,              */
 /* test/nufft_2d_2_test.cu:140-315), points in [-pi,pi), pirange=1.           */

@@ -67,6 +67,9 @@ def lib():
         L.orc_dft_dirty2ms.argtypes = [c_int, c_int, c_int, c_double, c_double,
                                        c_double, _dp, _cp, ctypes.c_void_p,
                                        c_int, c_int, _ip, _cp]
+        L.orc_dft_dirty2ms_sparse.argtypes = [c_int, c_int, c_double, c_double, c_double, _dp,
+                                              c_int, _ip, _cp, ctypes.c_void_p, c_int, c_int,
+                                              _ip, _cp]
         L.orc_nufft2d1.argtypes = [c_int, _dp, _dp, _cp, c_int, c_int,
                                    c_double, c_double, c_int, _cp]
         L.orc_nufft2d1.restype = c_int
@@ -280,6 +283,23 @@ def dft_dirty2ms(uvw, freq, dirty, wgt, pix_x, pix_y, sign=-1, rows=None):
     out = np.zeros(rows.size, dtype=np.complex128)
     lib().orc_dft_dirty2ms(uvw.shape[0], nx, ny, pix_x, pix_y, freq, uvw,
                            dirty, wp, sign, rows.size, rows, out)
+    return out
+
+
+def dft_dirty2ms_sparse(uvw, freq, nxdirty, nydirty, pixels, values, wgt, pix_x, pix_y, sign=-1,
+                        rows=None):
+    """dft_dirty2ms of an image that is zero except values at the flat pixel
+    indices `pixels` (row * nydirty + col)."""
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    pixels = np.ascontiguousarray(pixels, dtype=np.int64)
+    values = np.ascontiguousarray(values, dtype=np.complex128)
+    w, wp = _wptr(wgt)
+    if rows is None:
+        rows = np.arange(uvw.shape[0], dtype=np.int64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    out = np.zeros(rows.size, dtype=np.complex128)
+    lib().orc_dft_dirty2ms_sparse(nxdirty, nydirty, pix_x, pix_y, freq, uvw, pixels.size, pixels,
+                                  values, wp, sign, rows.size, rows, out)
     return out
 
 

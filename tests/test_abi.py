@@ -93,3 +93,25 @@ def test_planner_rejects_bad_arguments():
     assert L.cg_plan_query(10, 64, 64, 1.0, 1e9, 0.0, 1.0, 1e-5, 1.0, -1, 0, out) == 2
     # eps below the precision floor is clamped with ier = 1 (setup_conv_opts)
     assert L.cg_plan_query(10, 64, 64, 1.0, 1e9, 0.0, 1.0, 1e-9, 2.0, -1, 1, out) == 1
+
+
+def test_nf3_known_values():
+    """The w-plane count at the two W-stacking workloads, from the closed
+    form nf3 = max(even(floor(2 sigma S X / pi) + kw + 1), 2 kw) worked by
+    hand: 1024^2 at 2 deg, 1e8 Hz, |w| <= 300 m (S = 1.523e-4, X = 628.9,
+    so 7.12 -> 8 < 2 kw = 12): 12 planes; 8192^2 at 3 deg, 1.4 GHz, |w| =
+    5000 wavelengths (S = 3.428e-4, X = 2 pi 5000, 13.71 + 8 = 21.71): 22
+    planes, and 18 / 24 at 4000 / 5500 wavelengths.  An over- or under-sized
+    plane count fails here."""
+    c = 299792458.0
+    L = cugridder_amd.lib()
+    out = np.zeros(16)
+    w = 300.0 * 2 * np.pi * 1e8 / c
+    assert L.cg_plan_query(131072, 1024, 1024, 2.0, 1e8, -w, w, 1e-5, 2.0, -1, 1, out) == 0
+    assert int(out[2]) == 12 and int(out[3]) == 6
+    for wl, nf3 in ((5000.0, 22), (4000.0, 18), (5500.0, 24)):
+        w = 2 * np.pi * wl
+        assert L.cg_plan_query(1 << 27, 8192, 8192, 3.0, 1.4e9, -w, w, 1e-6, 2.0, -1, 1, out) == 0
+        assert (int(out[0]), int(out[2]), int(out[3])) == (16384, nf3, 7), wl
+        uvw = np.array([[0.0, 0.0, wl * c / 1.4e9], [0.0, 0.0, -wl * c / 1.4e9]])
+        assert O.plan(2, 8192, 8192, 3.0, 1.4e9, uvw, 1e-6)["nf3"] == nf3

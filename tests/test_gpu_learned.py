@@ -29,7 +29,7 @@ def trained(L):
     2^20 seeded visibilities of a 256 x 256 patch."""
     samples = L.training_samples(n_vis=1 << 20, patch=256, seed=0)
     out = {}
-    for kw in (5, 6):
+    for kw in (4, 5, 6):
         print("training kw", kw, flush=True)
         out[kw] = L.train(kw, samples=samples)
     return out
@@ -69,7 +69,7 @@ def test_es_start_is_the_es_kernel(L):
     assert loss == pytest.approx(np.mean(np.abs(F - 1) ** 2), rel=1e-6)
 
 
-@pytest.mark.parametrize("kw", [5, 6])
+@pytest.mark.parametrize("kw", [4, 5, 6])
 def test_training_beats_es_at_same_support(L, trained, kw):
     p, losses = trained[kw]
     assert np.all(np.isfinite(losses))
@@ -166,12 +166,33 @@ def test_learned_imaging_vs_dft(L, trained):
 
 
 def test_certification_accepts_and_rejects(L, trained):
-    """The ES judge (support 7) accepts the support-6 kernel at 2e-5 and
-    rejects the support-5 kernel at 1e-5 (its per-axis error alone is
-    above that)."""
-    ok = L.certify(6, trained[6][0], 2e-5, patch=256, nvis=1 << 15)
+    """The ES judge (support 7, eps 1e-6) accepts the support-6 kernel at
+    1e-5 (the learned workload's bar) and rejects supports 5 and 4 there
+    (their per-axis error alone is above it)."""
+    ok = L.certify(6, trained[6][0], 1e-5, patch=256, nvis=1 << 15)
     print("certify kw 6:", ok)
-    assert ok["accepted"] and ok["ms2dirty"] > 0
-    bad = L.certify(5, trained[5][0], 1e-5, patch=256, nvis=1 << 15)
-    print("certify kw 5:", bad)
-    assert not bad["accepted"]
+    assert ok["accepted"] and 0 < ok["ms2dirty"] <= 1e-5 and ok["dirty2ms"] <= 1e-5
+    for kw in (5, 4):
+        bad = L.certify(kw, trained[kw][0], 1e-5, patch=256, nvis=1 << 15)
+        print("certify kw %d:" % kw, bad)
+        assert not bad["accepted"]
+
+
+def test_learned_kernel_only_at_its_sigma(L, trained):
+    """A kernel trained and certified at sigma 2 is not used by a call at
+    another oversampling factor: that call plans the ES kernel of its own
+    sigma (support from the sigma != 2 rule of setup_conv_opts)."""
+    import torch
+    from cugridder_amd import Context
+    uvw, vis, fov, freq, uvw_t, vis_t = _problem(L, 64, 2000, seed=8)
+    ctx = Context()
+    try:
+        ctx.set_kernel(6, trained[6][0], 1e-5)
+        ctx.ms2dirty(uvw_t, vis_t, None, 64, 64, fov, freq, 1e-5)
+        assert ctx.last_plan()["kw"] == 6
+        ctx.ms2dirty(uvw_t, vis_t, None, 64, 64, fov, freq, 1e-5, sigma=1.5)
+        _, kw_es, _, _ = O.setup_conv(1e-5, 1.5, single=True)
+        assert ctx.last_plan()["kw"] == kw_es and kw_es > 6
+        torch.cuda.synchronize()
+    finally:
+        ctx.close()

@@ -25,9 +25,11 @@ def _floor(single):
     return 2e-6 if single else 1e-13
 
 
-# reference parameter grid, python/curagridder/test_wst.py:36-43 (nx/ny paired)
-SWEEP = list(itertools.product((2, 27, 100), ((32, 128), (64, 250), (16, 64)), (1, 10, 20),
-                               (2e-1, 5e-3, 5e-5, 5e-7, 5e-12), (False, True), (False, True)))
+# reference parameter grid, python/curagridder/test_wst.py:36-43: nrow, nxdirty
+# crossed with nydirty, fov, eps, precision, weights
+SWEEP = list(itertools.product((2, 27, 100), tuple(itertools.product((32, 64, 16), (128, 250, 64))),
+                               (1, 10, 20), (2e-1, 5e-3, 5e-5, 5e-7, 5e-12), (False, True),
+                               (False, True)))
 
 
 @pytest.mark.parametrize("nrow,shape,fov,eps,single,use_wgt", SWEEP)

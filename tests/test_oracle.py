@@ -234,3 +234,21 @@ def test_nufft2d_vs_dft(eps):
     fk = rng.standard_normal((N2, N1)) + 1j * rng.standard_normal((N2, N1))
     cc = O.nufft2d2(x, y, fk, eps)
     assert rel_l2(cc, O.dft2d2(x, y, fk)) < 10 * eps
+
+
+def test_sparse_image_dft_equals_dense_dft():
+    """The sparse-image direct sum (used for 8192^2 images) equals the dense
+    one on an image that is zero except at the listed pixels."""
+    k = wst_case(40, 24, 32, 10.0)
+    rng = np.random.default_rng(3)
+    pix = rng.choice(24 * 32, 50, replace=False).astype(np.int64)
+    val = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    img = np.zeros(24 * 32, np.complex128)
+    img[pix] = val
+    img = img.reshape(24, 32)
+    for sign in (-1, 1):
+        dense = O.dft_dirty2ms(k["uvw"], k["f0"], img, k["wgt"], k["xpix"], k["ypix"], sign=sign)
+        rows = np.array([0, 7, 39], np.int64)
+        sparse = O.dft_dirty2ms_sparse(k["uvw"], k["f0"], 24, 32, pix, val, k["wgt"], k["xpix"],
+                                       k["ypix"], sign=sign, rows=rows)
+        assert rel_l2(sparse, dense[rows]) < 1e-13
