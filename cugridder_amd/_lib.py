@@ -99,7 +99,16 @@ def lib():
     L.nufft2d2.argtypes = [c_int, f64, f64, c128, c_int, c_int, c128, c_double, c_double]
     L.nufft2d1f.argtypes = [c_int, f32, f32, c64, c_int, c_int, c64, c_float, c_float]
     L.nufft2d2f.argtypes = [c_int, f32, f32, c64, c_int, c_int, c64, c_float, c_float]
-    for name in ("ms2dirty_1", "ms2dirty_2", "dirty2ms_1", "dirty2ms_2", "ms2dirtyf_1",
+    # separate pixel sizes and plain 2-D gridding (prec 0 mixed, 1 double,
+    # 2 float); uvw / vis / wgt / dirty are raw host pointers
+    L.ms2dirty_px.argtypes = [c_int, c_int, c_int, c_int, c_int, c_double, c_double, f64,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int,
+                              c_int]
+    L.dirty2ms_px.argtypes = [c_int, c_int, c_int, c_int, c_int, c_double, c_double, f64,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int,
+                              c_int]
+    for name in ("ms2dirty_px", "dirty2ms_px", "ms2dirty_1", "ms2dirty_2", "dirty2ms_1",
+                 "dirty2ms_2", "ms2dirtyf_1",
                  "ms2dirtyf_2", "dirty2msf_1", "dirty2msf_2", "ms2dirty_mixed",
                  "dirty2ms_mixed", "ms2dirty_mc_2", "dirty2ms_mc_2", "ms2dirty_mc_mixed",
                  "dirty2ms_mc_mixed", "nufft2d1", "nufft2d2", "nufft2d1f", "nufft2d2f"):
@@ -124,6 +133,12 @@ def lib():
     L.cg_dirty2ms_mc_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, f64,
                                      c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                      c_double, c_double, c_int]
+    L.cg_ms2dirty_px_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double,
+                                     c_double, f64, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                                     c_void_p, c_double, c_double, c_int, c_int]
+    L.cg_dirty2ms_px_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double,
+                                     c_double, f64, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                                     c_void_p, c_double, c_double, c_int, c_int]
     L.cg_nufft2d_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_int, c_void_p, c_double, c_double]
     L.cg_batch_create.argtypes = [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int,
@@ -148,7 +163,8 @@ def lib():
     L.cg_ctx_set_kernel.argtypes = [c_void_p, c_int, c_int, f32n, c_double]
     for name in ("cg_ctx_create", "cg_ctx_destroy", "cg_ctx_set_option", "cg_ctx_stage_times",
                  "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ctx_last_band", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
-                 "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_batch_create", "cg_batch_run",
+                 "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_ms2dirty_px_dev",
+                 "cg_dirty2ms_px_dev", "cg_batch_create", "cg_batch_run",
                  "cg_batch_info", "cg_batch_destroy",
                  "cg_nufft2d_dev", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
                  "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split",

@@ -110,11 +110,27 @@ class Context:
             raise ValueError("freq is empty")
         return f
 
+    @staticmethod
+    def _pix(fov, nx, pix_y, do_wstacking):
+        """(pix_x, pix_y) in radians when the call needs the NIFTY geometry
+        (a pixel size along ny other than fov / ny, or plain 2-D gridding),
+        else None."""
+        if pix_y is None and do_wstacking:
+            return None
+        if not (fov > 0):
+            raise ValueError("fov must be positive, got %r" % fov)
+        pix_x = float(fov) * np.pi / 180.0 / nx
+        if pix_y is not None and not (pix_y > 0):
+            raise ValueError("pix_y must be a positive pixel size in radians, got %r" % pix_y)
+        return pix_x, float(pix_y) if pix_y is not None else 0.0
+
     def ms2dirty(self, uvw, vis, wgt, nxdirty, nydirty, fov, freq, epsilon, sigma=2.0, sign=-1,
-                 out=None):
+                 out=None, pix_y=None, do_wstacking=True):
         """Complex dirty image [nx, ny] from device visibilities.  freq is one
         frequency (vis [nrow]) or nchan of them (vis, wgt [nrow, nchan]); all
-        channels are summed into the image."""
+        channels are summed into the image.  fov (degrees) spans nxdirty;
+        pix_y (radians) overrides the pixel size along ny (default fov /
+        nydirty); do_wstacking=False grids in 2-D, ignoring w (no 1/n)."""
         torch = self._torch
         f = self._freqs(freq)
         if not isinstance(vis, torch.Tensor):
@@ -124,15 +140,25 @@ class Context:
         if out is None:
             out = torch.empty((nxdirty, nydirty), dtype=vis.dtype, device=vis.device)
         _check(out, "out", (vis.dtype,), shape=(nxdirty, nydirty))
-        ier = lib().cg_ms2dirty_mc_dev(self._h, prec, uvw.shape[0], f.size, nxdirty, nydirty,
-                                       float(fov), f, _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
-                                       _ptr(out), float(epsilon), float(sigma), int(sign))
+        px = self._pix(fov, nxdirty, pix_y, do_wstacking)
+        if px is None:
+            ier = lib().cg_ms2dirty_mc_dev(self._h, prec, uvw.shape[0], f.size, nxdirty, nydirty,
+                                           float(fov), f, _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
+                                           _ptr(out), float(epsilon), float(sigma), int(sign))
+        else:
+            py = px[1] if px[1] > 0 else float(fov) * np.pi / 180.0 / nydirty
+            ier = lib().cg_ms2dirty_px_dev(self._h, prec, uvw.shape[0], f.size, nxdirty, nydirty,
+                                           px[0], py, f, _ptr(uvw), up, _ptr(vis), _ptr(wgt), wp,
+                                           _ptr(out), float(epsilon), float(sigma), int(sign),
+                                           1 if do_wstacking else 0)
         check(ier, "ms2dirty")
         return out
 
-    def dirty2ms(self, uvw, dirty, wgt, fov, freq, epsilon, sigma=2.0, sign=-1, out=None):
+    def dirty2ms(self, uvw, dirty, wgt, fov, freq, epsilon, sigma=2.0, sign=-1, out=None,
+                 pix_y=None, do_wstacking=True):
         """Complex visibilities from a device image [nx, ny]: [nrow] for one
-        frequency, [nrow, nchan] for nchan of them."""
+        frequency, [nrow, nchan] for nchan of them.  pix_y and do_wstacking
+        as in ms2dirty."""
         torch = self._torch
         f = self._freqs(freq)
         if not isinstance(dirty, torch.Tensor):
@@ -144,9 +170,17 @@ class Context:
             shape = (nrow,) if f.size == 1 else (nrow, f.size)
             out = torch.empty(shape, dtype=dirty.dtype, device=dirty.device)
         _check(out, "out", (dirty.dtype,), numel=nrow * f.size)
-        ier = lib().cg_dirty2ms_mc_dev(self._h, prec, uvw.shape[0], f.size, nx, ny, float(fov), f,
-                                       _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
-                                       float(epsilon), float(sigma), int(sign))
+        px = self._pix(fov, nx, pix_y, do_wstacking)
+        if px is None:
+            ier = lib().cg_dirty2ms_mc_dev(self._h, prec, uvw.shape[0], f.size, nx, ny, float(fov),
+                                           f, _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
+                                           float(epsilon), float(sigma), int(sign))
+        else:
+            py = px[1] if px[1] > 0 else float(fov) * np.pi / 180.0 / ny
+            ier = lib().cg_dirty2ms_px_dev(self._h, prec, uvw.shape[0], f.size, nx, ny, px[0], py,
+                                           f, _ptr(uvw), up, _ptr(out), _ptr(wgt), wp, _ptr(dirty),
+                                           float(epsilon), float(sigma), int(sign),
+                                           1 if do_wstacking else 0)
         check(ier, "dirty2ms")
         return out
 

@@ -52,6 +52,10 @@ struct Ctx {
     int band_fft = 1;        // band-limited column passes when the rows points reach allow
     int compact_keys = 1;    // sort keys number only the tiles the u / v ranges allow
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
+    // geometry of the current call beyond (fov, nx, ny): a separate pixel
+    // size along ny and plain 2-D gridding; set only for the duration of a
+    // call by the entry points that take them
+    PlanGeom geom;
     int64_t last_m = 0;
     int last_cshift = 0;
     // workspace (grow-only)
@@ -328,7 +332,8 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
             wmax = std::max(wmax, std::max(a, b));
         }
     }
-    int ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin, wmax);
+    int ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin, wmax,
+                         nullptr, &C.geom);
     if (ier >= 2) return ier;
     // the narrowest learned kernel certified for eps and no wider than the
     // ES kernel (at equal support it is the more accurate one)
@@ -346,7 +351,7 @@ static int plan_call(Ctx &C, WPlan &P, int64_t nrow, int nchan, const double *fr
             o.tab_nq = L.K.nq;
             for (int q = 0; q < L.K.nq; q++) o.tab_psi[q] = L.K.qpsi[q] * std::exp(L.K.beta);
             ier = make_wplan(P, nrow * nchan, nx, ny, fov, freq[0], eps, sigma, sign, single, wmin,
-                             wmax, &o);
+                             wmax, &o, &C.geom);
             if (ier >= 2) return ier;
             break;
         }
@@ -507,14 +512,22 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
                               C.band_fft ? P.band_len : 0));
     if (C.stop_after == 2) return P.ier;
     CG_TRY(clk.mark(ST_IMAGE));
-    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
-    CG_TRY(ensure(C.zq, sizeof(double) * nq));
-    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
     CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
     CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
     WArgs w = make_wargs(P);
     CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
     CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
+    if (!P.wstack) {
+        // plain 2-D: the u/v deconvolution only (deconv_2d type 1)
+        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), dirty, as<double>(C.phi1),
+                                            as<double>(C.phi2), P.N1, P.N2, P.nf1, P.nf2, 1,
+                                            C.stream)));
+        CG_TRY(clk.mark(ST_COUNT));
+        return P.ier;
+    }
+    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    CG_TRY(ensure(C.zq, sizeof(double) * nq));
+    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
     CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
     CG_TRY((launch_ms2dirty_epilogue<T, cx_t<T>>(as<cx_t<T>>(C.grid), as<double>(C.zq),
                                                   as<double>(C.phi3), as<double>(C.phi1),
@@ -585,19 +598,26 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
                                                 as<int32_t>(C.vals), M, a, 1, as<geo_t<T>>(C.geo),
                                                 as<cx_t<T>>(C.val), rec, C.stream)));
     CG_TRY(clk.mark(ST_IMAGE));
-    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
-    CG_TRY(ensure(C.zq, sizeof(double) * nq));
-    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
     CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
     CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
     CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
     WArgs w = make_wargs(P);
     CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
     CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
-    CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
-    CG_TRY((launch_dirty2ms_prologue<T, cx_t<T>>(dirty, as<double>(C.zq), as<double>(C.phi3),
-                                                  as<double>(C.phi1), as<double>(C.phi2),
-                                                  as<cx_t<T>>(C.grid), w, C.stream)));
+    if (!P.wstack) {
+        // plain 2-D: the u/v deconvolution only (deconv_2d type 2)
+        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), const_cast<cx_t<T> *>(dirty),
+                                            as<double>(C.phi1), as<double>(C.phi2), P.N1, P.N2,
+                                            P.nf1, P.nf2, 2, C.stream)));
+    } else {
+        const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+        CG_TRY(ensure(C.zq, sizeof(double) * nq));
+        CG_TRY(ensure(C.phi3, sizeof(double) * nq));
+        CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
+        CG_TRY((launch_dirty2ms_prologue<T, cx_t<T>>(dirty, as<double>(C.zq), as<double>(C.phi3),
+                                                      as<double>(C.phi1), as<double>(C.phi2),
+                                                      as<cx_t<T>>(C.grid), w, C.stream)));
+    }
     if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     // the degridder reads only the rows its tiles' points reach
@@ -1066,13 +1086,14 @@ struct HostBuf {
 template <typename T, typename TU, typename TW>
 static int host_ms2dirty(int nrow, int nchan, const double *freq, int nx, int ny, double fov,
                          const TU *uvw, const void *vis, const TW *wgt, void *dirty, double eps,
-                         double sigma, int sign, bool single)
+                         double sigma, int sign, bool single, const PlanGeom &geom = PlanGeom())
 {
     if (nrow < 0 || nchan < 1 || !freq || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis))
         return CG_ERR_ARG;
     try {
         Ctx C;
         env_overrides(C);
+        C.geom = geom;
         const size_t R = (size_t)nrow, M = R * (size_t)nchan;
         HostBuf du, dv, dw, dd;
         CG_TRY(du.alloc(sizeof(TU) * 3 * R));
@@ -1098,13 +1119,14 @@ static int host_ms2dirty(int nrow, int nchan, const double *freq, int nx, int ny
 template <typename T, typename TU, typename TW>
 static int host_dirty2ms(int nrow, int nchan, const double *freq, int nx, int ny, double fov,
                          const TU *uvw, void *vis, const TW *wgt, const void *dirty, double eps,
-                         double sigma, int sign, bool single)
+                         double sigma, int sign, bool single, const PlanGeom &geom = PlanGeom())
 {
     if (nrow < 0 || nchan < 1 || !freq || nx <= 0 || ny <= 0 || !uvw || !dirty || (nrow > 0 && !vis))
         return CG_ERR_ARG;
     try {
         Ctx C;
         env_overrides(C);
+        C.geom = geom;
         const size_t R = (size_t)nrow, M = R * (size_t)nchan;
         HostBuf du, dv, dw, dd;
         CG_TRY(du.alloc(sizeof(TU) * 3 * R));
@@ -1463,6 +1485,99 @@ int cg_dirty2ms_dev(void *ctx, int prec, int nrow, int nxdirty, int nydirty, dou
 {
     return cg_dirty2ms_mc_dev(ctx, prec, nrow, 1, nxdirty, nydirty, fov, &freq, d_uvw, uvw_prec,
                               d_vis, d_wgt, wgt_prec, d_dirty, epsilon, sigma, sign);
+}
+
+// NIFTY geometry: fov from pix_x as the reference's binding derives it
+// (python/curagridder/cursl.py:99), pix_y honoured, plain 2-D on request
+static int px_geom(int nx, double pix_x, double pix_y, int do_wstacking, double &fov, PlanGeom &g)
+{
+    if (!(pix_x > 0) || !(pix_y > 0) || nx <= 0) return CG_ERR_ARG;
+    fov = pix_x * nx * 180.0 / kPi;
+    g.pix_y = pix_y;
+    g.wstack = do_wstacking ? 1 : 0;
+    return CG_OK;
+}
+
+struct GeomScope {
+    Ctx &C;
+    GeomScope(Ctx &c, const PlanGeom &g) : C(c) { C.geom = g; }
+    ~GeomScope() { C.geom = PlanGeom(); }
+};
+
+int cg_ms2dirty_px_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                       double pix_x, double pix_y, const double *freq, const void *d_uvw,
+                       int uvw_prec, const void *d_vis, const void *d_wgt, int wgt_prec,
+                       void *d_dirty, double epsilon, double sigma, int sign, int do_wstacking)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C) return CG_ERR_ARG;
+    double fov = 0;
+    PlanGeom g;
+    CG_TRY(px_geom(nxdirty, pix_x, pix_y, do_wstacking, fov, g));
+    GeomScope scope(*C, g);
+    return cg_ms2dirty_mc_dev(ctx, prec, nrow, nchan, nxdirty, nydirty, fov, freq, d_uvw,
+                              uvw_prec, d_vis, d_wgt, wgt_prec, d_dirty, epsilon, sigma, sign);
+}
+
+int cg_dirty2ms_px_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                       double pix_x, double pix_y, const double *freq, const void *d_uvw,
+                       int uvw_prec, void *d_vis, const void *d_wgt, int wgt_prec,
+                       const void *d_dirty, double epsilon, double sigma, int sign,
+                       int do_wstacking)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C) return CG_ERR_ARG;
+    double fov = 0;
+    PlanGeom g;
+    CG_TRY(px_geom(nxdirty, pix_x, pix_y, do_wstacking, fov, g));
+    GeomScope scope(*C, g);
+    return cg_dirty2ms_mc_dev(ctx, prec, nrow, nchan, nxdirty, nydirty, fov, freq, d_uvw,
+                              uvw_prec, d_vis, d_wgt, wgt_prec, d_dirty, epsilon, sigma, sign);
+}
+
+int ms2dirty_px(int prec, int nrow, int nchan, int nxdirty, int nydirty, double pix_x,
+                double pix_y, const double *freq, const void *uvw, const void *vis,
+                const void *wgt, void *dirty, double epsilon, double sigma, int sign,
+                int do_wstacking)
+{
+    double fov = 0;
+    PlanGeom g;
+    CG_TRY(px_geom(nxdirty, pix_x, pix_y, do_wstacking, fov, g));
+    if (prec == 0)
+        return host_ms2dirty<float, double, float>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                   (const double *)uvw, vis, (const float *)wgt,
+                                                   dirty, epsilon, sigma, sign, true, g);
+    if (prec == 1)
+        return host_ms2dirty<double, double, double>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                     (const double *)uvw, vis, (const double *)wgt,
+                                                     dirty, epsilon, sigma, sign, false, g);
+    if (prec == 2)
+        return host_ms2dirty<float, float, float>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                  (const float *)uvw, vis, (const float *)wgt,
+                                                  dirty, epsilon, sigma, sign, true, g);
+    return CG_ERR_ARG;
+}
+
+int dirty2ms_px(int prec, int nrow, int nchan, int nxdirty, int nydirty, double pix_x,
+                double pix_y, const double *freq, const void *uvw, void *vis, const void *wgt,
+                const void *dirty, double epsilon, double sigma, int sign, int do_wstacking)
+{
+    double fov = 0;
+    PlanGeom g;
+    CG_TRY(px_geom(nxdirty, pix_x, pix_y, do_wstacking, fov, g));
+    if (prec == 0)
+        return host_dirty2ms<float, double, float>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                   (const double *)uvw, vis, (const float *)wgt,
+                                                   dirty, epsilon, sigma, sign, true, g);
+    if (prec == 1)
+        return host_dirty2ms<double, double, double>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                     (const double *)uvw, vis, (const double *)wgt,
+                                                     dirty, epsilon, sigma, sign, false, g);
+    if (prec == 2)
+        return host_dirty2ms<float, float, float>(nrow, nchan, freq, nxdirty, nydirty, fov,
+                                                  (const float *)uvw, vis, (const float *)wgt,
+                                                  dirty, epsilon, sigma, sign, true, g);
+    return CG_ERR_ARG;
 }
 
 int cg_batch_create(void **batch, void *ctx, int direction, int n, const cg_batch_item *items,

@@ -216,7 +216,7 @@ Tiling choose_tiling(int nf1, int nf2, int nplanes, int kw, int elem_bytes,
 
 int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
                double freq, double eps, double sigma, int sign, bool single,
-               double wmin, double wmax, const ConvOpts *conv)
+               double wmin, double wmax, const ConvOpts *conv, const PlanGeom *geom)
 {
     if (nrow < 0 || nxdirty <= 0 || nydirty <= 0 || !(fov > 0) || !(freq > 0))
         return 2;
@@ -240,6 +240,18 @@ int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
     // src/astro/cugridder.cu:48-76
     p.pix_x = fov / 180.0 * PI / (double)p.N2;
     p.pix_y = fov / 180.0 * PI / (double)p.N1;
+    if (geom && geom->pix_y > 0) p.pix_y = geom->pix_y;
+    p.nf1 = num_cells(p.N1, sigma, p.conv.kw);
+    p.nf2 = num_cells(p.N2, sigma, p.conv.kw);
+    if (p.nf1 > 65536 || p.nf2 > 65536) return 3;
+    if (geom && !geom->wstack) {
+        // one plane, no w-terms: the w-phase, w-correction and 1/n vanish
+        p.wstack = 0;
+        p.nf3 = 1;
+        p.gamma = 1.0;
+        p.h = 0.0;
+        return ier;
+    }
     double l_min = -0.5 * p.pix_x * p.N2;
     double m_min = -0.5 * p.pix_y * p.N1;
     double temp = 1.0 - l_min * l_min - m_min * m_min;
@@ -254,9 +266,7 @@ int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
     p.o_center = o_min / 2.0;
     p.o_half_width = std::fabs(o_min / 2.0);
     set_nhg_w(p.o_half_width, p.i_half_width, p.conv.kw, sigma, p.nf3, p.h, p.gamma);
-    p.nf1 = num_cells(p.N1, sigma, p.conv.kw);
-    p.nf2 = num_cells(p.N2, sigma, p.conv.kw);
-    if (p.nf1 > 65536 || p.nf2 > 65536 || p.nf3 > 65535) return 3;
+    if (p.nf3 > 65535) return 3;
     return ier;
 }
 

@@ -69,7 +69,18 @@ struct WPlan {
     // from band_s, cyclic (band_len = 0: unknown or the whole axis); the
     // same for the grid columns (nf1 axis)
     int band_s = 0, band_len = 0, cband_s = 0, cband_len = 0;
+    // 1: improved W-stacking (w-planes, w-correction, 1/n); 0: plain 2-D
+    // gridding that ignores w (one plane, no n), NIFTY's do_wstacking=False
+    int wstack = 1;
     int64_t grid_cells() const { return (int64_t)nf1 * nf2 * nf3; }
+};
+
+// Image geometry beyond the reference's (fov, nx, ny): pix_y > 0 sets the
+// pixel size along ny (radians) instead of fov / ny, as NIFTY's rad_pix_y;
+// wstack = 0 asks for plain 2-D gridding (w ignored, no 1/n).
+struct PlanGeom {
+    double pix_y = 0;
+    int wstack = 1;
 };
 
 // Sets p.band_s / p.band_len from the range [ulo, uhi] of u * 2 pi f / c
@@ -85,7 +96,8 @@ void plan_bands(WPlan &p, double ulo, double uhi, double vlo, double vhi);
 // learned kernel of smaller support); every size follows its kw.
 int make_wplan(WPlan &p, int64_t nrow, int nxdirty, int nydirty, double fov,
                double freq, double eps, double sigma, int sign, bool single,
-               double wmin_scaled, double wmax_scaled, const ConvOpts *conv = nullptr);
+               double wmin_scaled, double wmax_scaled, const ConvOpts *conv = nullptr,
+               const PlanGeom *geom = nullptr);
 
 // Chooses tile extents and plane chunking for an LDS budget (bytes) and
 // complex element size; `nplanes` is the grid depth and `halo` the extra

@@ -33,7 +33,7 @@ CoordArgs make_coord_args(const WPlan &p, const Tiling &t)
     a.nf3 = p.nf3;
     a.kw = p.conv.kw;
     a.pirange = 0;
-    a.has_w = 1;
+    a.has_w = p.wstack;
     set_coord_tiling(a, t);
     return a;
 }

@@ -118,6 +118,25 @@ int dirty2ms_mc_mixed(int nrow, int nchan, int nxdirty, int nydirty, double fov,
                       const float *wgt, const float *dirty, double epsilon,
                       double sigma, int sign);
 
+/* NIFTY geometry (no counterpart in the reference C-ABI: its binding
+ * ignores rad_pix_y and do_wstacking, python/curagridder/cursl.py:78-99).
+ * pix_x, pix_y are the pixel sizes in radians along nxdirty and nydirty
+ * (the reference derives pix_y = pix_x * nxdirty / nydirty); do_wstacking
+ * 0 asks for plain 2-D gridding: w is ignored and there is no 1/n,
+ *   dirty(l,m) = sum_j wgt_j vis_j exp(2 pi i f/c (u l + v m)),
+ * one grid plane, as NIFTY's do_wstacking=False.  Host arrays; freq[nchan],
+ * vis and wgt [nrow][nchan] (wgt may be NULL).  prec 0: FP64 uvw, complex64
+ * vis/dirty, FP32 wgt; 1: FP64 uvw, complex128, FP64 wgt; 2: FP32 uvw,
+ * complex64, FP32 wgt. */
+int ms2dirty_px(int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                double pix_x, double pix_y, const double *freq, const void *uvw,
+                const void *vis, const void *wgt, void *dirty, double epsilon,
+                double sigma, int sign, int do_wstacking);
+int dirty2ms_px(int prec, int nrow, int nchan, int nxdirty, int nydirty,
+                double pix_x, double pix_y, const double *freq, const void *uvw,
+                void *vis, const void *wgt, const void *dirty, double epsilon,
+                double sigma, int sign, int do_wstacking);
+
 /* Contexts: a reusable device workspace bound to a HIP stream (NULL = the
  * null stream).  Device-pointer calls below run the same pipeline as the
  * host entry points on inputs already in HBM. */
@@ -179,6 +198,19 @@ int cg_dirty2ms_mc_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
                        const void *d_uvw, int uvw_prec, void *d_vis,
                        const void *d_wgt, int wgt_prec, const void *d_dirty,
                        double epsilon, double sigma, int sign);
+/* the same with the NIFTY geometry of ms2dirty_px / dirty2ms_px */
+int cg_ms2dirty_px_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
+                       int nydirty, double pix_x, double pix_y,
+                       const double *freq, const void *d_uvw, int uvw_prec,
+                       const void *d_vis, const void *d_wgt, int wgt_prec,
+                       void *d_dirty, double epsilon, double sigma, int sign,
+                       int do_wstacking);
+int cg_dirty2ms_px_dev(void *ctx, int prec, int nrow, int nchan, int nxdirty,
+                       int nydirty, double pix_x, double pix_y,
+                       const double *freq, const void *d_uvw, int uvw_prec,
+                       void *d_vis, const void *d_wgt, int wgt_prec,
+                       const void *d_dirty, double epsilon, double sigma,
+                       int sign, int do_wstacking);
 /* Batches of independent channels (no counterpart in the reference, which
  * images one channel per call): every item has its own rows, image size,
  * field of view and frequency; complex64 vis/image, FP64 uvw, FP32 wgt (may

@@ -444,6 +444,12 @@ typedef struct {
     int sign;
 } orc_plan;
 
+/* Pixel size along ny (radians) for the plans that follow, when > 0: the
+ * reference derives it as fov/ny (cugridder.cu:48-52); NIFTY's rad_pix_y
+ * sets it separately.  0 restores the reference's derivation. */
+static double orc_pix_y_override = 0.0;
+void orc_set_pix_y(double pix_y) { orc_pix_y_override = pix_y > 0 ? pix_y : 0.0; }
+
 /* Builds the plan exactly as gridder_setting/setup_gridder_plan do,
  * with the w range taken from the scaled w (w*f/c*sign*2pi). */
 int orc_make_plan(int nrow, int nxdirty, int nydirty, double fov, double freq,
@@ -464,6 +470,7 @@ int orc_make_plan(int nrow, int nxdirty, int nydirty, double fov, double freq,
         return 2;
     P->pix_x = fov / 180.0 * ORC_PI / (double)P->N2;
     P->pix_y = fov / 180.0 * ORC_PI / (double)P->N1;
+    if (orc_pix_y_override > 0) P->pix_y = orc_pix_y_override;
     double l_min = -0.5 * P->pix_x * P->N2;
     double m_min = -0.5 * P->pix_y * P->N1;
     double temp = 1.0 - l_min * l_min - m_min * m_min;

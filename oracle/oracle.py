@@ -13,6 +13,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+C_LIGHT = 299792458.0
 
 _dp = np.ctypeslib.ndpointer(np.float64, flags="C")
 _cp = np.ctypeslib.ndpointer(np.complex128, flags="C")
@@ -83,6 +84,7 @@ def lib():
                                       _dp, c_double, c_double, c_int, c_int, _dp]
         L.orc_plan_export.restype = c_int
         L.orc_num_threads.restype = c_int
+        L.orc_set_pix_y.argtypes = [c_double]
         L.orc_ms2dirty_grid.argtypes = [c_int, c_int, c_int, c_double, c_double, _dp, _cp,
                                         ctypes.c_void_p, c_double, c_double, c_int, c_int, _cp]
         L.orc_ms2dirty_grid.restype = c_int
@@ -204,32 +206,79 @@ def interp3d(x, y, z, fw, kw, beta, es_c, pirange=0):
 # ---------------------------------------------------- W-stacking pipelines
 
 def ms2dirty(uvw, freq, vis, wgt, nxdirty, nydirty, fov, eps, sigma=2.0,
-             sign=-1, single=False):
-    """FP64 restatement of ms2dirty_*; returns complex dirty [nx, ny]."""
+             sign=-1, single=False, pix_y=None):
+    """FP64 restatement of ms2dirty_*; returns complex dirty [nx, ny].
+    pix_y (radians) replaces the reference's fov / ny pixel size along ny."""
     uvw = np.ascontiguousarray(uvw, dtype=np.float64)
     vis = np.ascontiguousarray(vis, dtype=np.complex128).ravel()
     w, wp = _wptr(wgt)
     dirty = np.zeros((nxdirty, nydirty), dtype=np.complex128)
-    ier = lib().orc_ms2dirty(uvw.shape[0], nxdirty, nydirty, fov, freq, uvw,
-                             vis, wp, dirty, eps, sigma, sign, int(single))
+    lib().orc_set_pix_y(float(pix_y or 0.0))
+    try:
+        ier = lib().orc_ms2dirty(uvw.shape[0], nxdirty, nydirty, fov, freq, uvw,
+                                 vis, wp, dirty, eps, sigma, sign, int(single))
+    finally:
+        lib().orc_set_pix_y(0.0)
     if ier >= 2:
         raise ValueError("oracle ms2dirty: bad parameters (ier=%d)" % ier)
     return dirty
 
 
 def dirty2ms(uvw, freq, dirty, wgt, fov, eps, sigma=2.0, sign=-1,
-             single=False):
+             single=False, pix_y=None):
     """FP64 restatement of dirty2ms_*; returns complex vis [nrow]."""
     uvw = np.ascontiguousarray(uvw, dtype=np.float64)
     dirty = np.ascontiguousarray(dirty, dtype=np.complex128)
     nx, ny = dirty.shape
     w, wp = _wptr(wgt)
     vis = np.zeros(uvw.shape[0], dtype=np.complex128)
-    ier = lib().orc_dirty2ms(uvw.shape[0], nx, ny, fov, freq, uvw, vis, wp,
-                             dirty, eps, sigma, sign, int(single))
+    lib().orc_set_pix_y(float(pix_y or 0.0))
+    try:
+        ier = lib().orc_dirty2ms(uvw.shape[0], nx, ny, fov, freq, uvw, vis, wp,
+                                 dirty, eps, sigma, sign, int(single))
+    finally:
+        lib().orc_set_pix_y(0.0)
     if ier >= 2:
         raise ValueError("oracle dirty2ms: bad parameters (ier=%d)" % ier)
     return vis
+
+
+def _wrap_pi(x):
+    """x mod 2 pi in [-pi, pi) (SHIFT_RESCALE's wrap, include/conv.h:24)."""
+    return x - 2 * np.pi * np.floor((x + np.pi) / (2 * np.pi))
+
+
+def uv_coords_2d(uvw, freq, pix_x, pix_y):
+    """Plain 2-D gridding coordinates in [-pi, pi): x (grid axis 1, image
+    columns, ny) = 2 pi f/c v pix_y and y (axis 2, rows, nx) = 2 pi f/c u
+    pix_x, as the W-stacking coordinates of cugridder.cu:191,201-202
+    without w."""
+    uvw = np.asarray(uvw, dtype=np.float64)
+    s = 2 * np.pi * freq / C_LIGHT
+    return _wrap_pi(uvw[:, 1] * s * pix_y), _wrap_pi(uvw[:, 0] * s * pix_x)
+
+
+def ms2dirty_2d(uvw, freq, vis, wgt, nxdirty, nydirty, pix_x, pix_y, eps, sigma=2.0,
+                single=False):
+    """Plain 2-D gridding (NIFTY do_wstacking=False: w ignored, no 1/n):
+    dirty(l, m) = sum_j wgt_j vis_j e^{2 pi i f/c (u l + v m)}, the 2-D
+    NUFFT type 1 restatement (conv_2d_nputsdriven + deconv_2d) on the
+    scaled (v, u); returns complex dirty [nx, ny]."""
+    x, y = uv_coords_2d(uvw, freq, pix_x, pix_y)
+    c = np.asarray(vis, dtype=np.complex128).ravel()
+    if wgt is not None:
+        c = c * np.asarray(wgt, dtype=np.float64).ravel()
+    return nufft2d1(x, y, c, nydirty, nxdirty, eps, sigma, single)
+
+
+def dirty2ms_2d(uvw, freq, dirty, wgt, pix_x, pix_y, eps, sigma=2.0, single=False):
+    """Adjoint of ms2dirty_2d: vis_j = wgt_j sum_{l,m} dirty e^{-2 pi i f/c
+    (u l + v m)}, the 2-D NUFFT type 2 restatement."""
+    x, y = uv_coords_2d(uvw, freq, pix_x, pix_y)
+    v = nufft2d2(x, y, np.asarray(dirty, dtype=np.complex128), eps, sigma, single)
+    if wgt is not None:
+        v = v * np.asarray(wgt, dtype=np.float64).ravel()
+    return v
 
 
 def ms2dirty_grid(uvw, freq, vis, wgt, nxdirty, nydirty, fov, eps, sigma=2.0, sign=-1,

@@ -54,3 +54,28 @@ def test_dtype_shape_and_size_errors():
                        dev(torch.zeros(7, dtype=torch.float32)), 1)
     assert _check_imaging(torch, uvw, dev(torch.zeros(16, dtype=torch.complex64)),
                           dev(torch.zeros(16, dtype=torch.float32)), 2) == (0, 1, 0, 8)
+
+
+def test_cursl_geometry_checks():
+    """cursl refuses bad pixel sizes and image shapes before any call (the
+    reference binding passes them on unchecked), and routes a pixel size
+    along ny other than rad_pix_x * nx / ny, or do_wstacking=False, away
+    from the reference entry points."""
+    import numpy as np
+
+    from cugridder_amd.cursl import _check_rows, _geometry
+    with pytest.raises(ValueError, match="rad_pix_x"):
+        _geometry(64, 32, 0.0, None, True)
+    with pytest.raises(ValueError, match="rad_pix_y"):
+        _geometry(64, 32, 1e-4, -1e-4, True)
+    with pytest.raises(ValueError, match="positive integers"):
+        _geometry(64, 0, 1e-4, None, True)
+    assert _geometry(64, 32, 1e-4, 2e-4, True) == (2e-4, True)
+    assert _geometry(64, 32, 1e-4, None, True) == (2e-4, True)
+    assert _geometry(64, 32, 1e-4, 1e-4, True) == (1e-4, False)
+    assert _geometry(64, 32, 1e-4, 2e-4, False) == (2e-4, False)
+    with pytest.raises(ValueError, match="uvw"):
+        _check_rows(np.zeros((5, 2)), None, "ms", 1)
+    with pytest.raises(ValueError, match="ms must hold"):
+        _check_rows(np.zeros((5, 3)), np.zeros(4), "ms", 1)
+    _check_rows(np.zeros((5, 3)), np.zeros((5, 2)), "ms", 2)

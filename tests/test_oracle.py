@@ -252,3 +252,53 @@ def test_sparse_image_dft_equals_dense_dft():
         sparse = O.dft_dirty2ms_sparse(k["uvw"], k["f0"], 24, 32, pix, val, k["wgt"], k["xpix"],
                                        k["ypix"], sign=sign, rows=rows)
         assert rel_l2(sparse, dense[rows]) < 1e-13
+
+
+@pytest.mark.parametrize("nrow,nx,ny,fov,eps", [(60, 16, 24, 10.0, 1e-9), (200, 32, 20, 30.0, 1e-5)])
+def test_plain_2d_restatement_vs_numpy_sum(nrow, nx, ny, fov, eps):
+    """NIFTY's do_wstacking=False image, sum_j wgt_j vis_j e^{2 pi i f/c (u
+    l + v m)} (w ignored, no 1/n), written here in numpy, against the
+    oracle's 2-D NUFFT route (axis assignment u -> rows, v -> columns, and
+    the sign); its adjoint against the conjugate transpose."""
+    k = wst_case(nrow, nx, ny, fov)
+    k["uvw"] *= 40.0  # phases well beyond a radian at the image edge
+    l = (np.arange(nx) - nx // 2) * k["xpix"]
+    m = (np.arange(ny) - ny // 2) * k["ypix"]
+    s = 2 * np.pi * k["f0"] / 299792458.0
+    E = np.exp(1j * s * (k["uvw"][:, 0][:, None, None] * l[None, :, None] +
+                         k["uvw"][:, 1][:, None, None] * m[None, None, :]))
+    c = k["ms"] * k["wgt"]
+    brute = np.einsum("j,jab->ab", c, E)
+    got = O.ms2dirty_2d(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, k["xpix"], k["ypix"], eps)
+    assert rel_l2(got, brute) < 10 * eps
+    # a transposed image (u and v swapped) is far off
+    swapped = np.einsum("j,jab->ab", c, np.exp(1j * s * (
+        k["uvw"][:, 1][:, None, None] * l[None, :, None] +
+        k["uvw"][:, 0][:, None, None] * m[None, None, :])))
+    assert rel_l2(got, swapped) > 1e-2
+    d = k["dirty"] + 0.5j
+    v = O.dirty2ms_2d(k["uvw"], k["f0"], d, k["wgt"], k["xpix"], k["ypix"], eps)
+    vb = k["wgt"] * np.einsum("jab,ab->j", np.conj(E), d)
+    assert rel_l2(v, vb) < 10 * eps
+
+
+def test_pix_y_override_vs_dft():
+    """The W-stacking restatement planned with a pixel size along ny other
+    than fov / ny agrees with the direct sum at that pixel size and not at
+    the reference's derived one."""
+    nx, ny, fov, eps = 32, 48, 10.0, 1e-9
+    k = wst_case(150, nx, ny, fov)
+    xp = k["xpix"]
+    yp = xp  # square pixels on a non-square image
+    d = O.ms2dirty(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, fov, eps, pix_y=yp)
+    t = O.dft_ms2dirty(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, xp, yp)
+    assert rel_l2(d, t) < 5 * eps
+    t_ref = O.dft_ms2dirty(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, xp, fov * np.pi / 180 / ny)
+    assert rel_l2(d, t_ref) > 1e-3
+    dd = k["dirty"] + 0j
+    v = O.dirty2ms(k["uvw"], k["f0"], dd, k["wgt"], fov, eps, pix_y=yp)
+    tv = O.dft_dirty2ms(k["uvw"], k["f0"], dd, k["wgt"], xp, yp)
+    assert rel_l2(v, tv) < 100 * eps
+    # the override is reset after the call
+    d0 = O.ms2dirty(k["uvw"], k["f0"], k["ms"], k["wgt"], nx, ny, fov, eps)
+    assert rel_l2(d0, t_ref) < 5 * eps
