@@ -121,6 +121,7 @@ def lib():
     L.cg_ctx_reset_times.argtypes = [c_void_p]
     L.cg_ctx_last_plan.argtypes = [c_void_p, f64]
     L.cg_ctx_last_band.argtypes = [c_void_p, np.ctypeslib.ndpointer(np.int32, flags="C")]
+    L.cg_ctx_last_slab.argtypes = [c_void_p, np.ctypeslib.ndpointer(np.int32, flags="C")]
     L.cg_ms2dirty_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                   c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                   c_double, c_double, c_int]
@@ -162,7 +163,7 @@ def lib():
     L.cg_learn_tabulate.argtypes = [c_int, c_int, f32, f32, f64]
     L.cg_ctx_set_kernel.argtypes = [c_void_p, c_int, c_int, f32n, c_double]
     for name in ("cg_ctx_create", "cg_ctx_destroy", "cg_ctx_set_option", "cg_ctx_stage_times",
-                 "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ctx_last_band", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
+                 "cg_ctx_reset_times", "cg_ctx_last_plan", "cg_ctx_last_band", "cg_ctx_last_slab", "cg_ms2dirty_dev", "cg_dirty2ms_dev",
                  "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_ms2dirty_px_dev",
                  "cg_dirty2ms_px_dev", "cg_batch_create", "cg_batch_run",
                  "cg_batch_info", "cg_batch_destroy",

@@ -262,6 +262,13 @@ class Context:
         lib().cg_ctx_last_band(self._h, out)
         return int(out[0]), int(out[1])
 
+    def last_slab(self):
+        """(w-planes per slab, slabs) of the last W-stacking call's grid
+        buffer; (0, 1) when the whole grid was held at once."""
+        out = np.zeros(2, np.int32)
+        lib().cg_ctx_last_slab(self._h, out)
+        return int(out[0]), int(out[1])
+
     # ------------------------------------------------------------ test hooks
     @staticmethod
     def spectrum_rows(n2):

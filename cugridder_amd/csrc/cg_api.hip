@@ -51,6 +51,10 @@ struct Ctx {
     int prerec = 1;          // complex64 records in two coalescing passes
     int band_fft = 1;        // band-limited column passes when the rows points reach allow
     int compact_keys = 1;    // sort keys number only the tiles the u / v ranges allow
+    // grid buffer limit in MiB (0: what the device has free); a W-stacking
+    // grid larger than the limit is processed in slabs of w-planes
+    int grid_limit_mb = 0;
+    int last_slab = 0, last_nslab = 1;  // planes per slab and slabs of the last call
     int stop_after = 0;      // test hook: 1 = after gridding/prologue, 2 = after FFT
     // geometry of the current call beyond (fov, nx, ny): a separate pixel
     // size along ny and plain 2-D gridding; set only for the duration of a
@@ -133,6 +137,7 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_BAND_FFT")) C.band_fft = std::atoi(e);
     if (const char *e = std::getenv("CG_COMPACT_KEYS")) C.compact_keys = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
+    if (const char *e = std::getenv("CG_GRID_LIMIT_MB")) C.grid_limit_mb = std::atoi(e);
 }
 
 struct StageClock {
@@ -201,11 +206,11 @@ static int grid_masks(Ctx &C, GridArgs &g, const uint8_t *mask, int nf1, int nf2
 // uvw, vis and wgt; dirty2ms reads less and keeps the one-pass gather): the
 // grid buffer, which is not in use until after the records are built
 template <typename T>
-static int records_scratch(Ctx &C, const WPlan &P, int64_t M, int dir, float4 *&rec)
+static int records_scratch(Ctx &C, size_t grid_bytes, int64_t M, int dir, float4 *&rec)
 {
     rec = nullptr;
     if (sizeof(T) != 4 || M <= 0 || dir != 0 || !C.prerec) return CG_OK;
-    CG_TRY(ensure(C.grid, std::max((size_t)P.grid_cells() * sizeof(cx_t<T>), (size_t)32 * M)));
+    CG_TRY(ensure(C.grid, std::max(grid_bytes, (size_t)32 * M)));
     rec = as<float4>(C.grid);
     return CG_OK;
 }
@@ -410,8 +415,45 @@ struct CallPlan {
     CoordArgs a;
     int64_t M = 0;
     int dense_min = 0;  // > 0: dense bins go through the MFMA gridder
+    int slab = 0;       // > 0: w-planes per slab of the grid buffer (out-of-core flow)
     int ier = 0;
 };
+
+// Planes per slab when the W-stacking grid does not fit the grid limit (the
+// context option, else 90% of what the device has free, counting the grid
+// buffer the context already holds, less room for the call's other
+// buffers): 0 when it fits whole.  The slabs are processed one after the
+// other through one buffer, as the reference's mem_limit pipeline does
+// (src/astro/ra_exec.cu:292-389, 520-613).
+static int plan_slab(Ctx &C, const WPlan &P, int eb, int64_t M, int &slab)
+{
+    slab = 0;
+    if (!P.wstack || P.nf3 <= 1) return CG_OK;
+    const size_t plane = (size_t)P.nf1 * P.nf2 * eb;
+    const size_t whole = plane * P.nf3;
+    size_t limit;
+    if (C.grid_limit_mb > 0) {
+        limit = (size_t)C.grid_limit_mb << 20;
+    } else {
+        size_t fr = 0, tot = 0;
+        CG_HIP(hipMemGetInfo(&fr, &tot));
+        const size_t avail = fr + C.grid.bytes;
+        const size_t other = (size_t)M * 96 + ((size_t)256 << 20);
+        limit = avail > other ? (size_t)((avail - other) * 0.9) : 0;
+    }
+    if (whole <= limit) return CG_OK;
+    if (plane > limit) return CG_ERR_ALLOC;  // not even one plane fits
+    const int per = (int)std::min<size_t>(limit / plane, (size_t)P.nf3);
+    const int nslab = (P.nf3 + per - 1) / per;
+    slab = (P.nf3 + nslab - 1) / nslab;  // balanced slabs
+    return CG_OK;
+}
+
+// bytes of the grid buffer: the whole grid, or one slab of planes
+static size_t slab_bytes(const WPlan &P, int slab, int eb)
+{
+    return (size_t)P.nf1 * P.nf2 * (slab ? slab : P.nf3) * eb;
+}
 
 // vis and wgt are [nrow][nchan]; every (row, channel) is one point
 template <typename T, typename TU>
@@ -423,17 +465,20 @@ static int plan_ms2dirty(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const do
     cp.ier = plan_call<T, TU>(C, P, nrow, nchan, freq, nx, ny, fov, uvw, eps, sigma, sign, single);
     if (cp.ier >= 2) return cp.ier;
     const int eb = (int)sizeof(cx_t<T>);
+    cp.M = nrow * nchan;
+    CG_TRY(plan_slab(C, P, eb, cp.M, cp.slab));
     // gridding accumulates in FP64 (16 bytes per cell) whatever the precision
-    cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, P.conv.kw, 16, tn.grid_lds, tn.tile_grid, 0);
+    cp.t = choose_tiling(P.nf1, P.nf2, cp.slab ? cp.slab : P.nf3, P.conv.kw, 16, tn.grid_lds,
+                         tn.tile_grid, 0);
     cp.a = make_coord_args(P, cp.t);
     set_key_rect(cp.a, P, cp.t, C.compact_keys != 0);
     CG_TRY(channel_scales(C, nchan, freq, cp.a));
-    cp.M = nrow * nchan;
     // bins with many points per start cell go through the MFMA path, added
     // on top of what the sparse kernel writes; it needs the points of a bin
     // ordered by start cell, which the sparse kernel alone is better without
-    // (same-cell neighbours in a wavefront collide on the same LDS words)
-    const bool dense = sizeof(T) == 4 && tn.dense_min > 0 && cp.M > 0 &&
+    // (same-cell neighbours in a wavefront collide on the same LDS words).
+    // It holds every plane of its footprint region, so not with slabs.
+    const bool dense = sizeof(T) == 4 && tn.dense_min > 0 && cp.M > 0 && !cp.slab &&
                        grid_dense_supported(grid_args(C, P, cp.t, cp.M, cp.a.cshift), P.conv.kw, eb);
     if (!dense) cp.a.cshift = 0;
     cp.dense_min = dense ? tn.dense_min : 0;
@@ -459,7 +504,7 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
     float4 *rec = nullptr;
     if (M > 0) {
         if constexpr (sizeof(T) == 4 && std::is_same<TW, float>::value) {
-            CG_TRY(records_scratch<T>(C, P, M, 0, rec));
+            CG_TRY(records_scratch<T>(C, slab_bytes(P, cp.slab, eb), M, 0, rec));
             if (rec) CG_TRY(sort_rows<TU>(C, uvw, M, a, t, (const float2 *)vis, wgt, rec));
         }
         if (!rec) CG_TRY(sort_rows<TU>(C, uvw, M, a, t));
@@ -478,60 +523,89 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
                                                     as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), nullptr,
                                                     C.stream)));
     CG_TRY(clk.mark(ST_GRID));
-    CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
+    CG_TRY(ensure(C.grid, slab_bytes(P, cp.slab, eb)));
     const uint8_t *rm = nullptr;
     CG_TRY(row_mask(C, t, P.conv.kw, P.nf1, P.nf2, rm));
     GridArgs g = grid_args(C, P, t, M, a.cshift);
     g.kr = key_rect(a);
     g.dense_min = cp.dense_min;
     CG_TRY(grid_masks<T>(C, g, rm, P.nf1, P.nf2));
-    CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
-                          as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
-    if (cp.dense_min > 0) {
-        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
-        CG_TRY(clk.mark(ST_COUNT + 1));
-        CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
-                                 as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
-                                 (cx_t<float> *)C.grid.p, g, P.conv.kw, C.stream));
-        CG_TRY(clk.mark(ST_COUNT + 2));
-        C.dense_timed = C.timing;
+    // correction factors and the quarter tables, for the image stage
+    WArgs w = make_wargs(P);
+    CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
+    CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
+    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    if (P.wstack) {
+        CG_TRY(ensure(C.zq, sizeof(double) * nq));
+        CG_TRY(ensure(C.phi3, sizeof(double) * nq));
     }
-    if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
-    CG_TRY(clk.mark(ST_FFT));
     // only the columns holding image pixels are transformed along n2, only
     // the rows points reach hold non-zero data, and only the frequencies of
     // image rows are kept along n2
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
     const int sa = P.N2 - P.N2 / 2, sb = P.nf2 - P.N2 / 2;
-    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
-    CG_TRY(clk.mark(ST_FFT2));
-    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, +1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
-                              C.band_fft ? P.band_len : 0));
-    if (C.stop_after == 2) return P.ier;
-    CG_TRY(clk.mark(ST_IMAGE));
-    CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
-    CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
-    WArgs w = make_wargs(P);
-    CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
-    CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
-    if (!P.wstack) {
-        // plain 2-D: the u/v deconvolution only (deconv_2d type 1)
-        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), dirty, as<double>(C.phi1),
-                                            as<double>(C.phi2), P.N1, P.N2, P.nf1, P.nf2, 1,
-                                            C.stream)));
-        CG_TRY(clk.mark(ST_COUNT));
-        return P.ier;
+    // slabs of planes [zb, zb + zn) one after the other through the grid
+    // buffer (one slab holding every plane without the out-of-core flow):
+    // grid, transform, and add the slab's part of the w-DFT to the image
+    const int per = cp.slab ? cp.slab : P.nf3;
+    const int nslab = (P.nf3 + per - 1) / per;
+    C.last_slab = cp.slab;
+    C.last_nslab = nslab;
+    for (int zb = 0; zb < P.nf3; zb += per) {
+        const int zn = std::min(per, P.nf3 - zb);
+        const bool last = zb + zn >= P.nf3;
+        if (cp.slab) {
+            g.zb = zb;
+            g.zn = zn;
+            g.nchunk = (zn + t.pc - 1) / t.pc;
+        }
+        CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
+                              as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
+        if (cp.dense_min > 0) {
+            CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
+            CG_TRY(clk.mark(ST_COUNT + 1));
+            CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
+                                     as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
+                                     (cx_t<float> *)C.grid.p, g, P.conv.kw, C.stream));
+            CG_TRY(clk.mark(ST_COUNT + 2));
+            C.dense_timed = C.timing;
+        }
+        if (C.stop_after == 1) return P.ier;
+        // with slabs, the whole loop is reported as the grid stage
+        if (!cp.slab) CG_TRY(clk.mark(ST_FFT));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), zn, P.nf2, P.nf1, +1, true,
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
+        if (!cp.slab) CG_TRY(clk.mark(ST_FFT2));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), zn, P.nf2, P.nf1, +1, false,
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
+                                  C.band_fft ? P.band_len : 0));
+        if (C.stop_after == 2) return P.ier;
+        if (!cp.slab) CG_TRY(clk.mark(ST_IMAGE));
+        if (zb == 0) {
+            CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
+            CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
+            if (P.wstack) CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
+        }
+        if (!P.wstack) {
+            // plain 2-D: the u/v deconvolution only (deconv_2d type 1)
+            CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), dirty, as<double>(C.phi1),
+                                                as<double>(C.phi2), P.N1, P.N2, P.nf1, P.nf2, 1,
+                                                C.stream)));
+        } else {
+            w.zb = zb;
+            w.zn = cp.slab ? zn : 0;
+            w.accumulate = zb > 0;
+            CG_TRY((launch_ms2dirty_epilogue<T, cx_t<T>>(as<cx_t<T>>(C.grid), as<double>(C.zq),
+                                                          as<double>(C.phi3), as<double>(C.phi1),
+                                                          as<double>(C.phi2), dirty, w, C.stream)));
+        }
+        if (last && cp.slab) {
+            CG_TRY(clk.mark(ST_FFT));
+            CG_TRY(clk.mark(ST_FFT2));
+            CG_TRY(clk.mark(ST_IMAGE));
+        }
     }
-    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
-    CG_TRY(ensure(C.zq, sizeof(double) * nq));
-    CG_TRY(ensure(C.phi3, sizeof(double) * nq));
-    CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
-    CG_TRY((launch_ms2dirty_epilogue<T, cx_t<T>>(as<cx_t<T>>(C.grid), as<double>(C.zq),
-                                                  as<double>(C.phi3), as<double>(C.phi1),
-                                                  as<double>(C.phi2), dirty, w, C.stream)));
     CG_TRY(clk.mark(ST_COUNT));
     return P.ier;
 }
@@ -564,11 +638,13 @@ static int plan_dirty2ms(Ctx &C, CallPlan &cp, int64_t nrow, int nchan, const do
     if (cp.ier >= 2) return cp.ier;
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = P.conv.kw;
-    cp.t = choose_tiling(P.nf1, P.nf2, P.nf3, kw, eb, tn.lds, tn.tile_degrid, kw - 1);
+    cp.M = nrow * nchan;
+    CG_TRY(plan_slab(C, P, eb, cp.M, cp.slab));
+    cp.t = choose_tiling(P.nf1, P.nf2, cp.slab ? cp.slab : P.nf3, kw, eb, tn.lds, tn.tile_degrid,
+                         kw - 1);
     cp.a = make_coord_args(P, cp.t);
     set_key_rect(cp.a, P, cp.t, C.compact_keys != 0);
     CG_TRY(channel_scales(C, nchan, freq, cp.a));
-    cp.M = nrow * nchan;
     cp.dense_min = 0;
     return cp.ier;
 }
@@ -593,55 +669,80 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
     CG_TRY(ensure(C.geo, sizeof(geo_t<T>) * M));
     CG_TRY(ensure(C.val, sizeof(cx_t<T>) * M));
     float4 *rec = nullptr;
-    CG_TRY(records_scratch<T>(C, P, M, 1, rec));
+    CG_TRY(records_scratch<T>(C, slab_bytes(P, cp.slab, eb), M, 1, rec));
     CG_TRY((launch_records<T, TU, cx_t<T>, TW>(uvw, (const cx_t<T> *)nullptr, wgt,
                                                 as<int32_t>(C.vals), M, a, 1, as<geo_t<T>>(C.geo),
                                                 as<cx_t<T>>(C.val), rec, C.stream)));
-    CG_TRY(clk.mark(ST_IMAGE));
     CG_TRY(ensure(C.phi1, sizeof(double) * (P.nf1 / 2 + 1)));
     CG_TRY(ensure(C.phi2, sizeof(double) * (P.nf2 / 2 + 1)));
-    CG_TRY(ensure(C.grid, (size_t)P.grid_cells() * eb));
+    CG_TRY(ensure(C.grid, slab_bytes(P, cp.slab, eb)));
     WArgs w = make_wargs(P);
-    CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
-    CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
-    if (!P.wstack) {
-        // plain 2-D: the u/v deconvolution only (deconv_2d type 2)
-        CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), const_cast<cx_t<T> *>(dirty),
-                                            as<double>(C.phi1), as<double>(C.phi2), P.N1, P.N2,
-                                            P.nf1, P.nf2, 2, C.stream)));
-    } else {
-        const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    const int64_t nq = (int64_t)(P.N1 / 2 + 1) * (P.N2 / 2 + 1);
+    if (P.wstack) {
         CG_TRY(ensure(C.zq, sizeof(double) * nq));
         CG_TRY(ensure(C.phi3, sizeof(double) * nq));
-        CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
-        CG_TRY((launch_dirty2ms_prologue<T, cx_t<T>>(dirty, as<double>(C.zq), as<double>(C.phi3),
-                                                      as<double>(C.phi1), as<double>(C.phi2),
-                                                      as<cx_t<T>>(C.grid), w, C.stream)));
     }
-    if (C.stop_after == 1) return P.ier;
     CG_TRY(twiddles<T>(C, P.nf1, P.nf2));
     // the degridder reads only the rows its tiles' points reach
     const uint8_t *rm = nullptr;
     if (M > 0) CG_TRY(row_mask(C, t, kw, P.nf1, P.nf2, rm));
+    GridArgs g = grid_args(C, P, t, M, a.cshift);
+    g.kr = key_rect(a);
     // columns first (only those holding image pixels; frequencies outside
     // the image rows are zero), then the rows the degridder reads
     const int la = P.N1 - P.N1 / 2, lb = P.nf1 - P.N1 / 2;
     const int sa = P.N2 - P.N2 / 2, sb = P.nf2 - P.N2 / 2;
-    CG_TRY(clk.mark(ST_FFT2));
-    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, false,
-                              as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
-                              C.band_fft ? P.band_len : 0));
-    CG_TRY(clk.mark(ST_FFT));
-    CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), P.nf3, P.nf2, P.nf1, -1, true,
-                              as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
-    if (C.stop_after == 2) return P.ier;
-    CG_TRY(clk.mark(ST_GRID));
-    GridArgs g = grid_args(C, P, t, M, a.cshift);
-    g.kr = key_rect(a);
-    if (M > 0)
-        CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
-                                          as<int32_t>(C.vals), as<int32_t>(C.tstart),
-                                          as<cx_t<T>>(C.grid), vis, g, kw, C.stream)));
+    // slabs of planes [zb, zb + zn) one after the other through the grid
+    // buffer (one slab holding every plane without the out-of-core flow):
+    // w-IDFT onto the slab, transform, and add the slab's taps to vis
+    const int per = cp.slab ? cp.slab : P.nf3;
+    C.last_slab = cp.slab;
+    C.last_nslab = (P.nf3 + per - 1) / per;
+    CG_TRY(clk.mark(ST_IMAGE));
+    CG_TRY(launch_fseries_grid(w.qd, P.nf1, as<double>(C.phi1), C.stream));
+    CG_TRY(launch_fseries_grid(w.qd, P.nf2, as<double>(C.phi2), C.stream));
+    if (P.wstack) CG_TRY(launch_quarter_table(w, as<double>(C.zq), as<double>(C.phi3), C.stream));
+    if (cp.slab) {
+        // the whole loop is reported as the grid stage
+        CG_TRY(clk.mark(ST_FFT2));
+        CG_TRY(clk.mark(ST_FFT));
+        CG_TRY(clk.mark(ST_GRID));
+    }
+    for (int zb = 0; zb < P.nf3; zb += per) {
+        const int zn = std::min(per, P.nf3 - zb);
+        if (!P.wstack) {
+            // plain 2-D: the u/v deconvolution only (deconv_2d type 2)
+            CG_TRY((launch_deconv2d<T, cx_t<T>>(as<cx_t<T>>(C.grid), const_cast<cx_t<T> *>(dirty),
+                                                as<double>(C.phi1), as<double>(C.phi2), P.N1, P.N2,
+                                                P.nf1, P.nf2, 2, C.stream)));
+        } else {
+            w.zb = zb;
+            w.zn = cp.slab ? zn : 0;
+            CG_TRY((launch_dirty2ms_prologue<T, cx_t<T>>(dirty, as<double>(C.zq), as<double>(C.phi3),
+                                                          as<double>(C.phi1), as<double>(C.phi2),
+                                                          as<cx_t<T>>(C.grid), w, C.stream)));
+        }
+        if (C.stop_after == 1) return P.ier;
+        if (!cp.slab) CG_TRY(clk.mark(ST_FFT2));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), zn, P.nf2, P.nf1, -1, false,
+                                  as<cx_t<T>>(C.tw2), la, lb, rm, sa, sb, C.stream, P.band_s,
+                                  C.band_fft ? P.band_len : 0));
+        if (!cp.slab) CG_TRY(clk.mark(ST_FFT));
+        CG_TRY(launch_fft_pass<T>(as<cx_t<T>>(C.grid), zn, P.nf2, P.nf1, -1, true,
+                                  as<cx_t<T>>(C.tw1), la, lb, rm, sa, sb, C.stream));
+        if (C.stop_after == 2) return P.ier;
+        if (!cp.slab) CG_TRY(clk.mark(ST_GRID));
+        if (cp.slab) {
+            g.zb = zb;
+            g.zn = zn;
+            g.nchunk = (zn + t.pc - 1) / t.pc;
+            g.accumulate = zb > 0;
+        }
+        if (M > 0)
+            CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
+                                              as<int32_t>(C.vals), as<int32_t>(C.tstart),
+                                              as<cx_t<T>>(C.grid), vis, g, kw, C.stream)));
+    }
     CG_TRY(clk.mark(ST_COUNT));
     return P.ier;
 }
@@ -1361,6 +1462,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
     else if (!std::strcmp(name, "row_mask")) C->row_mask = value;
     else if (!std::strcmp(name, "use_learned")) C->use_learned = value;
+    else if (!std::strcmp(name, "grid_limit_mb")) C->grid_limit_mb = value;
     else return CG_ERR_ARG;
     return CG_OK;
 }
@@ -1403,6 +1505,17 @@ int cg_ctx_last_plan(void *ctx, double *out)
                     P.o_half_width, P.gamma, P.h, (double)C->last_tiling.tx,
                     (double)C->last_tiling.ty};
     std::memcpy(out, v, sizeof(v));
+    return CG_OK;
+}
+
+// out[0] = w-planes per slab of the last call's grid buffer (0: the whole
+// grid at once), out[1] = slabs
+int cg_ctx_last_slab(void *ctx, int *out)
+{
+    Ctx *C = (Ctx *)ctx;
+    if (!C || !out) return CG_ERR_ARG;
+    out[0] = C->last_slab;
+    out[1] = C->last_nslab;
     return CG_OK;
 }
 

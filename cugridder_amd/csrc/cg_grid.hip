@@ -170,8 +170,10 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
     const int chunk = wi % g.nchunk;
     const int tile = g.tile_list ? g.tile_list[1 + wi / g.nchunk] : wi / g.nchunk;
     const int bx = tile % g.ntx, by = tile / g.ntx;
-    const int p0 = chunk * g.pc;
-    const int np = min(g.pc, g.nf3 - p0);
+    // planes [p0, p0 + np) of the slab [zb, zb + zn) held by the buffer
+    const int zn = slab_planes(g);
+    const int p0 = g.zb + chunk * g.pc;
+    const int np = min(g.pc, g.zb + zn - p0);
     const int cells = np * g.ty * g.tx;
     for (int k = threadIdx.x; k < 2 * cells; k += blockDim.x) acc[k] = 0.0;
     // source bins whose footprints can reach this tile: bin (bx - sx, by - sy)
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(1024) void k_grid(const geo_t<T> *__restrict__ geo,
         int lz = k / plane_cells;
         int rem = k - lz * plane_cells;
         int y = rem / g.tx, x = rem - y * g.tx;
-        size_t gi = ((size_t)(p0 + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
+        size_t gi = ((size_t)(p0 - g.zb + lz) * g.nf2 + (size_t)(by * g.ty + y)) * g.nf1 + bx * g.tx + x;
         fw[gi] = mkc((T)acc[k], (T)acc[cells + k]);
     }
     __syncthreads();  // acc and the bin table are reused by the next item
@@ -833,12 +835,14 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
             if (gx >= g.nf1) gx -= g.nf1;
             int gy = y0 + y;
             if (gy >= g.nf2) gy -= g.nf2;
-            lds[(lz * hy + y) * hxs + x] = fw[((size_t)(p0 + lz) * g.nf2 + gy) * g.nf1 + gx];
+            lds[(lz * hy + y) * hxs + x] = fw[((size_t)(p0 - g.zb + lz) * g.nf2 + gy) * g.nf1 + gx];
         }
     };
 
+    // the buffer holds planes [zb, zb + zn); stage() takes absolute planes
+    const int zn = slab_planes(g);
     if (single_stage) {
-        stage(0, g.nf3);
+        stage(g.zb, zn);
         __syncthreads();
     }
     for (int base = beg; base < end; base += (int)blockDim.x * PPT) {
@@ -846,8 +850,8 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
 #pragma unroll
         for (int k = 0; k < PPT; k++) ar[k] = ai[k] = (T)0;
         for (int c = 0; c < g.nchunk; c++) {
-            const int p0 = c * g.pc;
-            const int np = min(g.pc, g.nf3 - p0);
+            const int p0 = g.zb + c * g.pc;
+            const int np = min(g.pc, g.zb + zn - p0);
             if (!single_stage) {
                 __syncthreads();
                 stage(p0, np);
@@ -901,8 +905,15 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
             if (i >= end) continue;
             const cx_t<T> f = fac[i];
             const int j = perm[i];
-            if (j >= 0 && j < g.m)  // a valid permutation always passes
-                out[j] = mkc(ar[k] * f.x - ai[k] * f.y, ar[k] * f.y + ai[k] * f.x);
+            if (j >= 0 && j < g.m) {  // a valid permutation always passes
+                cx_t<T> r = mkc(ar[k] * f.x - ai[k] * f.y, ar[k] * f.y + ai[k] * f.x);
+                if (g.accumulate) {
+                    const cx_t<T> prev = out[j];
+                    r.x += prev.x;
+                    r.y += prev.y;
+                }
+                out[j] = r;
+            }
         }
     }
 }

@@ -158,7 +158,15 @@ struct GridArgs {
     // 1: complex64 one-plane gridding may run as per-quadrant MFMA products
     // (grid2d_mfma_supported decides); 0: always the LDS-atomic kernel
     int mfma2d;
+    // w-plane slab held by the grid buffer: planes zb .. zb + zn - 1 of the
+    // nf3 (periodic) planes sit at buffer planes 0 .. zn - 1 (zn = 0: the
+    // whole grid, zb = 0).  Gridding writes only the slab; degridding adds
+    // only the slab's taps, and with accumulate != 0 adds its result to
+    // the output instead of storing it.
+    int zb, zn, accumulate;
 };
+// planes held by the grid buffer
+__host__ __device__ inline int slab_planes(const GridArgs &g) { return g.zn > 0 ? g.zn : g.nf3; }
 bool grid2d_mfma_supported(const GridArgs &g, int kw);
 template <typename T>
 int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
@@ -236,6 +244,10 @@ struct WArgs {
     int N1, N2, nf1, nf2, nf3;
     int cna, cnb;  // split of the column axis (0: natural order)
     int skip_dead; // 1: the prologue leaves spectrum rows outside the image unwritten
+    // w-plane slab of the grid buffer (planes zb .. zb + zn - 1 at buffer
+    // planes 0 .. zn - 1; zn = 0: every plane); the epilogue adds its part
+    // of the w-DFT to the image when accumulate != 0
+    int zb, zn, accumulate;
     double pix_x, pix_y, o_center, i_center, gamma, h;
     Quad qd;
 };

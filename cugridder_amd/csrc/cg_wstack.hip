@@ -148,14 +148,15 @@ __global__ __launch_bounds__(256) void k_ms2dirty_epi(const cx_t<T> *__restrict_
     const size_t plane = (size_t)w.nf1 * w.nf2;
     const cx_t<T> *src = fw + (size_t)spec_row(w2, w.cna, w.cnb) * w.nf1 + w1;
     const double zp = zq[q];
-    // e^{i z' (p - P/2)}, FP64 recurrence
+    // e^{i z' (p - P/2)} over the slab's planes p, FP64 recurrence
+    const int zn = w.zn > 0 ? w.zn : w.nf3;
     double sv, cv;
     sincos(zp, &sv, &cv);
     const double2 step = make_double2(cv, sv);
-    sincos(-zp * (w.nf3 / 2), &sv, &cv);
+    sincos(zp * (w.zb - w.nf3 / 2), &sv, &cv);
     double2 ph = make_double2(cv, sv);
     double2 acc = make_double2(0.0, 0.0);
-    for (int p = 0; p < w.nf3; p++) {
+    for (int p = 0; p < zn; p++) {
         const double2 f = to_d2(src[plane * p]);
         acc.x += f.x * ph.x - f.y * ph.y;
         acc.y += f.x * ph.y + f.y * ph.x;
@@ -167,7 +168,13 @@ __global__ __launch_bounds__(256) void k_ms2dirty_epi(const cx_t<T> *__restrict_
     sincos((nm1 - w.o_center) * w.i_center, &sv, &cv);
     const double scale = 1.0 / (phi1[abs(dc)] * phi2[abs(dr)] * phi3[q] * n);
     acc = zmul(acc, make_double2(cv, sv));
-    dirty[idx] = from_d2<TV>(make_double2(acc.x * scale, acc.y * scale));
+    double2 r = make_double2(acc.x * scale, acc.y * scale);
+    if (w.accumulate) {
+        const double2 prev = to_d2(dirty[idx]);
+        r.x += prev.x;
+        r.y += prev.y;
+    }
+    dirty[idx] = from_d2<TV>(r);
 }
 
 template <typename T, typename TV>
@@ -207,11 +214,12 @@ __global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dir
     live_col(cc, w.N1, w.nf1, w1, col);
     const int row = cell_to_img(spec_freq(r, w.cna, w.cnb), w.N2, w.nf2);
     cx_t<T> *dst = fw + (size_t)r * w.nf1 + w1;
+    const int zn = w.zn > 0 ? w.zn : w.nf3;
     if (row < 0) {
         // a frequency outside the image rows: zero, unless the column FFT
         // treats it as zero without reading it
         if (!w.skip_dead)
-            for (int p = 0; p < w.nf3; p++) dst[plane * p] = mkc((T)0, (T)0);
+            for (int p = 0; p < zn; p++) dst[plane * p] = mkc((T)0, (T)0);
         return;
     }
     const int dc = col - w.N1 / 2, dr = row - w.N2 / 2;
@@ -227,9 +235,9 @@ __global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dir
     const double zp = zq[q];
     sincos(-zp, &sv, &cv);
     const double2 step = make_double2(cv, sv);
-    sincos(zp * (w.nf3 / 2), &sv, &cv);
+    sincos(-zp * (w.zb - w.nf3 / 2), &sv, &cv);
     double2 ph = make_double2(cv, sv);
-    for (int p = 0; p < w.nf3; p++) {
+    for (int p = 0; p < zn; p++) {
         const double2 v = zmul(f, ph);
         dst[plane * p] = mkc((T)v.x, (T)v.y);
         ph = zmul(ph, step);

@@ -157,6 +157,11 @@ int cg_ctx_destroy(void *ctx);
  * rows points reach fit in 1024 / 2048 rows, see cg_ctx_last_band),
  * "compact_keys" (1: sort keys number only the tiles inside the u / v
  * bands: the same permutation with fewer radix passes),
+ * "grid_limit_mb" (the largest W-stacking grid buffer in MiB, 0 = 90% of
+ * the free device memory; a larger grid is processed in slabs of w-planes
+ * through one buffer, the out-of-core flow of the reference's mem_limit
+ * pipeline, src/astro/ra_exec.cu:292-389, 520-613; see cg_ctx_last_slab;
+ * the host entry points read it from the environment, CG_GRID_LIMIT_MB),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
 /* per-stage times summed over calls, out[15] = {ms2dirty[7], dirty2ms[7],
@@ -174,6 +179,9 @@ int cg_ctx_last_plan(void *ctx, double *out);
  * or 2048 rows the complex64 column passes run band-limited: the input is
  * read and the spectrum written once, no intermediate in HBM. */
 int cg_ctx_last_band(void *ctx, int *out);
+/* out[0] = w-planes per slab of the last call's grid buffer (0: the whole
+ * grid at once), out[1] = the number of slabs */
+int cg_ctx_last_slab(void *ctx, int *out);
 
 /* prec 0: complex64 vis/image, FP32 gridding and FFT; prec 1: complex128.
  * uvw_prec, wgt_prec: 0 float, 1 double (prec 1 requires both 1).  Supported
