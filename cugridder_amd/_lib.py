@@ -142,6 +142,8 @@ def lib():
                                      c_void_p, c_double, c_double, c_int, c_int]
     L.cg_nufft2d_dev.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_int, c_void_p, c_double, c_double]
+    L.cg_nufft2d_noalloc.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_int, c_int, c_void_p, c_double, c_double]
     L.cg_batch_create.argtypes = [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int,
                                   ctypes.POINTER(BatchItem), c_double, c_double, c_int, c_int]
     L.cg_batch_run.argtypes = [c_void_p]
@@ -167,7 +169,7 @@ def lib():
                  "cg_ms2dirty_mc_dev", "cg_dirty2ms_mc_dev", "cg_ms2dirty_px_dev",
                  "cg_dirty2ms_px_dev", "cg_batch_create", "cg_batch_run",
                  "cg_batch_info", "cg_batch_destroy",
-                 "cg_nufft2d_dev", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
+                 "cg_nufft2d_dev", "cg_nufft2d_noalloc", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
                  "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split",
                  "cg_version", "cg_learn_nparams", "cg_learn_grad", "cg_learn_train",
                  "cg_learn_eval", "cg_learn_tabulate", "cg_ctx_set_kernel"):

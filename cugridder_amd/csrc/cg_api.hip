@@ -31,9 +31,17 @@ void report_hip_error(hipError_t e, const char *what, const char *file, int line
                  (int)e, file, line, what);
 }
 
+// A fixed device region that a context's buffers are carved from instead of
+// hipMalloc (the allocation-free entry point); reset at the start of a call.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, off = 0;
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr;  // non-null: carved from the arena, never freed
 };
 
 enum Stage { ST_PLAN = 0, ST_SORT, ST_RECORDS, ST_GRID, ST_FFT, ST_FFT2, ST_IMAGE, ST_COUNT };
@@ -88,23 +96,68 @@ struct Ctx {
         LearnedKernel K;
     } learned[LEARN_MAXKW + 1];
     int use_learned = 1;
+    // set by use_arena(): every buffer is carved from this region
+    Arena arena;
+    bool arena_mode = false;
 
-    ~Ctx()
+    template <typename F>
+    void each_buf(F f)
     {
         DevBuf *bufs[] = {&keys, &keys2, &vals, &vals2, &hist, &scan, &tstart, &geo, &val,
                           &grid, &zq, &phi3, &phi1, &phi2, &part, &tw1, &tw2, &fftmp, &dlist,
                           &rmask, &chans, &tlist};
-        for (DevBuf *b : bufs)
-            if (b->p) (void)hipFree(b->p);
+        for (DevBuf *b : bufs) f(*b);
+    }
+    // switches this (fresh) context to carving its buffers from [base,
+    // base + cap); nothing is allocated or freed from then on
+    void use_arena(void *base, size_t cap)
+    {
+        arena.base = (char *)base;
+        arena.cap = cap;
+        arena.off = 0;
+        arena_mode = true;
+        each_buf([this](DevBuf &b) { b.arena = &arena; });
+    }
+    // forgets every carved buffer (their contents included, so cached
+    // twiddle tables are rebuilt): the next call carves from offset 0
+    void reset_arena()
+    {
+        arena.off = 0;
+        each_buf([](DevBuf &b) {
+            b.p = nullptr;
+            b.bytes = 0;
+        });
+        tw1_n = tw2_n = tw_bytes = 0;
+    }
+
+    ~Ctx()
+    {
+        each_buf([](DevBuf &b) {
+            if (b.p && !b.arena) (void)hipFree(b.p);
+        });
         if (ev_ok)
             for (auto &e : ev) (void)hipEventDestroy(e);
     }
 };
 
+constexpr size_t kArenaAlign = 256;
+inline size_t arena_round(size_t bytes) { return (bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign; }
+
 static int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (b.bytes >= bytes) return CG_OK;
+    if (b.arena) {
+        // a buffer that grows within a call takes a new region; the old one
+        // stays unused until the arena is reset
+        Arena &A = *b.arena;
+        const size_t need = arena_round(bytes);
+        if (A.off > A.cap || need > A.cap - A.off) return CG_ERR_SIZE;
+        b.p = A.base + A.off;
+        b.bytes = need;
+        A.off += need;
+        return CG_OK;
+    }
     if (b.p) {
         CG_HIP(hipFree(b.p));
         b.p = nullptr;
@@ -1283,6 +1336,53 @@ static int host_nufft2d(int type, int M, const TU *x, const TU *y, void *c, int 
     }
 }
 
+// ---- allocation-free 2-D NUFFT
+//
+// The same pipeline as cg_nufft2d_dev, with every workspace buffer carved
+// from `scratch`, a fixed device array of this library (one copy per GPU, held
+// until the process ends), so the call makes no hipMalloc / hipFree and no
+// other allocating call: usable under stream capture or where allocation is
+// not allowed.  The host side keeps one context per device whose buffers point
+// into that device's copy of the array.  Calls that overlap on one GPU (other
+// threads, other streams) race on the array; calls are serialised on the host
+// per device, not on the GPU.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct NoAllocDevice {
+    std::mutex m;
+    void *ptr = nullptr;  // device address of this GPU's copy of `scratch`
+    Ctx *ctx = nullptr;
+};
+constexpr int kMaxDevices = 64;
+static NoAllocDevice g_noalloc[kMaxDevices];
+
+// An upper bound of the bytes nufft2d_dev carves for one call: every buffer
+// it may ensure, each rounded to the arena alignment.
+static size_t nufft2d_workspace(int type, int prec, int64_t M, int N1, int N2, double eps,
+                                double sigma, int grid_lds, int lds, int tile2d, int tile_degrid)
+{
+    ConvOpts conv;
+    if (setup_conv_opts(eps, sigma, prec == 0, conv) >= 2) return 0;
+    const int nf1 = num_cells(N1, sigma, conv.kw), nf2 = num_cells(N2, sigma, conv.kw);
+    const size_t eb = prec == 0 ? 8 : 16, gb = prec == 0 ? 16 : 32;
+    const int kw = conv.kw;
+    const Tiling t = type == 1 ? choose_tiling(nf1, nf2, 1, kw, 16, grid_lds, tile2d, 0)
+                               : choose_tiling(nf1, nf2, 1, kw, (int)eb, lds, tile_degrid, kw - 1);
+    const int64_t ntiles = t.ntiles();
+    const int64_t hs = radix_hist_size(std::max<int64_t>(M, 1));
+    size_t b = 0;
+    b += arena_round(4 * (size_t)(ntiles + 1));                                       // tstart
+    b += arena_round(std::max((size_t)nf1 * nf2 * eb, (size_t)32 * M));              // grid / records
+    b += 4 * arena_round(4 * (size_t)std::max<int64_t>(M, 64));                       // keys, vals
+    b += arena_round(4 * (size_t)hs) + arena_round(4 * (size_t)scan_tmp_size(hs));   // hist, scan
+    b += arena_round(gb * (size_t)std::max<int64_t>(M, 16)) + arena_round(eb * (size_t)std::max<int64_t>(M, 32));
+    b += arena_round(8 * (size_t)(nf1 / 2 + 1)) + arena_round(8 * (size_t)(nf2 / 2 + 1));  // phi
+    b += arena_round(eb * (size_t)nf1) + arena_round(eb * (size_t)nf2);               // twiddles
+    b += arena_round((size_t)nf1 + nf2) + arena_round(4 * (size_t)(ntiles + 1));     // masks, tile list
+    return b;
+}
+
 }  // namespace cg
 
 using namespace cg;
@@ -1789,6 +1889,54 @@ int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x, const 
                                            (double2 *)d_c, N1, N2, (double2 *)d_fk, epsilon, sigma,
                                            false);
     return CG_ERR_ARG;
+}
+
+// cg_nufft2d_dev on the fixed workspace `scratch` (see above)
+int cg_nufft2d_noalloc(void *stream, int type, int prec, int M, const void *d_x,
+                       const void *d_y, void *d_c, int N1, int N2, void *d_fk, double epsilon,
+                       double sigma)
+{
+    if ((type != 1 && type != 2) || (prec != 0 && prec != 1) || M < 0 || N1 <= 0 || N2 <= 0)
+        return CG_ERR_ARG;
+    int dev = 0;
+    CG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return CG_ERR_ARG;
+    NoAllocDevice &D = g_noalloc[dev];
+    std::lock_guard<std::mutex> lock(D.m);
+    if (!D.ptr) {
+        // once per device, with the device current: the address of this
+        // GPU's copy of the array (the host-side name is not a device address)
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            report_hip_error(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+            (void)hipGetLastError();
+            return CG_ERR_HIP;
+        }
+        D.ptr = p;
+    }
+    if (!D.ctx) {
+        Ctx *C = new (std::nothrow) Ctx();
+        if (!C) return CG_ERR_ALLOC;
+        env_overrides(*C);
+        C->use_arena(D.ptr, kScratchBytes);
+        D.ctx = C;
+    }
+    Ctx &C = *D.ctx;
+    const size_t need = nufft2d_workspace(type, prec, M, N1, N2, epsilon, sigma, C.grid_lds_budget,
+                                          C.lds_budget, C.tile_grid2d, C.tile_degrid);
+    if (need == 0) return CG_ERR_ARG;
+    if (need > kScratchBytes) {
+        if (!std::getenv("CG_QUIET"))
+            std::fprintf(stderr,
+                         "curafft: cg_nufft2d_noalloc needs %zu bytes of workspace, more than its "
+                         "fixed %zu (use cg_nufft2d_dev)\n",
+                         need, kScratchBytes);
+        return CG_ERR_SIZE;
+    }
+    C.stream = (hipStream_t)stream;
+    C.reset_arena();
+    return cg_nufft2d_dev(&C, type, prec, M, d_x, d_y, d_c, N1, N2, d_fk, epsilon, sigma);
 }
 
 // ---- host-only planner query (no device calls): out[16] = nf1 nf2 nf3 kw

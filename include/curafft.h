@@ -259,6 +259,19 @@ int cg_nufft2d_dev(void *ctx, int type, int prec, int M, const void *d_x,
                    const void *d_y, void *d_c, int N1, int N2, void *d_fk,
                    double epsilon, double sigma);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cg_nufft2d_dev without a context, on `stream`, whose
+ * workspace is a fixed 64 MiB device array inside this library (one copy per
+ * GPU, held until the process ends).  It calls no hipMalloc / hipFree.  A
+ * call whose workspace does not fit returns 3 before launching anything
+ * (complex64 at 1024 x 1024 modes fits about 0.5 M points; use
+ * cg_nufft2d_dev beyond that).  Calls that overlap on one GPU race on the
+ * array; the array keeps the last call's contents. */
+int cg_nufft2d_noalloc(void *stream, int type, int prec, int M,
+                       const void *d_x, const void *d_y, void *d_c, int N1,
+                       int N2, void *d_fk, double epsilon, double sigma);
+
 /* Host-only planner (no device calls): out[16] = nf1 nf2 nf3 kw beta es_c
  * pix_x pix_y i_center i_half_width o_center o_half_width gamma h f_over_c
  * ier, given the range [wmin, wmax] of the scaled w = w * 2 pi f/c * sign. */
