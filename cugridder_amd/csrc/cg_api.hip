@@ -274,8 +274,11 @@ static int twiddles(Ctx &C, int n1, int n2)
 {
     const int eb = (int)sizeof(cx_t<T>);
     if (C.tw_bytes != eb || C.tw1_n != n1) {
-        CG_TRY(ensure(C.tw1, (size_t)n1 * eb));
+        // long rows also need the half-length table, stored after the first
+        const bool lng = fft_row_long<T>(n1);
+        CG_TRY(ensure(C.tw1, (size_t)(lng ? n1 + n1 / 2 : n1) * eb));
         CG_TRY(launch_twiddles<T>(as<cx_t<T>>(C.tw1), n1, C.stream));
+        if (lng) CG_TRY(launch_twiddles<T>(as<cx_t<T>>(C.tw1) + n1, n1 / 2, C.stream));
         C.tw1_n = n1;
     }
     if (C.tw_bytes != eb || C.tw2_n != n2) {

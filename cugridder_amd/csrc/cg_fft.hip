@@ -8,11 +8,7 @@
 // registers, barriers, then writes them back in place.
 //
 // Rows: persistent workgroups walk groups of whole rows (one 16384-point
-// fp32 row is 132 KB of padded LDS).  CG_FFT_PREFETCH=1 selects a variant
-// that prefetches the next group into registers while the current one is
-// transformed (the barriers between passes fence LDS only, so the prefetch
-// stays in flight); at 16 elements per thread it spills and measured slower,
-// so it is off by default.
+// fp32 row is 132 KB of padded LDS).
 //
 // Columns: a short column (n2 <= 128, or a length with no usable split) is
 // transformed whole by a workgroup holding a block of adjacent columns.  A
@@ -259,7 +255,7 @@ __device__ void lds_fft(cx_t<T> *buf, const FftShape &S, const cx_t<T> *tw)
 // Rows.  Group g holds rows [g nb, g nb + nb) of the nrows rows of all planes
 // (rows are contiguous across planes).  DIR > 0 stores only live outputs;
 // DIR < 0 reads only live inputs.
-template <typename T, int DIR, bool PF>
+template <typename T, int DIR>
 __global__ __launch_bounds__(1024) void k_fft_rows(cx_t<T> *data, int64_t nrows, int n1,
                                                    FftShape S, const cx_t<T> *__restrict__ tw,
                                                    Live live)
@@ -284,17 +280,14 @@ __global__ __launch_bounds__(1024) void k_fft_rows(cx_t<T> *data, int64_t nrows,
             pre[u] = (k < lim && (DIR > 0 || live(e))) ? src[k] : mkc((T)0, (T)0);
         }
     };
-    int64_t g = blockIdx.x;
-    if (PF && g < ngroups) load(g);
-    for (; g < ngroups; g += gridDim.x) {
-        if (!PF) load(g);
+    for (int64_t g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        load(g);
 #pragma unroll
         for (int u = 0; u < EPT; u++) {
             const int k = threadIdx.x + u * blockDim.x;
             if (k < E) buf[lds_pos(S, k)] = pre[u];
         }
         lds_barrier();
-        if (PF && g + gridDim.x < ngroups) load(g + gridDim.x);
         lds_fft<T, DIR>(buf, S, tw);
         cx_t<T> *dst = data + g * (int64_t)E;
         const int lim = count(g);
@@ -655,76 +648,9 @@ __global__ __launch_bounds__((1 << LOGN) / fft_ept<T>()) void k_fft_rows_ct(cx_t
     }
 }
 
-// one step of the split column FFT with sub-transform length L = 2^LOGL on
-// CB adjacent live columns (see k_fft_cols_step); tw2 is the length-n2 table
-template <typename T, int DIR, int LOGL, int CB>
-__global__ __launch_bounds__(CB * (1 << LOGL) / fft_ept<T>()) void k_fft_cols_step_ct(cx_t<T> *data, int n2, int n1, int se, int sj,
-                                                           int twmode, const cx_t<T> *__restrict__ tw2,
-                                                           Live live, const uint8_t *__restrict__ rowmask,
-                                                           int mode_in, int mode_out, int spec_a,
-                                                           int spec_b)
-{
-    constexpr int L = 1 << LOGL;
-    constexpr int EPT = fft_ept<T>();
-    constexpr int E = CB * L;
-    constexpr int NT = E / EPT;
-    constexpr int SH2 = 7;  // inter-step table: B = 128
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx_t<T> *buf = reinterpret_cast<cx_t<T> *>(smem_raw);
-    // sub-transform table (w_L^t = w_n2^(t n2/L), read with stride n2/L from
-    // the two-level n2 table) and the inter-step twiddles w_n2^(j e)
-    const TwTab<T> tab = load_twtab<T>(buf + E, tw2, n2, SH2);
-    lds_barrier();
-    const int j = blockIdx.y;
-    cx_t<T> *pl = data + (size_t)blockIdx.z * n2 * n1;
-    const int nlive = live.count(n1);
-    const int c0 = blockIdx.x * CB;
-    const size_t row0 = (size_t)j * sj;
-    const int tstride = n2 >> LOGL;
-    // a row is dead when it is known zero on input / not needed on output:
-    // natural-order rows (sj == 1) by the row mask, split-order rows
-    // (frequency j + na e, na = gridDim.y) outside [0, spec_a) u [spec_b, n2)
-    auto dead = [&](size_t row, int e) -> bool {
-        if (sj == 1) return rowmask && !rowmask[row];
-        const int k = j + (int)gridDim.y * e;
-        return k >= spec_a && k < spec_b;
-    };
-#pragma unroll
-    for (int u = 0; u < EPT; u++) {
-        const int k = threadIdx.x + u * NT;
-        const int e = k / CB, b = k % CB;
-        const int c = c0 + b;
-        const size_t row = row0 + (size_t)e * se;
-        const bool rd = c < nlive && !(mode_in && dead(row, e));
-        cx_t<T> v = rd ? pl[row * n1 + live.at(c)] : mkc((T)0, (T)0);
-        if (twmode == 2) {
-            cx_t<T> w = tab.get(j * e);
-            if (DIR < 0) w.y = -w.y;
-            v = cmul<T>(v, w);
-        }
-        buf[k] = v;
-    }
-    lds_barrier();
-    ct_fft<T, DIR, L, 1, CB, true, false, NT, ct_rmax<T>()>(buf, tab, tstride);
-#pragma unroll
-    for (int u = 0; u < EPT; u++) {
-        const int k = threadIdx.x + u * NT;
-        const int e = k / CB, b = k % CB;
-        const int c = c0 + b;
-        if (c >= nlive) continue;
-        const size_t row = row0 + (size_t)e * se;
-        if (mode_out && dead(row, e)) continue;
-        cx_t<T> v = buf[k];
-        if (twmode == 1) {
-            cx_t<T> w = tab.get(j * e);
-            if (DIR < 0) w.y = -w.y;
-            v = cmul<T>(v, w);
-        }
-        pl[row * n1 + live.at(c)] = v;
-    }
-}
-
-// The same step as a persistent kernel: a workgroup walks tiles (column
+// One step of the split column FFT with sub-transform length L = 2^LOGL on
+// CB adjacent live columns (see k_fft_cols_step; tw2 is the length-n2 table)
+// as a persistent kernel: a workgroup walks tiles (column
 // block x fastest, then sub-transform j, then plane) and loads the next
 // tile into registers while the current one is transformed in LDS and
 // stored, so the loads in flight are not limited to what fits in LDS.
@@ -1227,13 +1153,31 @@ static int row_lds_bytes(int n1, int nb)
     return (E + (E >> 5)) * (int)sizeof(cx_t<T>);
 }
 
+// a row transform held whole in one workgroup's LDS
 template <typename T>
-bool fft_supported(int n)
+static bool fft_row_whole(int n)
 {
     int rad[32];
     if (n < 1 || factor_radices(n, rad) < 0) return false;
-    if (n > 1024 * fft_ept<T>() || row_lds_bytes<T>(n, 1) > kLdsMax) return false;
-    return true;
+    return n <= 1024 * fft_ept<T>() && row_lds_bytes<T>(n, 1) <= kLdsMax;
+}
+
+template <typename T>
+bool fft_row_long(int n)
+{
+    // complex128 rows beyond one workgroup (n > 8192): one radix-2 step in
+    // HBM around two half-length whole-row transforms; the columns of such
+    // a grid go through the four-step split
+    int na, nb, rad[32];
+    return sizeof(T) == 8 && !fft_row_whole<T>(n) && n % 2 == 0 && n <= 16384 &&
+           fft_row_whole<T>(n / 2) && fft_col_split(n, na, nb) && factor_radices(na, rad) >= 0 &&
+           factor_radices(nb, rad) >= 0;
+}
+
+template <typename T>
+bool fft_supported(int n)
+{
+    return fft_row_whole<T>(n) || fft_row_long<T>(n);
 }
 
 static int log2_exact(int n)
@@ -1315,12 +1259,9 @@ static int cols_step_ct_cb(cx_t<T> *data, int batch, int n2, int n1, int se, int
         return 1;
     } else {
         const size_t lds = (size_t)(CB * L + 128 + (n2 + 127) / 128) * ELEM;
-        if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_ct<T, DIR, LOGL, CB>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const int nlive = live.count(n1);
         if constexpr (sizeof(T) == 4 && LOGL >= 5 && LOGL <= 8) {
-            if (fft_knob("CG_FFT_COL2", 1) && CB * L / 16 == NT) {
+            if (CB * L / 16 == NT) {
                 const size_t lds2 = (size_t)(CB * L + 128 + (n2 + 127) / 128) * ELEM;
                 if (lds2 > 65536)
                     CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_r2<DIR, LOGL, CB>,
@@ -1337,30 +1278,22 @@ static int cols_step_ct_cb(cx_t<T> *data, int batch, int n2, int n1, int se, int
                 return CG_OK;
             }
         }
-        if (fft_knob("CG_FFT_COLPF", 1)) {
-            if (lds > 65536)
-                CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_pf<T, DIR, LOGL, CB>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            const int64_t ntile = (int64_t)((nlive + CB - 1) / CB) * nsub * batch;
-            const int per_cu = std::max(1, std::min((int)(kLdsMax / lds), 2048 / NT));
-            const int64_t grid = std::min<int64_t>(ntile, (int64_t)256 * per_cu);
-            hipLaunchKernelGGL((k_fft_cols_step_pf<T, DIR, LOGL, CB>), dim3((unsigned)grid), dim3(NT),
-                               lds, s, data, n2, n1, se, sj, nsub, batch, twmode, tw2, live,
-                               dr.rowmask, dr.mode_in, dr.mode_out, dr.spec_a, dr.spec_b);
-            CG_HIP(hipGetLastError());
-            return CG_OK;
-        }
-        dim3 grid((unsigned)((nlive + CB - 1) / CB), (unsigned)nsub, (unsigned)batch);
-        hipLaunchKernelGGL((k_fft_cols_step_ct<T, DIR, LOGL, CB>), grid, dim3(NT), lds, s, data, n2,
-                           n1, se, sj, twmode, tw2, live, dr.rowmask, dr.mode_in, dr.mode_out,
-                           dr.spec_a, dr.spec_b);
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute((const void *)k_fft_cols_step_pf<T, DIR, LOGL, CB>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const int64_t ntile = (int64_t)((nlive + CB - 1) / CB) * nsub * batch;
+        const int per_cu = std::max(1, std::min((int)(kLdsMax / lds), 2048 / NT));
+        const int64_t grid = std::min<int64_t>(ntile, (int64_t)256 * per_cu);
+        hipLaunchKernelGGL((k_fft_cols_step_pf<T, DIR, LOGL, CB>), dim3((unsigned)grid), dim3(NT),
+                           lds, s, data, n2, n1, se, sj, nsub, batch, twmode, tw2, live,
+                           dr.rowmask, dr.mode_in, dr.mode_out, dr.spec_a, dr.spec_b);
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
 }
 
 // CB adjacent columns per workgroup: 512-byte row segments within 64 KB of
-// LDS by default; CG_FFT_CB = 16 / 32 selects narrower blocks
+// LDS
 template <typename T, int DIR, int LOGL>
 static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj, int nsub,
                         int twmode, const cx_t<T> *tw2, Live live, const DeadRows &dr, hipStream_t s)
@@ -1369,14 +1302,6 @@ static int cols_step_ct(cx_t<T> *data, int batch, int n2, int n1, int se, int sj
     constexpr int ELEM = (int)sizeof(cx_t<T>);
     constexpr int CB0 = 512 / ELEM;
     constexpr int CB = CB0 * L * ELEM <= 65536 ? CB0 : 65536 / (L * ELEM);
-    // (a narrower block only where it still has at least one wavefront)
-    const int want = fft_knob("CG_FFT_CB", 0);
-    if (want == 16 && CB > 16 && 16 * L / fft_ept<T>() >= 64)
-        return cols_step_ct_cb<T, DIR, LOGL, 16>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
-                                                 live, dr, s);
-    if (want == 32 && CB > 32 && 32 * L / fft_ept<T>() >= 64)
-        return cols_step_ct_cb<T, DIR, LOGL, 32>(data, batch, n2, n1, se, sj, nsub, twmode, tw2,
-                                                 live, dr, s);
     return cols_step_ct_cb<T, DIR, LOGL, CB>(data, batch, n2, n1, se, sj, nsub, twmode, tw2, live,
                                              dr, s);
 }
@@ -1396,11 +1321,103 @@ static int cols_step_ct_dispatch(cx_t<T> *data, int batch, int n2, int n1, int L
     }
 }
 
+// ---- long complex128 rows (n > 8192): X = DFT_n(x) as
+//   X[2m]     = DFT_{n/2}(x[i] + x[i + n/2])[m]
+//   X[2m + 1] = DFT_{n/2}((x[i] - x[i + n/2]) w^i)[m],  w = e^(dir 2 pi i / n),
+// i.e. one radix-2 decimation-in-frequency step in HBM, the two halves of
+// every row as rows of length n/2, and the even / odd outputs interleaved
+// back to natural order; the inverse pass runs the transposed steps
+// (de-interleave, half-length transforms, decimation-in-time combine).
+
+// forward: x[i], x[i + h] <- x[i] + x[i + h], (x[i] - x[i + h]) w^i;
+// inverse: x[i], x[i + h] <- y0 + y1 w^i, y0 - y1 w^i (w^i = conj(tw[i]))
+template <typename T, int DIR>
+__global__ __launch_bounds__(256) void k_rows_radix2(cx_t<T> *data, int64_t nrows, int n,
+                                                     const cx_t<T> *__restrict__ tw)
+{
+    const int h = n / 2;
+    const int64_t idx = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (idx >= nrows * h) return;
+    const int64_t row = idx / h;
+    const int i = (int)(idx - row * h);
+    cx_t<T> *x = data + row * n;
+    const cx_t<T> a = x[i], b = x[i + h];
+    cx_t<T> w = tw[i];
+    if (DIR < 0) w.y = -w.y;
+    if (DIR > 0) {
+        x[i] = mkc(a.x + b.x, a.y + b.y);
+        x[i + h] = cmul<T>(mkc(a.x - b.x, a.y - b.y), w);
+    } else {
+        const cx_t<T> t = cmul<T>(b, w);
+        x[i] = mkc(a.x + t.x, a.y + t.y);
+        x[i + h] = mkc(a.x - t.x, a.y - t.y);
+    }
+}
+
+// per row, in place through registers: DIR > 0 interleaves the halves
+// [Y0 | Y1] to X[2m] = Y0[m], X[2m + 1] = Y1[m]; DIR < 0 splits X into
+// [even | odd], reading only the live frequencies (the others are zero)
+template <typename T, int DIR>
+__global__ __launch_bounds__(1024) void k_rows_interleave(cx_t<T> *data, int64_t nrows, int n,
+                                                          Live live)
+{
+    constexpr int EPR = 16;  // n <= 16384
+    const int h = n / 2;
+    for (int64_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        cx_t<T> *x = data + row * n;
+        cx_t<T> v[EPR];
+#pragma unroll
+        for (int e = 0; e < EPR; e++) {
+            const int j = (int)threadIdx.x + 1024 * e;
+            v[e] = mkc((T)0, (T)0);
+            if (j < n && (DIR > 0 || live(j))) v[e] = x[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPR; e++) {
+            const int j = (int)threadIdx.x + 1024 * e;
+            if (j >= n) continue;
+            const int d = DIR > 0 ? (j < h ? 2 * j : 2 * (j - h) + 1) : ((j & 1) ? h + (j >> 1) : j >> 1);
+            x[d] = v[e];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int DIR>
+static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw, Live live,
+                    const uint8_t *rowmask, hipStream_t s);
+
+// tw holds the table of n1 followed by the table of n1 / 2
+template <typename T, int DIR>
+static int fft_rows_long(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw,
+                         Live live, hipStream_t s)
+{
+    const int64_t nrows = (int64_t)n2 * batch;
+    const int h = n1 / 2;
+    const unsigned nb2 = (unsigned)ceil_div(nrows * h, 256);
+    const unsigned nbi = (unsigned)std::min<int64_t>(nrows, 4096);
+    if (DIR > 0) {
+        hipLaunchKernelGGL((k_rows_radix2<T, DIR>), dim3(nb2), dim3(256), 0, s, data, nrows, n1, tw);
+        CG_HIP(hipGetLastError());
+        CG_TRY((fft_rows<T, DIR>(data, 2 * batch, n2, h, tw + n1, Live{h, h}, nullptr, s)));
+        hipLaunchKernelGGL((k_rows_interleave<T, DIR>), dim3(nbi), dim3(1024), 0, s, data, nrows, n1, live);
+    } else {
+        hipLaunchKernelGGL((k_rows_interleave<T, DIR>), dim3(nbi), dim3(1024), 0, s, data, nrows, n1, live);
+        CG_HIP(hipGetLastError());
+        CG_TRY((fft_rows<T, DIR>(data, 2 * batch, n2, h, tw + n1, Live{h, h}, nullptr, s)));
+        hipLaunchKernelGGL((k_rows_radix2<T, DIR>), dim3(nb2), dim3(256), 0, s, data, nrows, n1, tw);
+    }
+    CG_HIP(hipGetLastError());
+    return CG_OK;
+}
+
 template <typename T, int DIR>
 static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw, Live live,
                     const uint8_t *rowmask, hipStream_t s)
 {
-    if (fft_knob("CG_FFT_CT", 1)) {
+    if (fft_row_long<T>(n1)) return fft_rows_long<T, DIR>(data, batch, n2, n1, tw, live, s);
+    {
         const int r = rows_ct_dispatch<T, DIR>(data, (int64_t)n2 * batch, n1, tw, live, rowmask, n2, s);
         if (r != 1) return r;
     }
@@ -1417,26 +1434,18 @@ static int fft_rows(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw,
     S.nb = nb;
     S.lt = n1;
     S.le = 1;
-    S.pad = fft_knob("CG_FFT_PAD", 1) ? 5 : 0;
+    S.pad = 5;
     S.tws = 1;
     const int64_t nrows = (int64_t)n2 * batch;
     const int64_t ngroups = (nrows + nb - 1) / nb;
     int per_cu = std::max<int>(1, (int)(kLdsMax / lds));
     per_cu = std::min(per_cu, std::max(1, 2048 / threads));
     const int64_t grid = std::min<int64_t>(ngroups, (int64_t)256 * per_cu);
-    if (fft_knob("CG_FFT_PREFETCH", 0)) {
-        if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR, true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_fft_rows<T, DIR, true>), dim3((unsigned)grid), dim3(threads), lds, s,
-                           data, nrows, n1, S, tw, live);
-    } else {
-        if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR, false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_fft_rows<T, DIR, false>), dim3((unsigned)grid), dim3(threads), lds, s,
-                           data, nrows, n1, S, tw, live);
-    }
+    if (lds > 65536)
+        CG_HIP(hipFuncSetAttribute((const void *)k_fft_rows<T, DIR>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_fft_rows<T, DIR>), dim3((unsigned)grid), dim3(threads), lds, s, data,
+                       nrows, n1, S, tw, live);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
@@ -1456,7 +1465,7 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         const int M = band_len <= 1024 ? 1024 : 2048, CB = M == 1024 ? 16 : 8;
         const size_t lds = sizeof(float2) * ((size_t)M * CB + (1 << kBandTwSh) + (n2 >> kBandTwSh));
         if (band_len > 0 && band_len <= M && lds <= (size_t)kLdsMax && band_s >= 0 && band_s < n2 &&
-            fft_knob("CG_FFT_BAND", 1) && fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
+            fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
             n2 >= 2 * M && fft_col_split(n2, na, nb)) {
             int nblk = (nlive + CB - 1) / CB;
             if (M == 2048) nblk = (nblk + 15) / 16 * 16;  // whole groups of the block remap
@@ -1500,13 +1509,12 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         CG_HIP(hipGetLastError());
         return CG_OK;
     }
-    const bool ct = fft_knob("CG_FFT_CT", 1) != 0;
     // dead rows: forward, the natural-order step skips loading masked-out
     // rows and the split-order step skips storing dead frequencies; inverse,
     // the split-order step skips loading dead frequencies and the
     // natural-order step skips storing masked-out rows
     auto step = [&](int L, int se, int sj, int nsub, int twmode) -> int {
-        if (ct) {
+        {
             DeadRows d{rowmask, spec_a, spec_b, 0, 0};
             const bool natural = sj == 1;
             const bool active = natural ? rowmask != nullptr : spec_a < spec_b;
@@ -1578,7 +1586,7 @@ bool fft_rows_masked(int n1)
     // the conditions under which fft_rows takes rows_ct (which honours the
     // row and column masks)
     const int l = log2_exact(n1);
-    if (!fft_knob("CG_FFT_CT", 1) || l < 9 || l > 14) return false;
+    if (l < 9 || l > 14) return false;
     const int nt = n1 / fft_ept<T>();
     if (nt < 64 || nt > 1024) return false;
     const int sh = l / 2;
@@ -1589,7 +1597,7 @@ template <typename T>
 bool fft_spectrum_pruned(int n2)
 {
     int na, nb;
-    if (n2 <= 1 || !fft_knob("CG_FFT_CT", 1) || !fft_col_split(n2, na, nb)) return false;
+    if (n2 <= 1 || !fft_col_split(n2, na, nb)) return false;
     const int ln = log2_exact(na), lb = log2_exact(nb);
     return ln >= 4 && ln <= 8 && lb >= 4 && lb <= 8;
 }
@@ -1654,6 +1662,8 @@ template bool fft_spectrum_pruned<float>(int);
 template bool fft_spectrum_pruned<double>(int);
 template bool fft_supported<float>(int);
 template bool fft_supported<double>(int);
+template bool fft_row_long<float>(int);
+template bool fft_row_long<double>(int);
 template int launch_twiddles<float>(float2 *, int, hipStream_t);
 template int launch_twiddles<double>(double2 *, int, hipStream_t);
 

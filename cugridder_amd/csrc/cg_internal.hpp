@@ -226,6 +226,11 @@ bool fft_rows_masked(int n1);
 bool fft_col_split(int n, int &na, int &nb);
 template <typename T>
 bool fft_supported(int n);
+// complex128 rows too long for one workgroup, transformed with a radix-2
+// step around two half-length rows: their twiddle table (tw1) holds the
+// table of n followed by the table of n / 2
+template <typename T>
+bool fft_row_long(int n);
 template <typename T>
 int launch_twiddles(cx_t<T> *tw, int n, hipStream_t s);
 

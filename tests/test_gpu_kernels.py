@@ -91,15 +91,16 @@ def test_tile_keys_and_permutation_bit_exact(gpu, nrow, nx, ny, fov, eps, freq):
                                    (2, 2000, 12), (1, 1250, 54), (4, 16, 5000),
                                    # power-of-two rows and split columns (compile-time plans)
                                    (1, 2048, 64), (1, 16384, 32), (1, 4096, 512), (2, 256, 1024),
-                                   (1, 32768 // 64, 8192)])
+                                   (1, 32768 // 64, 8192),
+                                   # complex128 rows beyond one workgroup (radix-2 step
+                                   # around two half-length rows)
+                                   (1, 40, 11250), (2, 6, 16384)])
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_fft2d_vs_rocfft(gpu, shape, dtype):
     torch = gpu
     from cugridder_amd import Context
     b, n2, n1 = shape
     dt = getattr(torch, dtype)
-    if dt == torch.complex128 and max(n1, n2) > 8192:
-        pytest.skip("FP64 transforms are limited to 8192 points (LDS)")
     ctx = Context()
     torch.manual_seed(0)
     x = torch.randn(b, n2, n1, dtype=dt, device="cuda")

@@ -573,3 +573,54 @@ def test_batch_matches_single_calls(gpu, graph):
                 assert rel_l2(it["vis"].cpu().numpy(), ref_v[k]) < 2e-6, (rep, k)
     b1.close()
     b2.close()
+
+
+@pytest.mark.parametrize("nx,ny", [(48, 8192), (40, 5625)])
+def test_fp64_long_rows_vs_oracle(gpu, nx, ny):
+    """complex128 images whose fast grid axis (nf1 = 2 ny) is beyond one
+    workgroup's LDS (the radix-2 step around two half-length rows): both
+    directions against the FP64 restatement, and adjointness."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(ny)
+    nrow, fov, f0, eps = 3000, 1.0, 3e8, 1e-9
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = rng.normal(0, 200.0, (nrow, 2))
+    uvw[:, 2] = rng.uniform(-50, 50, nrow)
+    vis = rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)
+    img = rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))
+    ctx = Context()
+    try:
+        tu, tv, ti = (torch.from_numpy(a).cuda() for a in (uvw, vis, img))
+        d = ctx.ms2dirty(tu, tv, None, nx, ny, fov, f0, eps)
+        assert ctx.last_plan()["nf1"] > 8192
+        v = ctx.dirty2ms(tu, ti, None, fov, f0, eps)
+        d, v = d.cpu().numpy(), v.cpu().numpy()
+    finally:
+        ctx.close()
+    ref_d = O.ms2dirty(uvw, f0, vis, None, nx, ny, fov, eps)
+    ref_v = O.dirty2ms(uvw, f0, img, None, fov, eps)
+    assert rel_l2(d, ref_d) <= eps and rel_l2(v, ref_v) <= eps
+    lhs, rhs = np.vdot(vis, v), np.vdot(d, img)
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
+
+
+def test_fp64_nufft2d_long_rows_vs_oracle(gpu):
+    """complex128 2-D NUFFT with 8192 modes along the fast axis (nf1 = 16384)."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(77)
+    M, N1, N2, eps = 20000, 8192, 32, 1e-9
+    x, y = (a.astype(np.float64) for a in _clustered_xy(rng, M))
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    fk = rng.standard_normal((N2, N1)) + 1j * rng.standard_normal((N2, N1))
+    ctx = Context()
+    try:
+        tx, ty, tc, tf = (torch.from_numpy(a).cuda() for a in (x, y, c, fk))
+        f1 = ctx.nufft2d1(tx, ty, tc, N1, N2, eps).cpu().numpy()
+        assert ctx.last_plan()["nf1"] == 2 * N1
+        c2 = ctx.nufft2d2(tx, ty, tf, eps).cpu().numpy()
+    finally:
+        ctx.close()
+    assert rel_l2(f1, O.nufft2d1(x, y, c, N1, N2, eps)) <= eps
+    assert rel_l2(c2, O.nufft2d2(x, y, fk, eps)) <= eps
