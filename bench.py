@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 C_LIGHT = 299792458.0
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
+# points per bin from which the complex64 gridder / degridder use their MFMA
+# kernels (the library's defaults, set explicitly on the timed context)
+DENSE_MIN = 64
+DEGRID_DENSE_MIN = 2048
 METRIC = "Mvis/s ms2dirty+dirty2ms at eps 1e-5, plain vs batched/learned; HBM GB/s"
 
 
@@ -208,11 +212,12 @@ def grid_stats(ctx, plan, kw, dense_min):
         cols[(bx * tx + np.arange(tx + kw - 1)) % nf1] = True
     row_t = rows.reshape(nty, ty).any(1)
     col_t = cols.reshape(ntx, tx).any(1)
-    dense = int(cnt[cnt >= dense_min].sum()) if (dense_min > 0 and ctx.key_shift > 0) else 0
+    is_dense = (cnt >= dense_min) if (dense_min > 0 and ctx.key_shift > 0) else np.zeros_like(nz)
     band_s, band_len = ctx.last_band()
     return {"rows": int(rows.sum()), "cols": int(cols.sum()),
             "tiles_written": int(row_t.sum() * col_t.sum()), "tile": tx * ty,
-            "nonempty_tiles": int(nz.sum()), "dense_points": dense, "band": band_len}
+            "nonempty_tiles": int(nz.sum()), "dense_points": int(cnt[is_dense].sum()),
+            "dense_tiles": int(is_dense.sum()), "band": band_len}
 
 
 def band_kernel(nf2, band):
@@ -268,7 +273,12 @@ def kernel_table(W, plan, fw, bw, stage_ms):
         rows.append((col_kernel, "dirty2ms", 2, sb["fft_cols"],
                      (8 * P * (N2 * N1 + nf2 * N1) + 8 * P * (nf2 * N1 + Rb * N1)) // 2))
     rows.append(("k_fft_rows_ct", "dirty2ms", 1, sb["fft_rows"], 8 * P * Rb * (N1 + Cb)))
-    rows.append(("k_degrid", "dirty2ms", 1, sb["grid"], 36 * M + 8 * P * bw["tiles_written"] * bw["tile"]))
+    # the degridders read each point once and each cell of their tiles once
+    Mb, Tb = bw["dense_points"], bw["dense_tiles"]
+    rows.append(("k_degrid", "dirty2ms", 1, sb["grid"] - sb["grid_dense"],
+                 36 * (M - Mb) + 8 * P * max(bw["tiles_written"] - Tb, 0) * bw["tile"]))
+    if Mb:
+        rows.append(("k_degrid_dense", "dirty2ms", 1, sb["grid_dense"], 36 * Mb + 8 * P * Tb * bw["tile"]))
     return rows
 
 
@@ -456,7 +466,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("copy probe: 1 GiB read + 1 GiB written")
-    ctx = Context(timing=True)
+    ctx = Context(timing=True, dense_min=DENSE_MIN, degrid_dense_min=DEGRID_DENSE_MIN)
 
     learned_info = None
     if args.workload == "learned":
@@ -535,9 +545,9 @@ def main():
         # per-call geometry of each direction (untimed calls after the steps)
         ctx.ms2dirty(W["uvw"], W["vis"], W["wgt"], W["nx"], W["ny"], W["fov"], W["freq"], W["eps"],
                      out=out_d)
-        fw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], 64)
+        fw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], DENSE_MIN)
         ctx.dirty2ms(W["uvw"], W["img"], W["wgt"], W["fov"], W["freq"], W["eps"], out=out_v)
-        bw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], 0)
+        bw_stats = grid_stats(ctx, ctx.last_plan(), plan["kw"], DEGRID_DENSE_MIN)
         torch.cuda.synchronize()
         kernels = kernel_table(W, plan, fw_stats, bw_stats, stage_ms)
         # the dominant kernel: most time per step (a kernel name counts the
@@ -552,12 +562,13 @@ def main():
                 "launches_per_call": launches}
         if traffic:
             roof["traffic_GBs"] = round(traffic / (t_launch * 1e-3) / 1e9, 1)
-        if name in ("k_grid_dense", "k_grid", "k_degrid"):
+        if name in ("k_grid_dense", "k_grid", "k_degrid", "k_degrid_dense"):
             # useful arithmetic: kw^3 taps per point, a complex value times a
             # real kernel product (4 flop) per tap
-            pts = fw_stats["dense_points"] if name == "k_grid_dense" else W["M"]
-            if name == "k_grid":
-                pts = W["M"] - fw_stats["dense_points"]
+            pts = {"k_grid_dense": fw_stats["dense_points"],
+                   "k_grid": W["M"] - fw_stats["dense_points"],
+                   "k_degrid_dense": bw_stats["dense_points"],
+                   "k_degrid": W["M"] - bw_stats["dense_points"]}[name]
             kw = plan["kw"]
             useful = 4.0 * kw ** 3 * pts
             roof["useful_TFLOP_s"] = round(useful / (t_launch * 1e-3) / 1e12, 2)

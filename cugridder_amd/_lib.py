@@ -158,6 +158,7 @@ def lib():
     L.cg_sort_pairs_dev.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int]
     L.cg_memcpy_dev.argtypes = [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]
     L.cg_fft_col_split.argtypes = [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    L.cg_tap_poly.argtypes = [c_int, c_double, c_double, f32n, c_int, c_int, f32]
     # learned kernel
     L.cg_learn_grad.argtypes = [c_int, f32, f64, f64, c_int, f32, f64]
     L.cg_learn_train.argtypes = [c_int, c_int, f64, f64, c_int, c_int, c_int, c_double, f32, f64n]
@@ -170,7 +171,7 @@ def lib():
                  "cg_dirty2ms_px_dev", "cg_batch_create", "cg_batch_run",
                  "cg_batch_info", "cg_batch_destroy",
                  "cg_nufft2d_dev", "cg_nufft2d_noalloc", "cg_plan_query", "cg_ctx_grid", "cg_ctx_sorted",
-                 "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split",
+                 "cg_fft2d_dev", "cg_sort_pairs_dev", "cg_memcpy_dev", "cg_fft_col_split", "cg_tap_poly",
                  "cg_version", "cg_learn_nparams", "cg_learn_grad", "cg_learn_train",
                  "cg_learn_eval", "cg_learn_tabulate", "cg_ctx_set_kernel"):
         getattr(L, name).restype = c_int

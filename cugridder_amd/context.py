@@ -232,15 +232,16 @@ class Context:
     # ------------------------------------------------------------ introspection
     def stage_times(self):
         """{'ms2dirty': {stage: ms}, 'dirty2ms': {...}, 'calls': (n1, n2)} summed;
-        ms2dirty's 'grid_dense' is the MFMA kernel's part of its 'grid'."""
+        'grid_dense' is the MFMA kernel's part of each direction's 'grid'."""
         n = len(STAGES)
-        out = np.zeros(2 * n + 1)
+        out = np.zeros(2 * n + 2)
         calls = (ctypes.c_longlong * 2)()
         check(lib().cg_ctx_stage_times(self._h, out, calls), "stage_times")
         fw = dict(zip(STAGES, out[:n].tolist()))
         fw["grid_dense"] = float(out[2 * n])
-        return {"ms2dirty": fw, "dirty2ms": dict(zip(STAGES, out[n:2 * n].tolist())),
-                "calls": (calls[0], calls[1])}
+        bw = dict(zip(STAGES, out[n:2 * n].tolist()))
+        bw["grid_dense"] = float(out[2 * n + 1])
+        return {"ms2dirty": fw, "dirty2ms": bw, "calls": (calls[0], calls[1])}
 
     def reset_times(self):
         lib().cg_ctx_reset_times(self._h)

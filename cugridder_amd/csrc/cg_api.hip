@@ -55,6 +55,8 @@ struct Ctx {
     int mfma2d = 1;        // 2-D complex64 gridding as per-quadrant MFMA products
     int tile_degrid = 16;
     int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
+    int degrid_dense_min = 2048;  // points per degridding tile for the MFMA degridder (0: off)
+    int tap_poly = 0;        // complex64 MFMA kernels: kernel values from per-tap polynomials
     int row_mask = 1;        // let the FFT skip grid rows no point reaches
     int prerec = 1;          // complex64 records in two coalescing passes
     int band_fft = 1;        // band-limited column passes when the rows points reach allow
@@ -81,7 +83,7 @@ struct Ctx {
     hipEvent_t ev[ST_COUNT + 3] = {};
     bool ev_ok = false, dense_timed = false;
     double stage_ms[2][ST_COUNT] = {};
-    double dense_ms = 0;
+    double dense_ms = 0, degrid_dense_ms = 0;  // the MFMA kernels' parts of the grid stages
     int64_t stage_calls[2] = {0, 0};
     WPlan last_plan;
     Tiling last_tiling;
@@ -191,6 +193,7 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_COMPACT_KEYS")) C.compact_keys = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
     if (const char *e = std::getenv("CG_GRID_LIMIT_MB")) C.grid_limit_mb = std::atoi(e);
+    if (const char *e = std::getenv("CG_TAP_POLY")) C.tap_poly = std::atoi(e);
 }
 
 struct StageClock {
@@ -365,6 +368,13 @@ static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, in
     g.f_e1 = (float)std::exp(g.beta);
     g.f_e2 = (float)std::exp(2.0 * g.beta);
     g.e3 = std::exp(3.0 * g.beta);
+    // per-tap polynomials of the same normalised values (complex64 MFMA kernels)
+    g.tp_on = 0;
+    if (C.tap_poly && P.conv.kw <= 8) {
+        tap_polynomials(P.conv.kw, P.conv.beta, P.conv.es_c, g.learned ? g.kq : nullptr,
+                        LEARN_MAXDEG, 10, g.tp);
+        g.tp_on = 1;
+    }
     return g;
 }
 
@@ -794,10 +804,25 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
             g.nchunk = (zn + t.pc - 1) / t.pc;
             g.accumulate = zb > 0;
         }
+        // complex64: tiles with many points go to the MFMA degridder
+        g.dense_min = 0;
+        if (sizeof(T) == 4 && M > 0 && C.degrid_dense_min > 0 && degrid_dense_supported(g, kw))
+            g.dense_min = C.degrid_dense_min;
+        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                               as<int32_t>(C.vals), as<int32_t>(C.tstart),
-                                              as<cx_t<T>>(C.grid), vis, g, kw, C.stream)));
+                                              as<cx_t<T>>(C.grid), vis, g, kw,
+                                              as<int32_t>(C.dlist), C.stream)));
+        if (g.dense_min > 0) {
+            CG_TRY(clk.mark(ST_COUNT + 1));
+            C.dense_timed = C.timing && !cp.slab;
+            CG_TRY(launch_degrid_dense((const float4 *)C.geo.p, (const float2 *)C.val.p,
+                                       as<int32_t>(C.vals), as<uint32_t>(C.keys),
+                                       as<int32_t>(C.tstart), as<int32_t>(C.dlist),
+                                       (const float2 *)C.grid.p, (float2 *)vis, g, kw, C.stream));
+            CG_TRY(clk.mark(ST_COUNT + 2));
+        }
     }
     CG_TRY(clk.mark(ST_COUNT));
     return P.ier;
@@ -818,6 +843,12 @@ static int finish_dirty2ms_clock(Ctx &C)
     CG_HIP(hipEventElapsedTime(&ms[ST_FFT], C.ev[ST_FFT], C.ev[ST_GRID]));
     CG_HIP(hipEventElapsedTime(&ms[ST_GRID], C.ev[ST_GRID], C.ev[ST_COUNT]));
     for (int i = 0; i < ST_COUNT; i++) C.stage_ms[1][i] += ms[i];
+    if (C.dense_timed) {
+        float dm = 0.f;
+        CG_HIP(hipEventElapsedTime(&dm, C.ev[ST_COUNT + 1], C.ev[ST_COUNT + 2]));
+        C.degrid_dense_ms += dm;
+        C.dense_timed = false;
+    }
     C.stage_calls[1]++;
     return CG_OK;
 }
@@ -1211,7 +1242,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                               as<int32_t>(C.vals), as<int32_t>(C.tstart),
-                                              as<cx_t<T>>(C.grid), c, g, kw, C.stream)));
+                                              as<cx_t<T>>(C.grid), c, g, kw, nullptr, C.stream)));
     }
     CG_TRY(clk.mark(ST_COUNT));
     CG_TRY(clk.finish());
@@ -1563,6 +1594,8 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
+    else if (!std::strcmp(name, "degrid_dense_min")) C->degrid_dense_min = value;
+    else if (!std::strcmp(name, "tap_poly")) C->tap_poly = value;
     else if (!std::strcmp(name, "row_mask")) C->row_mask = value;
     else if (!std::strcmp(name, "use_learned")) C->use_learned = value;
     else if (!std::strcmp(name, "grid_limit_mb")) C->grid_limit_mb = value;
@@ -1579,6 +1612,7 @@ int cg_ctx_stage_times(void *ctx, double *out, long long *calls)
     for (int d = 0; d < 2; d++)
         for (int i = 0; i < ST_COUNT; i++) out[d * ST_COUNT + i] = C->stage_ms[d][i];
     out[2 * ST_COUNT] = C->dense_ms;
+    out[2 * ST_COUNT + 1] = C->degrid_dense_ms;
     if (calls) {
         calls[0] = C->stage_calls[0];
         calls[1] = C->stage_calls[1];
@@ -1591,7 +1625,7 @@ int cg_ctx_reset_times(void *ctx)
     Ctx *C = (Ctx *)ctx;
     if (!C) return CG_ERR_ARG;
     std::memset(C->stage_ms, 0, sizeof(C->stage_ms));
-    C->dense_ms = 0;
+    C->dense_ms = C->degrid_dense_ms = 0;
     C->stage_calls[0] = C->stage_calls[1] = 0;
     return CG_OK;
 }
@@ -2029,6 +2063,15 @@ int cg_memcpy_dev(void *dst, const void *src, void *stream, size_t bytes)
 {
     if (!bytes) return CG_OK;
     CG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return CG_OK;
+}
+
+// the per-tap polynomial coefficients the complex64 kernels use (host only)
+int cg_tap_poly(int kw, double beta, double c, const float *kq, int ndeg, int deg, float *out)
+{
+    if (kw < 1 || kw > 16 || deg < 0 || deg >= kTapStride || ndeg < 0 || ndeg > LEARN_MAXDEG || !out)
+        return CG_ERR_ARG;
+    tap_polynomials(kw, beta, c, kq, ndeg, deg, out);
     return CG_OK;
 }
 

@@ -23,6 +23,7 @@
 // the per-point factor (weight and w-phase, ra_exec.cu:55-93) and scattered
 // back to the caller's row order.
 #include <cstdlib>
+#include <type_traits>
 
 #include "cg_internal.hpp"
 
@@ -53,12 +54,15 @@ struct KernelEvalN {
         : nb(g.f_nb), ec(g.f_ec), hw(0.5f * (float)kw), learned(g.learned), q(g.kq)
     {
     }
-    __device__ __forceinline__ float operator()(float off0, int t) const
+    // LEARNED: -1 follows the launch's flag, 0 / 1 fix the kernel at compile
+    // time (no learned-polynomial code in the ES instantiations)
+    template <int LEARNED = -1>
+    __device__ __forceinline__ float at(float off0, int t) const
     {
         const float d = off0 + (float)t;
         const float u = ec * d * d;
         float y = nb * u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_sqrtf(1.0f - u));
-        if (learned) {
+        if (LEARNED == 1 || (LEARNED < 0 && learned)) {
             const float s = 2.0f * u - 1.0f;
             float p = q[LEARN_MAXDEG];
 #pragma unroll
@@ -67,6 +71,42 @@ struct KernelEvalN {
         }
         const float v = __builtin_amdgcn_exp2f(y);
         return fabsf(d) < hw ? v : 0.0f;
+    }
+    __device__ __forceinline__ float operator()(float off0, int t) const { return at(off0, t); }
+};
+
+// Kernel-value source of the complex64 MFMA kernels, fixed per
+// instantiation: the ES exp2 form, the learned exp2 form, or the per-tap
+// polynomials (tap_on).
+enum : int { kKerES = 0, kKerLearned = 1, kKerPoly = 2 };
+inline int ker_mode(const GridArgs &g) { return g.tp_on ? kKerPoly : g.learned ? kKerLearned : kKerES; }
+
+// FP32 kernel values psi(off0 + t) / e^beta (the normalisation of
+// KernelEvalN) from per-tap polynomials in x = 2 off0 + kw - 1 (host fit,
+// tap_polynomials), coefficients in LDS; zero outside the taps.  Eleven FMAs
+// instead of the square root, reciprocal and exp2 of KernelEvalN, and the
+// same cost for the ES and the learned kernel.
+struct TapPolyN {
+    const float *tab;  // kw rows of kTapStride coefficients
+    int kw;
+    __device__ __forceinline__ float operator()(float off0, int t) const
+    {
+        if (t < 0 || t >= kw) return 0.f;
+        const float x = 2.f * off0 + (float)(kw - 1);
+        const float4 *c = reinterpret_cast<const float4 *>(tab + t * kTapStride);
+        const float4 c0 = c[0], c1 = c[1], c2 = c[2];
+        float p = c2.z;
+        p = p * x + c2.y;
+        p = p * x + c2.x;
+        p = p * x + c1.w;
+        p = p * x + c1.z;
+        p = p * x + c1.y;
+        p = p * x + c1.x;
+        p = p * x + c0.w;
+        p = p * x + c0.z;
+        p = p * x + c0.y;
+        p = p * x + c0.x;
+        return p;
     }
 };
 
@@ -529,25 +569,19 @@ static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, c
 // ------------------------------------------------------------ dense bins
 //
 // A bin holding many points per start cell (dense uv coverage) is gridded as
-// a product per start cell: the points j of one cell share the footprint
-// origin, so their contributions to the cell's kw x kw x nf3 footprint are
-//   C[tap][p] = sum_j A[tap][j] B[j][p],  A = v_j psi_x psi_y,  B = psi_z,
-// a (64 taps) x (2 points) x (32 planes) FP32 MFMA step per point pair
-// (v_mfma_f32_32x32x2f32, real and imaginary parts as two products; taps are
-// t1 + 8 t2, planes the whole w axis).  C is then added once per cell into
-// the bin's footprint region in LDS (FP64), and the region once per bin into
-// the grid with FP32 global atomics, on top of what the sparse kernel wrote.
-
-typedef float v16f __attribute__((ext_vector_type(16)));
-
-constexpr int kDenseThreads = 512;
+// matrix products: the points j of a group of start cells share a footprint
+// frame, so their contributions are
+//   C[tap][p] = sum_j A[tap][j] B[j][p],  A = psi_x psi_y,  B = v_j psi_z,
+// FP32 MFMA steps (k_grid_dense2 below).  The frame is added into the bin's
+// footprint region in LDS (FP64), and the region once per bin into the grid
+// with FP32 global atomics, on top of what the sparse kernel wrote.
 
 __global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ tile_start,
                                                     int64_t ntiles, int dense_min,
                                                     int32_t *__restrict__ list)
 {
     // list[0] counts the dense bins, list[1..] holds them (any order);
-    // list[ntiles + 1] is the work counter of k_grid_dense
+    // list[ntiles + 1] is the work counter of the dense kernels
     const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t >= ntiles) return;
     if (tile_start[t + 1] - tile_start[t] >= dense_min) list[1 + atomicAdd(list, 1)] = (int32_t)t;
@@ -561,172 +595,310 @@ __global__ void k_zero_i32(int32_t *p, int64_t second)
     }
 }
 
-// BATCH: the points of a start cell are loaded 64 at a time (one per lane)
-// and handed to the lanes of each point pair by shuffles, so the MFMA loop
-// waits on no global load; bins are taken from a work counter.
-template <int KW, bool BATCH>
-__global__ __launch_bounds__(kDenseThreads) void k_grid_dense(const float4 *__restrict__ geo,
-                                                              const float2 *__restrict__ val,
-                                                              const uint32_t *__restrict__ keys,
-                                                              const int32_t *__restrict__ tile_start,
-                                                              int32_t *__restrict__ list,
-                                                              int64_t ntiles,
-                                                              float2 *__restrict__ fw, GridArgs g)
+// Dense bins, second form.  Work items are quads, 2 x 2 start cells (single
+// cells when kw = 8): with kw <= 7 the footprints of a quad's points lie in
+// one 8 x 8 tap frame.  A quad's points (two runs of the sorted order) are
+// first ordered by w-window (window w holds the points whose first plane z0
+// lies in [S w, S w + S), S = 9 - kw, so every footprint of the window fits
+// the eight planes [S w, S w + 8)) with a stable counting sort of the wave in
+// LDS (ballot ranks, so the order and the sums are reproducible).  The
+// window's points are then gridded four at a time with
+// v_mfma_f32_16x16x4f32:
+//   C[tap][col] += A[tap][k] B[k][col],  A = kx(x - dx) ky(y - dy) of point k
+//   for the 16 taps (x = tap % 8, y = 2 blk + tap / 8) of tap block blk,
+//   (dx, dy) its start cell in the quad,
+//   B = Re / Im(v_k) kz(plane) for the eight planes (col = 2 plane + part,
+//   part 0 real, 1 imaginary),
+// so the complex value rides in B.  Each lane evaluates two kernel values of
+// its point per step (x or y position r % 8, and the plane of its column);
+// the others come by lane shuffles.  Moving from window w to w + 1, the S
+// planes that leave the frame are added to the FP64 LDS footprint region
+// (x fastest) and the frame shifts by S planes, 2 S lanes down each 16-lane
+// row (one DPP move per register); the last frame is added whole.
+constexpr int kDense2Threads = 512;
+constexpr int kDense2Max = 512;  // points of a quad sorted at once
+
+// Stable counting sort by w-window (window = first plane z0 / S) of the n
+// (<= kDense2Max) points at positions c0 .. c0 + n - 1 of a work item whose
+// points are the sorted runs [a0, a0 + n0) then [a1, ...): lst[] receives the
+// positions p < n in window order, wb[w] .. wb[w + 1] bounds window w.  One
+// wavefront; ranks come from ballots, so the order is reproducible.
+struct Runs {
+    int a0, n0, a1, n;  // n = all points of the item
+    __device__ __forceinline__ int row(int i) const { return i < n0 ? a0 + i : a1 + (i - n0); }
+};
+
+template <int S>
+__device__ __forceinline__ void sort_by_window(const float4 *__restrict__ geo, const Runs &R, int c0,
+                                               int n, unsigned short *lst, int *cnt, int *wb, int lane)
 {
+    const uint64_t lt = (1ull << lane) - 1ull;
+    if (lane < 33) cnt[lane] = 0;
+    wave_lds_sync();
+    auto window_of = [&](int p, bool &ok) {
+        ok = p < n;
+        const int z0 = ok ? (int)(__float_as_uint(geo[R.row(c0 + p)].w) >> 16) : 0;
+        return ok ? min(z0 / S, 31) : 31;
+    };
+    auto match = [&](int wv, bool ok) {
+        uint64_t m = __ballot(ok);
+#pragma unroll
+        for (int bt = 0; bt < 5; bt++) {
+            const bool bit = (wv >> bt) & 1;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        return m;
+    };
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        bool ok;
+        const int wv = window_of(i0 + lane, ok);
+        const uint64_t m = match(wv, ok);
+        if (ok && (m & lt) == 0) cnt[wv] += __popcll(m);  // one leader per window
+        wave_lds_sync();
+    }
+    if (lane < 32) {
+        // exclusive prefix over the 32 window counts
+        const int v = cnt[lane];
+        int x = v;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const int y = __shfl_up(x, d, 32);
+            if (lane >= d) x += y;
+        }
+        wb[lane] = x - v;
+        cnt[lane] = x - v;
+        if (lane == 31) wb[32] = x;
+    }
+    wave_lds_sync();
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        bool ok;
+        const int wv = window_of(i0 + lane, ok);
+        const uint64_t m = match(wv, ok);
+        const int base = ok ? cnt[wv] : 0;
+        wave_lds_sync();
+        if (ok) {
+            lst[base + __popcll(m & lt)] = (unsigned short)(i0 + lane);
+            if ((m & lt) == 0) cnt[wv] = base + __popcll(m);
+        }
+        wave_lds_sync();
+    }
+}
+
+// first sorted position of every start cell of a tile (run[0 .. ncell]):
+// the tile's keys (tile << cshift | cell, ascending) are scanned once with
+// coalesced loads; position i starts the cells after its predecessor's up
+// to its own, and end starts the cells after the last one up to ncell.
+__device__ __forceinline__ void cell_runs(const uint32_t *__restrict__ keys, int beg, int end,
+                                          uint32_t key0, int cshift, int ncell, int *run)
+{
+    const uint32_t kb = key0 << cshift;
+    for (int i = beg + (int)threadIdx.x; i <= end; i += blockDim.x) {
+        const int cprev = i > beg ? (int)min(keys[i - 1] - kb, (uint32_t)ncell) : -1;
+        const int ccur = i < end ? (int)min(keys[i] - kb, (uint32_t)ncell) : ncell;
+        for (int c = cprev + 1; c <= ccur; c++) run[c] = i;
+    }
+}
+
+// the points of work item q of a tile: quad (QS = 2) or cell (QS = 1)
+template <int QS>
+__device__ __forceinline__ Runs item_runs(const int *run, int q, int tx, int &lx0, int &ly0)
+{
+    const int qx = tx / QS;
+    lx0 = (q % qx) * QS;
+    ly0 = (q / qx) * QS;
+    const int c = ly0 * tx + lx0;
+    Runs R;
+    R.a0 = run[c];
+    R.n0 = run[c + QS] - R.a0;
+    R.a1 = QS == 2 ? run[c + tx] : 0;
+    R.n = R.n0 + (QS == 2 ? run[c + tx + 2] - R.a1 : 0);
+    return R;
+}
+
+template <int KW, int QS, int KM>
+__global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__restrict__ geo,
+                                                                const float2 *__restrict__ val,
+                                                                const uint32_t *__restrict__ keys,
+                                                                const int32_t *__restrict__ tile_start,
+                                                                int32_t *__restrict__ list,
+                                                                int64_t ntiles,
+                                                                float2 *__restrict__ fw, GridArgs g)
+{
+    constexpr int S = 9 - KW;             // planes between windows
+    constexpr int SPAN = KW + QS - 1;     // frame positions per axis (<= 8)
+    constexpr int NB = (SPAN + 1) / 2;    // 16-tap blocks (x stride 8, two y per block)
+    constexpr int NWAVES = kDense2Threads / 64;
+    static_assert(SPAN <= 8, "frame");
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // footprint region, x fastest: [part][plane][y][x] (part 0 real, 1 imaginary)
     const int RX = g.tx + KW - 1, RY = g.ty + KW - 1, NP = g.nf3;
     const int rcells = RX * RY * NP;
     double *are = reinterpret_cast<double *>(smem_raw);
-    double *aim = are + rcells;
-    int *run = reinterpret_cast<int *>(aim + rcells);  // tx*ty + 1 run starts
+    int *run = reinterpret_cast<int *>(are + 2 * rcells);  // tx*ty + 1 run starts
+    __shared__ unsigned short s_lst[NWAVES][kDense2Max];
+    __shared__ int s_cnt[NWAVES][33], s_wb[NWAVES][33];
     __shared__ int s_next, s_tile;
     const int ncell = g.tx * g.ty;
+    const int nitem = ncell / (QS * QS);
     const int nlist = list[0];
-    const int lane = threadIdx.x & 63;
-    const int half = lane >> 5, i = lane & 31;
-    const KernelEvalN ker(g, KW);  // values / e^beta: the sums are scaled by e^(3 beta)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // lane (k, r): point slot k of a group; MFMA column r = frame plane
+    // col = r / 2, part r % 2; A row r = x position r (r < 8) or y parity
+    const int k = lane >> 4, r = lane & 15, col = r >> 1, part = r & 1;
+    const bool isx = r < 8;
+    const int NW = (NP + S - 1) / S;  // windows (<= 32)
+    // kernel values / e^beta (sums scaled by e^(3 beta)): the exp2 form or
+    // the per-tap polynomials
+    __shared__ __align__(16) float s_tp[8 * kTapStride];
+    if (KM == kKerPoly && threadIdx.x < 8 * kTapStride) s_tp[threadIdx.x] = g.tp[threadIdx.x];
+    __syncthreads();
+    const KernelEvalN kex(g, KW);
+    const TapPolyN kpo{s_tp, KW};
+    auto ker = [&](float off0, int t) {
+        if constexpr (KM == kKerPoly)
+            return kpo(off0, t);
+        else
+            return kex.template at<KM>(off0, t);
+    };
     const double sc = g.e3;
-    // lane roles: A[i][half] with tap i = t1 + 8 t2 (two 32-tap blocks),
-    // B[half][j] with plane j = i.  Each lane evaluates one kernel value of
-    // its point (i < 8: x tap i, 8 <= i < 16: y tap i - 8, 16 <= i < 24: z tap
-    // i - 16) and the operands are gathered from those lanes.
-    const int t1 = i & 7, t2a = i >> 3, t2b = 4 + (i >> 3);
-    const int src_kx = half * 32 + t1, src_ky0 = half * 32 + 8 + t2a, src_ky1 = half * 32 + 8 + t2b;
-    const int tap = i & 7;
-    const int axis = i >> 3;  // 0 x, 1 y, 2 z, 3 none
+    unsigned short *lst = s_lst[wave];
+    int *cnt = s_cnt[wave], *wb = s_wb[wave];
+    // lane constants: the e1 position (x of A row r, or y of r - 8) and the
+    // start-cell byte of the point record it subtracts
+    const int e1pos = r & 7, e1shift = isx ? 0 : 8;
+    const int src_kx = isx ? lane : lane - 8;  // x value of A row r
 
-    for (int li0 = blockIdx.x;; li0 += gridDim.x) {
-        int li = li0;
-        if (BATCH) {
-            if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
-            __syncthreads();
-            li = s_tile;
-        }
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+        __syncthreads();
+        const int li = s_tile;
         if (li >= nlist) break;
         const int tile = list[1 + li];
         const int bx = tile % g.ntx, by = tile / g.ntx;
-        const int beg = tile_start[tile], end = tile_start[tile + 1];
-        for (int k = threadIdx.x; k < 2 * rcells; k += blockDim.x) are[k] = 0.0;
-        // first sorted position of every start cell of the bin
-        for (int c = threadIdx.x; c <= ncell; c += blockDim.x) {
-            const uint32_t key = ((uint32_t)g.kr.lower(tile, ntiles) << g.cshift) + (uint32_t)c;
-            int lo = beg, hi = end;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (keys[mid] < key) lo = mid + 1;
-                else hi = mid;
-            }
-            run[c] = lo;
-        }
+        for (int q = threadIdx.x; q < 2 * rcells; q += blockDim.x) are[q] = 0.0;
+        cell_runs(keys, tile_start[tile], tile_start[tile + 1], (uint32_t)g.kr.lower(tile, ntiles),
+                  g.cshift, ncell, run);
         if (threadIdx.x == 0) s_next = 0;
         __syncthreads();
         for (;;) {
-            int c = 0;
-            if (lane == 0) c = atomicAdd(&s_next, 1);
-            c = __shfl(c, 0);
-            if (c >= ncell) break;
-            const int a = run[c], b = run[c + 1];
-            if (a >= b) continue;
-            v16f r0 = {}, q0 = {}, r1 = {}, q1 = {};
-            // one MFMA step for the point pair (G, v) of this half
-            auto pair = [&](bool ok, const float4 &G, const float2 &v) {
-                float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f, bz = 0.f;
-                // this lane's kernel value, then the operands from the lanes
-                // holding them (every lane takes part in the shuffles)
-                const float off = axis == 0 ? G.x : axis == 1 ? G.y : G.z;
-                const float kv = (ok && axis < 3 && tap < KW) ? ker(off, tap) : 0.f;
-                const float kx = __shfl(kv, src_kx);
-                const float ky0 = __shfl(kv, src_ky0);
-                const float ky1 = __shfl(kv, src_ky1);
-                int d = i - (int)(__float_as_uint(G.w) >> 16);
-                if (d < 0) d += NP;
-                const float kz = __shfl(kv, half * 32 + 16 + (d & 7));
-                if (ok) {
-                    const float w0 = kx * ky0, w1 = kx * ky1;
-                    a0r = v.x * w0;
-                    a0i = v.y * w0;
-                    a1r = v.x * w1;
-                    a1i = v.y * w1;
-                    if (i < NP && d < KW) bz = kz;
-                }
-                r0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0r, bz, r0, 0, 0, 0);
-                q0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0i, bz, q0, 0, 0, 0);
-                r1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1r, bz, r1, 0, 0, 0);
-                q1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1i, bz, q1, 0, 0, 0);
-            };
-            if (BATCH) {
-                for (int kb = a; kb < b; kb += 64) {
-                    const int n = min(64, b - kb);
-                    float4 Gl = make_float4(0.f, 0.f, 0.f, 0.f);
-                    float2 vl = make_float2(0.f, 0.f);
-                    if (lane < n) {
-                        Gl = geo[kb + lane];
-                        vl = val[kb + lane];
-                    }
-                    for (int j0 = 0; j0 < n; j0 += 2) {
-                        const int src = j0 + half;  // <= 63
-                        float4 G;
-                        float2 v;
-                        G.x = __shfl(Gl.x, src);
-                        G.y = __shfl(Gl.y, src);
-                        G.z = __shfl(Gl.z, src);
-                        G.w = __shfl(Gl.w, src);
-                        v.x = __shfl(vl.x, src);
-                        v.y = __shfl(vl.y, src);
-                        pair(src < n, G, v);
-                    }
-                }
-            } else {
-                for (int k0 = a; k0 < b; k0 += 2) {
-                    const int k = k0 + half;
-                    const bool ok = k < b;
-                    float4 G = make_float4(0.f, 0.f, 0.f, 0.f);
-                    float2 v = make_float2(0.f, 0.f);
-                    if (ok) {
-                        G = geo[k];
-                        v = val[k];
-                    }
-                    pair(ok, G, v);
-                }
-            }
-            // D[tap][plane]: lane holds plane i, taps (v/4)*8 + half*4 + v%4 (+32)
-            const int lx = c % g.tx, ly = c / g.tx;
-            if (i < NP) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(&s_next, 1);
+            it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+            if (it >= nitem) break;
+            int lx0, ly0;
+            Runs R = item_runs<QS>(run, it, g.tx, lx0, ly0);
+            R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
+            R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
+            R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
+            R.n = __builtin_amdgcn_readfirstlane(R.n);
+            // the item's records from a0: position i at i, or i + d1 past the
+            // first run (d1 >= 0)
+            const float4 *gi = geo + R.a0;
+            const float2 *vi = val + R.a0;
+            const unsigned d1 = (unsigned)(R.a1 - R.a0 - R.n0);
+            const int l0 = isx ? lx0 : ly0;
+            for (int c0 = 0; c0 < R.n; c0 += kDense2Max) {
+                const int n = min(kDense2Max, R.n - c0);
+                sort_by_window<S>(geo, R, c0, n, lst, cnt, wb, lane);
+                // ---- windows present
+                const bool has = lane < NW && wb[lane + 1] > wb[lane];
+                const uint64_t hm = __ballot(has);
+                if (hm == 0) continue;  // no point in range (cannot happen for valid records)
+                const int wfirst = __ffsll((unsigned long long)hm) - 1;
+                const int wlast = 63 - __clzll(hm);
+                v4f C[NB];
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const int tap = (v >> 2) * 8 + half * 4 + (v & 3);
-                    const int u1 = tap & 7, u2 = tap >> 3;  // block 0: t2 < 4; block 1: t2 + 4
-                    if (u1 < KW && u2 < KW) {
-                        const int idx = ((ly + u2) * RX + lx + u1) * NP + i;
-                        atomicAdd(are + idx, (double)r0[v] * sc);
-                        atomicAdd(aim + idx, (double)q0[v] * sc);
+                for (int q = 0; q < NB; q++) C[q] = v4f{0.f, 0.f, 0.f, 0.f};
+                // add frame planes rel [0, np) of window w to the region: the
+                // lane's registers (blk q, v) hold x = 4 (k % 2) + v, y = 2 q + k / 2
+                auto flush = [&](int w, int np) {
+                    if (col >= np) return;
+                    int p = S * w + col;
+                    if (p >= NP) p -= NP;
+                    double *b = are + part * rcells + ((p * RY + ly0 + (k >> 1)) * RX + lx0 + 4 * (k & 1));
+#pragma unroll
+                    for (int q = 0; q < NB; q++)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            const int x = 4 * (k & 1) + v, y = 2 * q + (k >> 1);
+                            if (x < SPAN && y < SPAN) atomicAdd(b + 2 * q * RX + v, (double)C[q][v]);
+                        }
+                };
+                // slot k of the next group, loaded one group ahead: groups
+                // never cross a window, and the group after the last one of
+                // window w starts at wb[w + 1] whichever window is next;
+                // positions past the item's end read its last point
+                auto fetch = [&](int pos, float4 &Gq, float2 &vq) {
+                    const unsigned i = (unsigned)(c0 + lst[min(pos, n - 1)]);
+                    const unsigned o = i < (unsigned)R.n0 ? i : i + d1;
+                    Gq = gi[o];
+                    vq = vi[o];
+                };
+                float4 Gn;
+                float2 vn;
+                fetch(k, Gn, vn);  // wb[wfirst] = 0
+                for (int w = wfirst; w <= wlast; w++) {
+                    if (w > wfirst) {
+                        flush(w - 1, S);
+                        // shift the frame down by S planes: DPP row_shl:2S
+                        // reads lane r + 2S of the same 16-lane row, and reads
+                        // past the row (planes col + S >= 8) give zero
+#pragma unroll
+                        for (int q = 0; q < NB; q++)
+#pragma unroll
+                            for (int v = 0; v < 4; v++)
+                                C[q][v] = __int_as_float(__builtin_amdgcn_update_dpp(
+                                    0, __float_as_int(C[q][v]), 0x100 + 2 * S, 0xf, 0xf, true));
                     }
-                    if (u1 < KW && u2 + 4 < KW) {
-                        const int idx = ((ly + u2 + 4) * RX + lx + u1) * NP + i;
-                        atomicAdd(are + idx, (double)r1[v] * sc);
-                        atomicAdd(aim + idx, (double)q1[v] * sc);
+                    const int e = wb[w + 1];
+                    const int cw = col + S * w;  // plane of the lane's column
+                    for (int j0 = wb[w]; j0 < e; j0 += 4) {
+                        const bool ok = j0 + k < e;
+                        const float4 G = Gn;
+                        const float2 v = vn;
+                        fetch((j0 + 4 < e ? j0 + 4 : e) + k, Gn, vn);
+                        // tap index of the lane's e1 position (x or y) and of
+                        // its plane: position minus the point's start cell
+                        const uint32_t pk = __float_as_uint(G.w);
+                        const int dq = (int)((pk >> e1shift) & 0xffu) - l0;
+                        const float e1 = ker(isx ? G.x : G.y, e1pos - dq);
+                        const float ex = __shfl(e1, src_kx);
+                        const float kz = ker(G.z, cw - (int)(pk >> 16));
+                        // an empty slot (past the window's end) holds another
+                        // point: its kernel values are finite, and a zero B
+                        // row drops it
+                        const float bv = ok ? (part ? v.y : v.x) * kz : 0.f;
+#pragma unroll
+                        for (int q = 0; q < NB; q++) {
+                            const float ey = __shfl(e1, 16 * k + 8 + 2 * q + (r >> 3));
+                            C[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(ex * ey, bv, C[q], 0, 0, 0);
+                        }
                     }
                 }
+                flush(wlast, 8);
             }
         }
         __syncthreads();
         // footprint region -> grid, x fastest so a wavefront covers row runs
-        for (int k = threadIdx.x; k < rcells; k += blockDim.x) {
-            const int rx = k % RX;
-            const int rest = k / RX;
+        for (int q = threadIdx.x; q < rcells; q += blockDim.x) {
+            const int rx = q % RX;
+            const int rest = q / RX;
             const int ry = rest % RY, p = rest / RY;
-            const int idx = (ry * RX + rx) * NP + p;
             int gx = bx * g.tx + rx;
             if (gx >= g.nf1) gx -= g.nf1;
             int gy = by * g.ty + ry;
             if (gy >= g.nf2) gy -= g.nf2;
             float2 *dst = fw + ((size_t)p * g.nf2 + gy) * g.nf1 + gx;
-            atomicAdd(&dst->x, (float)are[idx]);
-            atomicAdd(&dst->y, (float)aim[idx]);
+            atomicAdd(&dst->x, (float)(are[q] * sc));
+            atomicAdd(&dst->y, (float)(are[rcells + q] * sc));
         }
         __syncthreads();
     }
 }
+
+// static LDS of k_grid_dense2 (sorted lists and window counters of 8 waves)
+constexpr size_t kDense2Static = (size_t)(kDense2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4) + 64;
 
 static size_t dense_lds_bytes(const GridArgs &g, int kw)
 {
@@ -737,13 +909,15 @@ static size_t dense_lds_bytes(const GridArgs &g, int kw)
 bool grid_dense_supported(const GridArgs &g, int kw, int elem)
 {
     return elem == 8 && kw >= 2 && kw <= 8 && g.nf3 >= 2 * kw && g.nf3 <= 32 && g.cshift > 0 &&
-           g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 && dense_lds_bytes(g, kw) <= 163840 - 64;
+           g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 &&
+           dense_lds_bytes(g, kw) + kDense2Static <= 163840 - 64;
 }
 
+
 template <int KW>
-static int grid_dense_kw(const float4 *geo, const float2 *val, const uint32_t *keys,
-                         const int32_t *ts, int32_t *list, float2 *fw, const GridArgs &g,
-                         hipStream_t s)
+static int grid_dense2_kw(const float4 *geo, const float2 *val, const uint32_t *keys,
+                          const int32_t *ts, int32_t *list, float2 *fw, const GridArgs &g,
+                          hipStream_t s)
 {
     const int64_t ntiles = (int64_t)g.ntx * g.nty;
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
@@ -751,24 +925,28 @@ static int grid_dense_kw(const float4 *geo, const float2 *val, const uint32_t *k
                        ntiles, g.dense_min, list);
     CG_HIP(hipGetLastError());
     const size_t lds = dense_lds_bytes(g, KW);
-    const int per_cu = std::max<int>(1, (int)(163840 / lds));
+    const int per_cu = std::max<int>(1, (int)(163840 / (lds + kDense2Static)));
     const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
-    const char *e = std::getenv("CG_DENSE_BATCH");
-    if (!e || std::atoi(e) != 0) {
+    // quads of start cells where their footprints fit the 8 x 8 frame
+    auto go = [&](const void *fn, auto kern) -> int {
         if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW, true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_grid_dense<KW, true>), dim3(nblk), dim3(kDenseThreads), lds, s, geo,
-                           val, keys, ts, list, ntiles, fw, g);
-    } else {
-        if (lds > 65536)
-            CG_HIP(hipFuncSetAttribute((const void *)k_grid_dense<KW, false>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_grid_dense<KW, false>), dim3(nblk), dim3(kDenseThreads), lds, s, geo,
-                           val, keys, ts, list, ntiles, fw, g);
-    }
-    CG_HIP(hipGetLastError());
-    return CG_OK;
+            CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(kDense2Threads), lds, s, geo, val, keys, ts, list,
+                           ntiles, fw, g);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    };
+    const int km = ker_mode(g);
+    auto pick = [&](auto qs) -> int {
+        constexpr int QS = decltype(qs)::value;
+        if (km == kKerPoly) return go((const void *)k_grid_dense2<KW, QS, kKerPoly>, k_grid_dense2<KW, QS, kKerPoly>);
+        if (km == kKerLearned)
+            return go((const void *)k_grid_dense2<KW, QS, kKerLearned>, k_grid_dense2<KW, QS, kKerLearned>);
+        return go((const void *)k_grid_dense2<KW, QS, kKerES>, k_grid_dense2<KW, QS, kKerES>);
+    };
+    if constexpr (KW <= 7)
+        if (g.tx % 2 == 0 && g.ty % 2 == 0) return pick(std::integral_constant<int, 2>{});
+    return pick(std::integral_constant<int, 1>{});
 }
 
 int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uint32_t *keys,
@@ -778,7 +956,7 @@ int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uin
     if (!grid_dense_supported(g, kw, 8) || g.dense_min <= 0) return CG_ERR_ARG;
     switch (kw) {
 #define CG_CASE(K) \
-    case K: return grid_dense_kw<K>(geo, val, keys, ts, list, fw, g, s);
+    case K: return grid_dense2_kw<K>(geo, val, keys, ts, list, fw, g, s);
         CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8)
 #undef CG_CASE
     default: return CG_ERR_ARG;
@@ -801,19 +979,33 @@ int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, cx_t
 
 // ------------------------------------------------------------ degridding
 
+// Tiles of the degridder's list: the non-empty ones below dense_min (all
+// non-empty ones when dense_min <= 0).  list[0] counts them, list[1..]
+// holds them (any order), list[ntiles + 1] is the work counter.
+__global__ __launch_bounds__(256) void k_sparse_tiles(const int32_t *__restrict__ tile_start,
+                                                      int64_t ntiles, int dense_min,
+                                                      int32_t *__restrict__ list)
+{
+    const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (t >= ntiles) return;
+    const int n = tile_start[t + 1] - tile_start[t];
+    if (n > 0 && (dense_min <= 0 || n < dense_min)) list[1 + atomicAdd(list, 1)] = (int32_t)t;
+}
+
 template <typename T, int KW, int PPT>
-__global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ geo,
-                                                const cx_t<T> *__restrict__ fac,
-                                                const int32_t *__restrict__ perm,
-                                                const int32_t *__restrict__ tile_start,
-                                                const cx_t<T> *__restrict__ fw,
-                                                cx_t<T> *__restrict__ out, GridArgs g, int is2d)
+__device__ __forceinline__ void degrid_tile(const geo_t<T> *__restrict__ geo,
+                                            const cx_t<T> *__restrict__ fac,
+                                            const int32_t *__restrict__ perm,
+                                            const int32_t *__restrict__ tile_start,
+                                            const cx_t<T> *__restrict__ fw,
+                                            cx_t<T> *__restrict__ out, const GridArgs &g, int is2d,
+                                            int tile)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx_t<T> *lds = reinterpret_cast<cx_t<T> *>(smem_raw);
-    const int tile = blockIdx.x;
     const int beg = tile_start[tile], end = tile_start[tile + 1];
     if (beg == end) return;
+    if (g.dense_min > 0 && end - beg >= g.dense_min) return;  // k_degrid_dense's
     const int bx = tile % g.ntx, by = tile / g.ntx;
     const int hx = g.tx + KW - 1, hy = g.ty + KW - 1;
     // LDS row stride: odd, so that cells a power-of-two number of rows or
@@ -918,19 +1110,368 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
     }
 }
 
+// Dense bins of the degridder (complex64), the adjoint of k_grid_dense2.  A
+// workgroup stages its tile plus the (kw-1) halo of every plane in LDS (as
+// k_degrid does) and its waves take quads (2 x 2 start cells; cells at kw =
+// 8).  A quad with at least kDegrid2Item points is ordered by w-window
+// (sort_by_window) and each window's points are interpolated sixteen at a
+// time as one 16 x 16 product on v_mfma_f32_16x16x4f32:
+//   R[j][col] = sum_tap W[j][tap] F[tap][col],  W = kx_j(x - dx_j) ky_j(y - dy_j)
+// over the 8 x 8 frame taps (four per step: x = 4 (step % 2) + k, y =
+// step / 2), with F the staged grid values of the window's eight planes
+// (col = 2 plane + part, part 0 real, 1 imaginary; loaded once per window
+// and shared by its groups), then v_j = sum_plane kz_j(plane) R[j][plane],
+// a DPP reduction across the lanes of each 16-lane row.  Each lane evaluates
+// six kernel values of one point per group (two each along x, y and z); the
+// y and z values reach the lanes that need them through a small per-wave
+// LDS table.  Quads with fewer points give each point a lane.
+constexpr int kDegrid2Threads = 1024;
+constexpr int kDegrid2Item = 48;
+
+template <int KW, int QS, int KM>
+__global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *__restrict__ geo,
+                                                                  const float2 *__restrict__ fac,
+                                                                  const int32_t *__restrict__ perm,
+                                                                  const uint32_t *__restrict__ keys,
+                                                                  const int32_t *__restrict__ tile_start,
+                                                                  int32_t *__restrict__ list,
+                                                                  int64_t ntiles,
+                                                                  const float2 *__restrict__ fw,
+                                                                  float2 *__restrict__ out, GridArgs g)
+{
+    constexpr int S = 9 - KW;
+    constexpr int SPAN = KW + QS - 1;
+    constexpr int NCH = 2 * SPAN;  // tap steps: y = ch / 2 < SPAN
+    constexpr int NWAVES = kDegrid2Threads / 64;
+    static_assert(SPAN <= 8, "frame");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(smem_raw);
+    const float *ldsf = reinterpret_cast<const float *>(smem_raw);
+    __shared__ unsigned short s_lst[NWAVES][kDense2Max];
+    __shared__ int s_cnt[NWAVES][33], s_wb[NWAVES][33];
+    __shared__ int s_run[257];
+    __shared__ int s_next, s_tile;
+    // per-wave exchange of a group's kernel values: ky[point][y], kz[plane][point]
+    __shared__ __align__(16) float s_ky[NWAVES][16 * 8];
+    __shared__ __align__(16) float s_kz[NWAVES][8 * 16];
+    const int NP = g.nf3;
+    const int hx = g.tx + KW - 1, hy = g.ty + KW - 1;
+    const int hxs = degrid_row_stride(g.tx, KW);
+    const int ncell = g.tx * g.ty;  // <= 256
+    const int nitem = ncell / (QS * QS);
+    const int nlist = list[0];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // lane (hi, lo): A operand point lo, tap hi of a step; B operand column
+    // lo (plane lo / 2, part lo % 2); result points 4 hi + v at column lo
+    const int hi = lane >> 4, lo = lane & 15;
+    const int NW = (NP + S - 1) / S;
+    // kernel values / e^beta (sums of products scaled by e^(3 beta)): the
+    // exp2 form or the per-tap polynomials
+    __shared__ __align__(16) float s_tp[8 * kTapStride];
+    if (KM == kKerPoly && threadIdx.x < 8 * kTapStride) s_tp[threadIdx.x] = g.tp[threadIdx.x];
+    __syncthreads();
+    const KernelEvalN kex(g, KW);
+    const TapPolyN kpo{s_tp, KW};
+    auto ker = [&](float off0, int t) {
+        if constexpr (KM == kKerPoly)
+            return kpo(off0, t);
+        else
+            return kex.template at<KM>(off0, t);
+    };
+    const float sc = (float)g.e3;
+    unsigned short *lst = s_lst[wave];
+    int *cnt = s_cnt[wave], *wb = s_wb[wave];
+    float *ky_t = s_ky[wave], *kz_t = s_kz[wave];
+
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+        __syncthreads();
+        const int li = s_tile;
+        if (li >= nlist) break;
+        const int tile = list[1 + li];
+        const int bx = tile % g.ntx, by = tile / g.ntx;
+        const int x0 = bx * g.tx, y0 = by * g.ty;
+        // the tile and its halo of every plane
+        const int pcells = hy * hx;
+        for (int q = threadIdx.x; q < NP * pcells; q += blockDim.x) {
+            const int lz = q / pcells;
+            const int rem = q - lz * pcells;
+            const int y = rem / hx, x = rem - y * hx;
+            int gx = x0 + x;
+            if (gx >= g.nf1) gx -= g.nf1;
+            int gy = y0 + y;
+            if (gy >= g.nf2) gy -= g.nf2;
+            lds[(lz * hy + y) * hxs + x] = fw[((size_t)lz * g.nf2 + gy) * g.nf1 + gx];
+        }
+        cell_runs(keys, tile_start[tile], tile_start[tile + 1], (uint32_t)g.kr.lower(tile, ntiles),
+                  g.cshift, ncell, s_run);
+        if (threadIdx.x == 0) s_next = 0;
+        __syncthreads();
+        for (;;) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(&s_next, 1);
+            it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+            if (it >= nitem) break;
+            int lx0, ly0;
+            Runs R = item_runs<QS>(s_run, it, g.tx, lx0, ly0);
+            R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
+            R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
+            R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
+            R.n = __builtin_amdgcn_readfirstlane(R.n);
+            if (R.n < kDegrid2Item) {
+                // one lane per point: all KW^3 taps from LDS
+                for (int i = lane; i < R.n; i += 64) {
+                    const int row = R.row(i);
+                    const float4 G = geo[row];
+                    const uint32_t pk = __float_as_uint(G.w);
+                    const int lx = (int)(pk & 0xffu), ly = (int)((pk >> 8) & 0xffu);
+                    float kx[KW], ky[KW];
+#pragma unroll
+                    for (int t = 0; t < KW; t++) {
+                        kx[t] = ker(G.x, t);
+                        ky[t] = ker(G.y, t);
+                    }
+                    const int z0 = (int)(pk >> 16);
+                    float ar = 0.f, ai = 0.f;
+                    for (int tz = 0; tz < KW; tz++) {
+                        int zz = z0 + tz;
+                        if (zz >= NP) zz -= NP;
+                        const float kzv = ker(G.z, tz);
+                        const float2 *pl = lds + ((size_t)zz * hy + ly) * hxs + lx;
+                        float sr = 0.f, si = 0.f;
+#pragma unroll
+                        for (int t2 = 0; t2 < KW; t2++) {
+                            float rr = 0.f, ri = 0.f;
+#pragma unroll
+                            for (int t1 = 0; t1 < KW; t1++) {
+                                const float2 f = pl[t2 * hxs + t1];
+                                rr += f.x * kx[t1];
+                                ri += f.y * kx[t1];
+                            }
+                            sr += rr * ky[t2];
+                            si += ri * ky[t2];
+                        }
+                        ar += sr * kzv;
+                        ai += si * kzv;
+                    }
+                    ar *= sc;
+                    ai *= sc;
+                    const float2 f = fac[row];
+                    const int dst = perm[row];
+                    if (dst >= 0 && dst < g.m) out[dst] = make_float2(ar * f.x - ai * f.y, ar * f.y + ai * f.x);
+                }
+                continue;
+            }
+            // the item's records from a0: position i at i, or i + d1 past the
+            // first run (d1 >= 0)
+            const float4 *gi = geo + R.a0;
+            const unsigned d1 = (unsigned)(R.a1 - R.a0 - R.n0);
+            auto rec = [&](int i) -> unsigned {
+                return (unsigned)i < (unsigned)R.n0 ? (unsigned)i : (unsigned)i + d1;
+            };
+            for (int c0 = 0; c0 < R.n; c0 += kDense2Max) {
+                const int n = min(kDense2Max, R.n - c0);
+                sort_by_window<S>(geo, R, c0, n, lst, cnt, wb, lane);
+                const bool has = lane < NW && wb[lane + 1] > wb[lane];
+                const uint64_t hm = __ballot(has);
+                if (hm == 0) continue;
+                const int wfirst = __ffsll((unsigned long long)hm) - 1;
+                const int wlast = 63 - __clzll(hm);
+                // point lo of the next group, loaded one group ahead (the
+                // group after the last one of window w starts at wb[w + 1]);
+                // positions past the item's end read its last point
+                auto fetch = [&](int pos) { return gi[rec(c0 + lst[min(pos, n - 1)])]; };
+                float4 Gn = fetch(lo);  // wb[wfirst] = 0
+                for (int w = wfirst; w <= wlast; w++) {
+                    const int e = wb[w + 1];
+                    if (wb[w] == e) continue;
+                    // the window's grid values: step ch holds tap (x, y) =
+                    // (4 (ch % 2) + hi, ch / 2) of column lo
+                    int p = S * w + (lo >> 1);
+                    if (p >= NP) p -= NP;
+                    const float *fb = ldsf + 2 * (((size_t)p * hy + ly0) * hxs + lx0 + hi) + (lo & 1);
+                    float B[NCH];
+#pragma unroll
+                    for (int ch = 0; ch < NCH; ch++) {
+                        const int x = 4 * (ch & 1) + hi;
+                        B[ch] = x < SPAN ? fb[2 * ((ch >> 1) * hxs + 4 * (ch & 1))] : 0.f;
+                    }
+                    const int zw = S * w;
+                    for (int j0 = wb[w]; j0 < e; j0 += 16) {
+                        const float4 G = Gn;
+                        Gn = fetch((j0 + 16 < e ? j0 + 16 : e) + lo);
+                        // the writer lanes (lo even, lo < 8) own point
+                        // 4 hi + lo / 2 of the group: its record, early
+                        const int pw = 4 * hi + (lo >> 1);
+                        const bool wr = (lo & 9) == 0 && j0 + pw < e;
+                        unsigned ow = 0;
+                        float2 fw2 = make_float2(0.f, 0.f);
+                        int dst = -1;
+                        if (wr) {
+                            ow = R.a0 + rec(c0 + lst[j0 + pw]);
+                            fw2 = fac[ow];
+                            dst = perm[ow];
+                        }
+                        const uint32_t pk = __float_as_uint(G.w);
+                        const int dx = (int)(pk & 0xffu) - lx0, dy = (int)((pk >> 8) & 0xffu) - ly0;
+                        const int dz = (int)(pk >> 16) - zw;
+                        const float kxa = ker(G.x, hi - dx), kxb = ker(G.x, hi + 4 - dx);
+                        const float2 kyv = make_float2(ker(G.y, 2 * hi - dy), ker(G.y, 2 * hi + 1 - dy));
+                        const float kz0 = ker(G.z, 2 * hi - dz), kz1 = ker(G.z, 2 * hi + 1 - dz);
+                        wave_lds_sync();  // the previous group's reads are done
+                        *reinterpret_cast<float2 *>(ky_t + lo * 8 + 2 * hi) = kyv;
+                        kz_t[(2 * hi) * 16 + lo] = kz0;
+                        kz_t[(2 * hi + 1) * 16 + lo] = kz1;
+                        wave_lds_sync();
+                        const float4 ya = *reinterpret_cast<const float4 *>(ky_t + lo * 8);
+                        const float4 yb = *reinterpret_cast<const float4 *>(ky_t + lo * 8 + 4);
+                        const float4 kzv = *reinterpret_cast<const float4 *>(kz_t + (lo >> 1) * 16 + 4 * hi);
+                        const float ky[8] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y, yb.z, yb.w};
+                        v4f D0 = v4f{0.f, 0.f, 0.f, 0.f}, D1 = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int ch = 0; ch < NCH; ch += 2) {
+                            D0 = __builtin_amdgcn_mfma_f32_16x16x4f32(kxa * ky[ch >> 1], B[ch], D0, 0, 0, 0);
+                            D1 = __builtin_amdgcn_mfma_f32_16x16x4f32(kxb * ky[ch >> 1], B[ch + 1], D1, 0, 0, 0);
+                        }
+                        // z weights, then the sum over the eight planes of
+                        // each part: lanes of equal parity in the row
+                        float s[4] = {(D0[0] + D1[0]) * kzv.x, (D0[1] + D1[1]) * kzv.y,
+                                      (D0[2] + D1[2]) * kzv.z, (D0[3] + D1[3]) * kzv.w};
+#pragma unroll
+                        for (int v = 0; v < 4; v++) {
+                            s[v] += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s[v]), 0x122, 0xf, 0xf, false));
+                            s[v] += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s[v]), 0x124, 0xf, 0xf, false));
+                            s[v] += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s[v]), 0x128, 0xf, 0xf, false));
+                        }
+                        // lane lo keeps point 4 hi + (lo / 2) % 4; an even lane
+                        // takes the imaginary part from lane lo + 1
+                        const int vs = (lo >> 1) & 3;
+                        const float mine = vs == 0 ? s[0] : vs == 1 ? s[1] : vs == 2 ? s[2] : s[3];
+                        const float other = __int_as_float(
+                            __builtin_amdgcn_update_dpp(0, __float_as_int(mine), 0xb1, 0xf, 0xf, false));
+                        if (wr && dst >= 0 && dst < g.m) {
+                            const float sr = mine * sc, si = other * sc;
+                            out[dst] = make_float2(sr * fw2.x - si * fw2.y, sr * fw2.y + si * fw2.x);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the staged tile and the run table are reused
+    }
+}
+
+// static LDS of k_degrid_dense
+constexpr size_t kDegrid2Static =
+    (size_t)(kDegrid2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4 + 2 * 128 * 4) + 257 * 4 + 64;
+
+template <int KW>
+static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *perm,
+                           const uint32_t *keys, const int32_t *ts, int32_t *list, const float2 *fw,
+                           float2 *out, const GridArgs &g, hipStream_t s)
+{
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
+    hipLaunchKernelGGL(k_dense_bins, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, ts,
+                       ntiles, g.dense_min, list);
+    CG_HIP(hipGetLastError());
+    const size_t lds = (size_t)degrid_lds_bytes(g, KW, 8);
+    const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, 256);
+    auto go = [&](const void *fn, auto kern) -> int {
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(kDegrid2Threads), lds, s, geo, fac, perm, keys, ts,
+                           list, ntiles, fw, out, g);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    };
+    const int km = ker_mode(g);
+    auto pick = [&](auto qs) -> int {
+        constexpr int QS = decltype(qs)::value;
+        if (km == kKerPoly) return go((const void *)k_degrid_dense<KW, QS, kKerPoly>, k_degrid_dense<KW, QS, kKerPoly>);
+        if (km == kKerLearned)
+            return go((const void *)k_degrid_dense<KW, QS, kKerLearned>, k_degrid_dense<KW, QS, kKerLearned>);
+        return go((const void *)k_degrid_dense<KW, QS, kKerES>, k_degrid_dense<KW, QS, kKerES>);
+    };
+    if constexpr (KW <= 7)
+        if (g.tx % 2 == 0 && g.ty % 2 == 0) return pick(std::integral_constant<int, 2>{});
+    return pick(std::integral_constant<int, 1>{});
+}
+
+bool degrid_dense_supported(const GridArgs &g, int kw)
+{
+    return kw >= 2 && kw <= 8 && g.nf3 >= 2 * kw && g.nf3 <= 32 && g.cshift > 0 && g.nchunk == 1 &&
+           g.zn == 0 && g.tx * g.ty <= 256 && g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 &&
+           (size_t)degrid_lds_bytes(g, kw, 8) + kDegrid2Static <= 163840 - 64;
+}
+
+// One tile per workgroup (list == nullptr: tile blockIdx.x), or a
+// persistent grid taking the tiles of list (k_sparse_tiles) one at a time.
+template <typename T, int KW, int PPT>
+__global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ geo,
+                                                const cx_t<T> *__restrict__ fac,
+                                                const int32_t *__restrict__ perm,
+                                                const int32_t *__restrict__ tile_start,
+                                                const cx_t<T> *__restrict__ fw,
+                                                cx_t<T> *__restrict__ out, GridArgs g, int is2d,
+                                                int32_t *__restrict__ list, int64_t ntiles)
+{
+    __shared__ int s_tile;
+    const int nlist = list ? list[0] : 1;
+    for (int round = 0;; round++) {
+        int tile = blockIdx.x;
+        if (list) {
+            if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+            __syncthreads();
+            const int li = s_tile;
+            if (li >= nlist) break;
+            tile = list[1 + li];
+        } else if (round > 0) {
+            break;
+        }
+        degrid_tile<T, KW, PPT>(geo, fac, perm, tile_start, fw, out, g, is2d, tile);
+        __syncthreads();  // the staged planes and s_tile are reused
+    }
+}
+
+int launch_degrid_dense(const float4 *geo, const float2 *fac, const int32_t *perm,
+                        const uint32_t *keys, const int32_t *ts, int32_t *list, const float2 *fw,
+                        float2 *out, const GridArgs &g, int kw, hipStream_t s)
+{
+    if (!degrid_dense_supported(g, kw) || g.dense_min <= 0) return CG_ERR_ARG;
+    switch (kw) {
+#define CG_CASE(K) \
+    case K: return degrid_dense_kw<K>(geo, fac, perm, keys, ts, list, fw, out, g, s);
+        CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8)
+#undef CG_CASE
+    default: return CG_ERR_ARG;
+    }
+}
+
 template <typename T, int KW>
 static int degrid_kw(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
                      const int32_t *ts, const cx_t<T> *fw, cx_t<T> *out, const GridArgs &g,
-                     hipStream_t s)
+                     int32_t *list, hipStream_t s)
 {
     int is2d = g.nf3 == 1 ? 1 : 0;
     size_t lds = (size_t)degrid_lds_bytes(g, KW, (int)sizeof(cx_t<T>));
-    unsigned nblk = (unsigned)((int64_t)g.ntx * g.nty);
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
+    unsigned nblk = (unsigned)ntiles;
+    const int threads = grid_threads(lds);
+    if (list) {
+        // a persistent grid over the listed tiles: as many workgroups as fit
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
+        hipLaunchKernelGGL(k_sparse_tiles, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s,
+                           ts, ntiles, g.dense_min, list);
+        CG_HIP(hipGetLastError());
+        const int per_cu = std::max(1, std::min((int)(163840 / std::max<size_t>(lds, 1)), 2048 / threads));
+        nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
+    }
     if (lds > 65536)
         CG_HIP(hipFuncSetAttribute((const void *)k_degrid<T, KW, 4>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(grid_threads(lds)), lds, s, geo, fac,
-                       perm, ts, fw, out, g, is2d);
+    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(threads), lds, s, geo, fac, perm, ts,
+                       fw, out, g, is2d, list, ntiles);
     CG_HIP(hipGetLastError());
     return CG_OK;
 }
@@ -938,11 +1479,11 @@ static int degrid_kw(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *per
 template <typename T, typename TV>
 int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
                   const int32_t *ts, const cx_t<T> *fw, TV *out, const GridArgs &g, int kw,
-                  hipStream_t s)
+                  int32_t *list, hipStream_t s)
 {
     switch (kw) {
 #define CG_CASE(K) \
-    case K: return degrid_kw<T, K>(geo, fac, perm, ts, fw, out, g, s);
+    case K: return degrid_kw<T, K>(geo, fac, perm, ts, fw, out, g, list, s);
         CG_CASE(2) CG_CASE(3) CG_CASE(4) CG_CASE(5) CG_CASE(6) CG_CASE(7) CG_CASE(8) CG_CASE(9)
         CG_CASE(10) CG_CASE(11) CG_CASE(12) CG_CASE(13) CG_CASE(14) CG_CASE(15) CG_CASE(16)
 #undef CG_CASE
@@ -952,7 +1493,7 @@ int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
 
 template int launch_grid<float>(const float4 *, const float2 *, const int32_t *, float2 *, const GridArgs &, int, hipStream_t);
 template int launch_grid<double>(const double4 *, const double2 *, const int32_t *, double2 *, const GridArgs &, int, hipStream_t);
-template int launch_degrid<float, float2>(const float4 *, const float2 *, const int32_t *, const int32_t *, const float2 *, float2 *, const GridArgs &, int, hipStream_t);
-template int launch_degrid<double, double2>(const double4 *, const double2 *, const int32_t *, const int32_t *, const double2 *, double2 *, const GridArgs &, int, hipStream_t);
+template int launch_degrid<float, float2>(const float4 *, const float2 *, const int32_t *, const int32_t *, const float2 *, float2 *, const GridArgs &, int, int32_t *, hipStream_t);
+template int launch_degrid<double, double2>(const double4 *, const double2 *, const int32_t *, const int32_t *, const double2 *, double2 *, const GridArgs &, int, int32_t *, hipStream_t);
 
 }  // namespace cg

@@ -158,6 +158,10 @@ struct GridArgs {
     // 1: complex64 one-plane gridding may run as per-quadrant MFMA products
     // (grid2d_mfma_supported decides); 0: always the LDS-atomic kernel
     int mfma2d;
+    // FP32 kernel values as per-tap polynomials (tap_polynomials, kw <= 8):
+    // tp_on != 0 selects them over the exp2 form in the complex64 kernels
+    int tp_on;
+    float tp[8 * kTapStride];
     // w-plane slab held by the grid buffer: planes zb .. zb + zn - 1 of the
     // nf3 (periodic) planes sit at buffer planes 0 .. zn - 1 (zn = 0: the
     // whole grid, zb = 0).  Gridding writes only the slab; degridding adds
@@ -171,10 +175,12 @@ bool grid2d_mfma_supported(const GridArgs &g, int kw);
 template <typename T>
 int launch_grid(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *tile_start,
                 cx_t<T> *fw, const GridArgs &g, int kw, hipStream_t s);
+// list (ntiles + 2 ints of scratch, or nullptr): run over the listed
+// non-empty tiles below g.dense_min instead of one workgroup per tile
 template <typename T, typename TV>
 int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
                   const int32_t *tile_start, const cx_t<T> *fw, TV *out,
-                  const GridArgs &g, int kw, hipStream_t s);
+                  const GridArgs &g, int kw, int32_t *list, hipStream_t s);
 int grid_lds_bytes(const GridArgs &g, int elem);
 // Dense bins (fp32, kw <= 8, nf3 <= 32, cshift > 0): the sparse kernel
 // launched with g.dense_min > 0 leaves them out; this adds them to fw with
@@ -184,6 +190,14 @@ int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uin
                       const int32_t *tile_start, int32_t *list, cx_t<float> *fw, const GridArgs &g,
                       int kw, hipStream_t s);
 int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
+// Dense bins of the complex64 degridder (kw <= 8, nf3 <= 32, start cell in
+// the sort key, whole grid staged, no slabs): k_degrid launched with
+// g.dense_min > 0 leaves the bins with at least that many points to this
+// one, which interpolates them on MFMA.  list needs ntiles + 2 ints.
+bool degrid_dense_supported(const GridArgs &g, int kw);
+int launch_degrid_dense(const float4 *geo, const float2 *fac, const int32_t *perm,
+                        const uint32_t *keys, const int32_t *tile_start, int32_t *list,
+                        const float2 *fw, float2 *out, const GridArgs &g, int kw, hipStream_t s);
 
 // ---- FFT (cg_fft.hip): in-place unnormalised 2-D FFT of `batch` planes
 // [batch][n2][n1], exponent sign dir (+1 / -1), natural order in and out.

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace cg {
 
@@ -139,6 +140,46 @@ int kernel_quadrature(const ConvOpts &o, double *xq, double *gq)
         gq[n] = alpha * w[n] * phi;
     }
     return p;
+}
+
+void tap_polynomials(int kw, double beta, double c, const float *kq, int ndeg, int deg,
+                     float *out)
+{
+    const int n = deg + 1;
+    std::vector<double> xs(n), f(n), a(n), T(n * n, 0.0), mono(n);
+    // T[k][q]: monomial coefficients of the Chebyshev polynomial T_k
+    T[0] = 1.0;
+    if (n > 1) T[1 * n + 1] = 1.0;
+    for (int k = 2; k < n; k++)
+        for (int q = 0; q < n; q++)
+            T[k * n + q] = (q > 0 ? 2.0 * T[(k - 1) * n + q - 1] : 0.0) - T[(k - 2) * n + q];
+    for (int i = 0; i < n; i++) xs[i] = std::cos(PI * (i + 0.5) / n);
+    for (int t = 0; t < kw; t++) {
+        for (int i = 0; i < n; i++) {
+            const double d = (xs[i] + 1.0) / 2.0 - kw / 2.0 + t;
+            const double u = c * d * d;
+            double e = beta * (std::sqrt(std::max(1.0 - u, 0.0)) - 1.0);
+            if (kq) {
+                const double sv = 2.0 * u - 1.0;
+                double y = 0.0;
+                for (int k = ndeg; k >= 0; k--) y = y * sv + kq[k];
+                e += y;
+            }
+            f[i] = std::fabs(d) < kw / 2.0 ? std::exp(e) : 0.0;
+        }
+        // Chebyshev coefficients from the values at the nodes
+        for (int k = 0; k < n; k++) {
+            double sacc = 0.0;
+            for (int i = 0; i < n; i++) sacc += f[i] * std::cos(PI * k * (i + 0.5) / n);
+            a[k] = (k == 0 ? 1.0 : 2.0) * sacc / n;
+        }
+        for (int q = 0; q < n; q++) {
+            double m = 0.0;
+            for (int k = 0; k < n; k++) m += a[k] * T[k * n + q];
+            mono[q] = m;
+        }
+        for (int q = 0; q < kTapStride; q++) out[t * kTapStride + q] = q < n ? (float)mono[q] : 0.0f;
+    }
 }
 
 void fseries_grid(const ConvOpts &o, int nf, double *out)

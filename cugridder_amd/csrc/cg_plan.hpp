@@ -40,6 +40,18 @@ int kernel_quadrature(const ConvOpts &o, double *xq, double *gq);
 // out[k] = 2 sum g_i cos(pi k (x_i - N + 1)/(N - 1)), N = nf/2+1.
 void fseries_grid(const ConvOpts &o, int nf, double *out);
 
+// Per-tap polynomials of the FP32 kernel values psi(d) / e^beta: for tap t
+// of a point whose first tap sits at offset d0 in [-kw/2, -kw/2 + 1),
+//   psi(d0 + t) / e^beta = sum_q out[t * kTapStride + q] x^q,  x = 2 d0 + kw - 1,
+// fitted in FP64 by Chebyshev interpolation at deg + 1 nodes and returned as
+// monomial coefficients (their absolute sums stay near 1, so FP32 Horner
+// evaluation is stable).  psi is the ES kernel exp(beta sqrt(1 - c d^2)), or
+// with kq (ndeg + 1 coefficients) the learned kernel, whose exponent adds
+// sum_k kq[k] s^k, s = 2 c d^2 - 1.  Fills kw * kTapStride floats.
+constexpr int kTapStride = 12;
+void tap_polynomials(int kw, double beta, double c, const float *kq, int ndeg, int deg,
+                     float *out);
+
 struct Tiling {
     int tx = 0, ty = 0;      // tile extent along nf1 (fast) and nf2
     int ntx = 0, nty = 0;    // tiles per axis

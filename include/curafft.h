@@ -164,10 +164,11 @@ int cg_ctx_destroy(void *ctx);
  * the host entry points read it from the environment, CG_GRID_LIMIT_MB),
  * "stop_after" (test hook, see below) */
 int cg_ctx_set_option(void *ctx, const char *name, int value);
-/* per-stage times summed over calls, out[15] = {ms2dirty[7], dirty2ms[7],
- * dense} in ms, stages: plan, sort, records, grid/degrid, fft rows, fft
- * columns, image (epilogue/prologue); dense = the MFMA gridding kernel's
- * part of the ms2dirty grid stage; calls[2] */
+/* per-stage times summed over calls, out[16] = {ms2dirty[7], dirty2ms[7],
+ * dense, dense2} in ms, stages: plan, sort, records, grid/degrid, fft rows,
+ * fft columns, image (epilogue/prologue); dense / dense2 = the MFMA
+ * gridding / degridding kernel's part of the ms2dirty / dirty2ms grid
+ * stage; calls[2] */
 int cg_ctx_stage_times(void *ctx, double *out, long long *calls);
 int cg_ctx_reset_times(void *ctx);
 /* parameters of the last call: nf1 nf2 nf3 kw beta es_c pix_x pix_y
@@ -285,6 +286,12 @@ int cg_plan_query(int nrow, int nxdirty, int nydirty, double fov, double freq,
  * permutation (int32), tile starts (int32) with n[0] = rows, n[1] = tiles.  Option "stop_after" (1: after gridding or
  * the dirty2ms prologue, 2: after the plane FFTs) stops a call early. */
 int cg_ctx_grid(void *ctx, void **ptr, long long *cells);
+/* host only: per-tap polynomial coefficients of the complex64 kernel values
+ * psi(d0 + t) / e^beta in x = 2 d0 + kw - 1 (out[t * 12 + q], q <= deg <
+ * 12), for the ES kernel (kq NULL) or the learned one (ndeg + 1 coefficients
+ * of its exponent's polynomial in s = 2 c d^2 - 1) */
+int cg_tap_poly(int kw, double beta, double c, const float *kq, int ndeg, int deg,
+                float *out);
 /* Order of the spectrum's column axis in the grid after the forward column
  * FFT: returns 1 and n = na * nb when frequency ka + na * kb is stored at
  * row ka * nb + kb ("split order"), 0 when rows are in natural order.  The

@@ -135,11 +135,12 @@ def test_gridding_stage_vs_oracle(gpu, single, case):
 
 
 @pytest.mark.parametrize("case", [(3000, 64, 96, 10.0, 1e-6, 1), (20000, 256, 256, 2.0, 1e-5, 1),
-                                  (131072, 1024, 1024, 2.0, 1e-5, 64)])
+                                  (131072, 1024, 1024, 2.0, 1e-5, 64),
+                                  (20000, 128, 128, 5.0, 1e-3, 1), (50000, 200, 200, 3.0, 1e-4, 1)])
 def test_dense_gridding_vs_oracle(gpu, case):
-    """Bins holding at least dense_min points are gridded by per-start-cell
-    MFMA products (complex64): the planes must match the oracle's spreading
-    and the all-sparse path."""
+    """Bins holding at least dense_min points are gridded by MFMA products
+    over quads of start cells (complex64, kw 4 to 7): the planes must match
+    the oracle's spreading and the all-sparse path."""
     torch = gpu
     from cugridder_amd import Context
     nrow, nx, ny, fov, eps, dmin = case
@@ -170,6 +171,38 @@ def test_dense_gridding_vs_oracle(gpu, case):
     assert rel_l2(grids[dmin], grids[0]) < 3e-6
 
 
+@pytest.mark.parametrize("case", [(131072, 1024, 1024, 2.0, 1e-5), (60000, 256, 256, 2.0, 1e-6),
+                                  (60000, 128, 160, 5.0, 1e-3), (60000, 200, 200, 3.0, 1e-4)])
+def test_dense_degridding_vs_oracle(gpu, case):
+    """dirty2ms with the tiles of many points interpolated by MFMA products
+    over quads of start cells (complex64, kw 4 to 7, quads below the point
+    threshold one lane per point) against the all-scalar degridder and the
+    oracle."""
+    torch = gpu
+    from cugridder_amd import Context
+    nrow, nx, ny, fov, eps = case
+    rng = np.random.default_rng(nrow + nx)
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = rng.normal(0, 1500 if nx == 1024 else 300, (nrow, 2))
+    uvw[:, 2] = rng.uniform(-300, 300, nrow)
+    f0 = 1e8
+    wgt = rng.uniform(0.5, 1.5, nrow).astype(np.float32)
+    img = (rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))).astype(np.complex64)
+    tu, tw, ti = (torch.from_numpy(a).cuda() for a in (uvw, wgt, img))
+    out = {}
+    for d in (0, 1):
+        ctx = Context(degrid_dense_min=d)
+        try:
+            out[d] = ctx.dirty2ms(tu, ti, tw, fov, f0, eps).cpu().numpy()
+            p = ctx.last_plan()
+        finally:
+            ctx.close()
+    assert p["kw"] <= 7 and p["nf3"] <= 32, "case must be eligible for the dense path"
+    ref = O.dirty2ms(uvw, f0, img.astype(np.complex128), wgt.astype(np.float64), fov, eps, single=True)
+    assert rel_l2(out[1], out[0]) < 2e-6
+    assert rel_l2(out[1], ref) < max(eps, 2e-6)
+
+
 @pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("shape", [(48, 80), (100, 64), (160, 50), (128, 64)])
 def test_dirty2ms_prologue_vs_oracle(gpu, single, shape):
@@ -195,3 +228,32 @@ def test_dirty2ms_prologue_vs_oracle(gpu, single, shape):
     g = got[:, keep][:, :, live]
     r = ref[:, rows[keep]][:, :, live]
     assert rel_l2(g, r) < (1e-6 if single else 1e-13)
+
+
+@pytest.mark.parametrize("eps", [1e-5, 1e-6, 1e-4])
+def test_tap_polynomial_kernel_values(gpu, eps):
+    """The MFMA gridder and degridder with kernel values from per-tap
+    polynomials (option tap_poly) against the exp2 form and the oracle."""
+    torch = gpu
+    from cugridder_amd import Context
+    rng = np.random.default_rng(int(-np.log10(eps)))
+    nrow, nx, ny, fov, f0 = 131072, 512, 512, 2.0, 1e8
+    uvw = np.empty((nrow, 3))
+    uvw[:, :2] = rng.normal(0, 600, (nrow, 2))
+    uvw[:, 2] = rng.uniform(-300, 300, nrow)
+    vis = (rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)).astype(np.complex64)
+    img = (rng.standard_normal((nx, ny)) + 1j * rng.standard_normal((nx, ny))).astype(np.complex64)
+    tu, tv, ti = (torch.from_numpy(a).cuda() for a in (uvw, vis, img))
+    res = {}
+    for tp in (0, 1):
+        ctx = Context(tap_poly=tp, dense_min=1, degrid_dense_min=1)
+        try:
+            res[tp] = (ctx.ms2dirty(tu, tv, None, nx, ny, fov, f0, eps).cpu().numpy(),
+                       ctx.dirty2ms(tu, ti, None, fov, f0, eps).cpu().numpy())
+        finally:
+            ctx.close()
+    ref_d = O.ms2dirty(uvw, f0, vis.astype(np.complex128), None, nx, ny, fov, eps, single=True)
+    ref_v = O.dirty2ms(uvw, f0, img.astype(np.complex128), None, fov, eps, single=True)
+    for k, ref in ((0, ref_d), (1, ref_v)):
+        assert rel_l2(res[1][k], res[0][k]) < max(0.2 * eps, 1e-6)
+        assert rel_l2(res[1][k], ref) < max(eps, 2e-6)
