@@ -878,7 +878,8 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16, CB % 64 == 0 ? 4 : 2) void k
 //   inverse  x[n] = w^(n ka) summed over ka of IFFT_M(X[ka + P kb] over kb)[n mod M]
 // (w = e^(dir 2 pi i / n2), n_r the band row of residue r).  A workgroup
 // holds CB adjacent live columns (16 for M = 1024, 8 for M = 2048, so the
-// transform buffer is 128 KB of LDS); thread (e, b) keeps the band rows of
+// transform buffer is 128 KB of LDS, next to a 16 KB table of the length-M
+// roots the passes use); thread (e, b) keeps the band rows of
 // residues e + S j (S = 1024 / CB, j < 16) of column b in registers.  Those
 // residues are the inputs of the first radix-16 Stockham butterfly, which
 // therefore runs in registers; the other two passes run in LDS.  The
@@ -889,9 +890,9 @@ __global__ __launch_bounds__(CB * (1 << LOGL) / 16, CB % 64 == 0 ? 4 : 2) void k
 constexpr int kBandTwSh = 7;  // two-level twiddle table: 128 low + n2 / 128 high entries
 
 // one Stockham pass of radix R (stage p = PP) over the CB columns of length
-// M in LDS, 1024 threads
+// M in LDS, 1024 threads; mt[t] = e^{2 pi i t / M} (an LDS table)
 template <int R, int PP, int DIR, int M, int CB>
-__device__ __forceinline__ void band_pass(float2 *buf, const TwTab<float> &tab, int tws)
+__device__ __forceinline__ void band_pass(float2 *buf, const float2 *mt)
 {
     constexpr int Tn = M / R, K = CB * Tn / 1024, LCB = CB == 16 ? 4 : 3;
     static_assert(K >= 1 && CB * Tn % 1024 == 0 && (1 << LCB) == CB, "shape");
@@ -913,7 +914,7 @@ __device__ __forceinline__ void band_pass(float2 *buf, const TwTab<float> &tab, 
         const int kk = i % PP;
 #pragma unroll
         for (int r = 1; r < R; r++) {
-            float2 w = tab.get(r * kk * (M / (PP * R)) * tws);
+            float2 w = mt[r * kk * (M / (PP * R))];
             if (DIR < 0) w.y = -w.y;
             u[k][r] = cmul<float>(u[k][r], w);
         }
@@ -932,8 +933,7 @@ __device__ __forceinline__ void band_pass(float2 *buf, const TwTab<float> &tab, 
 // its own (1024 is a multiple of CB).  The caller may rewrite buf after this
 // returns (every read precedes its barrier).
 template <int R, int PP, int DIR, int M, int CB, typename F>
-__device__ __forceinline__ void band_pass_out(const float2 *buf, const TwTab<float> &tab, int tws,
-                                              F &&out)
+__device__ __forceinline__ void band_pass_out(const float2 *buf, const float2 *mt, F &&out)
 {
     constexpr int Tn = M / R, K = CB * Tn / 1024, LCB = CB == 16 ? 4 : 3;
     static_assert(K >= 1 && CB * Tn % 1024 == 0 && (1 << LCB) == CB, "shape");
@@ -953,7 +953,7 @@ __device__ __forceinline__ void band_pass_out(const float2 *buf, const TwTab<flo
         const int kk = i % PP;
 #pragma unroll
         for (int r = 1; r < R; r++) {
-            float2 w = tab.get(r * kk * (M / (PP * R)) * tws);
+            float2 w = mt[r * kk * (M / (PP * R))];
             if (DIR < 0) w.y = -w.y;
             u[k][r] = cmul<float>(u[k][r], w);
         }
@@ -984,13 +984,17 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
     const int w = blockIdx.x;
     const int cb = M == 2048 ? 2 * (8 * (w >> 4) + (w & 7)) + ((w >> 3) & 1) : w;
     if (cb * CB >= nlive) return;  // padding workgroups (uniform, before any barrier)
-    const TwTab<T> tab = load_twtab<T>(buf + M * CB, tw2, n2, kBandTwSh);
+    const TwTab<T> tab = load_twtab<T>(buf + M * CB + M, tw2, n2, kBandTwSh);
+    // the length-M roots of the passes, one LDS read per twiddle
+    float2 *mt = buf + M * CB;
+    const int tws = n2 / M;
+    for (int t = threadIdx.x; t < M; t += blockDim.x) mt[t] = tw2[t * tws];
     const int tid = threadIdx.x, b = tid & (CB - 1), e = tid >> LCB;
     const int c = cb * CB + b;
     const bool cok = c < nlive;
     const int lc = cok ? live.at(c) : 0;
     float2 *pl = data + (size_t)blockIdx.y * n2 * n1;
-    const int mask = n2 - 1, tws = n2 / M;
+    const int mask = n2 - 1;
     auto dead = [&](int k) { return k >= spec_a && k < spec_b; };
     // split-order row of frequency k (na = 2^nash)
     auto srow = [&](int k) { return (k & ((1 << nash) - 1)) * nb + (k >> nash); };
@@ -1037,15 +1041,22 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
 #pragma unroll
         for (int r = 0; r < 16; r++) buf[(16 * eo + r) * CB + b] = v[r];
         lds_barrier();
-        band_pass<16, 16, DIR, M, CB>(buf, tab, tws);
+        band_pass<16, 16, DIR, M, CB>(buf, mt);
         if constexpr (DIR > 0) {
-            // the last pass stores its live outputs straight to the spectrum
-            auto store = [&](int, int kb, float2 val) {
-                const int k = ka + np * kb;
-                if (cok && !dead(k)) pl[(size_t)srow(k) * n1 + lc] = val;
+            // the last pass stores its live outputs straight to the spectrum.
+            // Output r of a butterfly is frequency k0 + np 256 r; np 256 =
+            // n2 / 8 (M = 2048) or n2 / 4 (M = 1024) is a multiple of the
+            // split factor na (powers of two, na <= 256 <= n2 / 8), so its
+            // split-order row is srow(k0) + r np 256 / na
+            constexpr int R = M == 1024 ? 4 : 8;
+            const int rstep = (np * 256) >> nash;
+            auto store = [&](int slot, int kb, float2 val) {
+                const int r = slot % R, k0 = ka + np * (kb - 256 * r);
+                const int k = k0 + np * 256 * r;
+                if (cok && !dead(k)) pl[(size_t)(srow(k0) + r * rstep) * n1 + lc] = val;
             };
-            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, tab, tws, store);
-            else band_pass_out<8, 256, DIR, M, CB>(buf, tab, tws, store);
+            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, mt, store);
+            else band_pass_out<8, 256, DIR, M, CB>(buf, mt, store);
             continue;  // buf was read before the pass's barrier
         } else {
             // the last pass's outputs (residue i + 256 r of butterfly i, slot
@@ -1056,8 +1067,8 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
                 xb[slot].x += z.x;
                 xb[slot].y += z.y;
             };
-            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, tab, tws, acc);
-            else band_pass_out<8, 256, DIR, M, CB>(buf, tab, tws, acc);
+            if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, mt, acc);
+            else band_pass_out<8, 256, DIR, M, CB>(buf, mt, acc);
         }
     }
     if constexpr (DIR < 0) {
@@ -1463,7 +1474,7 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         // M = 1024 (16 columns per workgroup) for bands of <= 1024 rows, else
         // M = 2048 (8 columns); the transform buffer is 128 KB either way
         const int M = band_len <= 1024 ? 1024 : 2048, CB = M == 1024 ? 16 : 8;
-        const size_t lds = sizeof(float2) * ((size_t)M * CB + (1 << kBandTwSh) + (n2 >> kBandTwSh));
+        const size_t lds = sizeof(float2) * ((size_t)M * CB + M + (1 << kBandTwSh) + (n2 >> kBandTwSh));
         if (band_len > 0 && band_len <= M && lds <= (size_t)kLdsMax && band_s >= 0 && band_s < n2 &&
             fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
             n2 >= 2 * M && fft_col_split(n2, na, nb)) {
