@@ -368,9 +368,11 @@ static GridArgs grid_args(Ctx &C, const WPlan &P, const Tiling &t, int64_t m, in
     g.f_e1 = (float)std::exp(g.beta);
     g.f_e2 = (float)std::exp(2.0 * g.beta);
     g.e3 = std::exp(3.0 * g.beta);
-    // per-tap polynomials of the same normalised values (complex64 MFMA kernels)
+    // per-tap polynomials of the same normalised values (complex64 MFMA
+    // kernels): on request, and always for a learned kernel, whose exponent
+    // polynomial would otherwise cost 16 more FMAs per tap
     g.tp_on = 0;
-    if (C.tap_poly && P.conv.kw <= 8) {
+    if ((C.tap_poly || g.learned) && P.conv.kw <= 8) {
         tap_polynomials(P.conv.kw, P.conv.beta, P.conv.es_c, g.learned ? g.kq : nullptr,
                         LEARN_MAXDEG, 10, g.tp);
         g.tp_on = 1;
