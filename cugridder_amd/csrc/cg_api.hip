@@ -309,14 +309,17 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
     CG_TRY(ensure(C.scan, sizeof(int32_t) * std::max(scan_tmp_size(hs), scan_tmp_size(ntiles + 1))));
     CG_TRY(ensure(C.tstart, sizeof(int32_t) * (ntiles + 1)));
+    // an odd number of radix passes starts in the second buffers, so the
+    // sorted keys and permutation end in C.keys / C.vals without a copy
+    const int bits = sort_key_bits(a, ntiles);
+    const bool odd = radix_passes(bits) & 1;
+    uint32_t *k0 = as<uint32_t>(odd ? C.keys2 : C.keys), *k1 = as<uint32_t>(odd ? C.keys : C.keys2);
+    int32_t *v0 = as<int32_t>(odd ? C.vals2 : C.vals), *v1 = as<int32_t>(odd ? C.vals : C.vals2);
     if (rec)
-        CG_TRY((launch_keys_prerecords<TU, float2, float>(uvw, vis, wgt, M, a, 0, as<uint32_t>(C.keys),
-                                                           as<int32_t>(C.vals), rec, C.stream)));
+        CG_TRY((launch_keys_prerecords<TU, float2, float>(uvw, vis, wgt, M, a, 0, k0, v0, rec, C.stream)));
     else
-        CG_TRY(launch_keys<TU>(uvw, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
-    CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                             as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles), as<int32_t>(C.hist),
-                             as<int32_t>(C.scan), C.stream));
+        CG_TRY(launch_keys<TU>(uvw, M, a, k0, v0, C.stream));
+    CG_TRY(launch_radix_sort(k0, v0, k1, v1, M, bits, as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream, 0));
     CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a), as<int32_t>(C.tstart),
                               C.stream));
     return CG_OK;
@@ -1174,17 +1177,19 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         const int64_t hs = radix_hist_size(M);
         CG_TRY(ensure(C.hist, sizeof(int32_t) * hs));
         CG_TRY(ensure(C.scan, sizeof(int32_t) * scan_tmp_size(hs)));
+        // odd pass counts start in the second buffers (no copy at the end)
+        const int bits = sort_key_bits(a, ntiles);
+        const bool odd = radix_passes(bits) & 1;
+        uint32_t *k0 = as<uint32_t>(odd ? C.keys2 : C.keys), *k1 = as<uint32_t>(odd ? C.keys : C.keys2);
+        int32_t *v0 = as<int32_t>(odd ? C.vals2 : C.vals), *v1 = as<int32_t>(odd ? C.vals : C.vals2);
         if constexpr (sizeof(T) == 4 && std::is_same<TU, float>::value) {
             if (rec)
                 CG_TRY(launch_keys_prerecords_xy(x, y, type == 1 ? (const float2 *)c : nullptr, M, a,
-                                                 as<uint32_t>(C.keys), as<int32_t>(C.vals), rec,
-                                                 C.stream));
+                                                 k0, v0, rec, C.stream));
         }
-        if (!rec)
-            CG_TRY(launch_keys_xy<TU>(x, y, M, a, as<uint32_t>(C.keys), as<int32_t>(C.vals), C.stream));
-        CG_TRY(launch_radix_sort(as<uint32_t>(C.keys), as<int32_t>(C.vals), as<uint32_t>(C.keys2),
-                                 as<int32_t>(C.vals2), M, sort_key_bits(a, ntiles),
-                                 as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream));
+        if (!rec) CG_TRY(launch_keys_xy<TU>(x, y, M, a, k0, v0, C.stream));
+        CG_TRY(launch_radix_sort(k0, v0, k1, v1, M, bits, as<int32_t>(C.hist), as<int32_t>(C.scan),
+                                 C.stream, 0));
         CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a),
                                   as<int32_t>(C.tstart), C.stream));
     } else {

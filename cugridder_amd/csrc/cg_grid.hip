@@ -774,7 +774,7 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
         __syncthreads();
         const int li = s_tile;
         if (li >= nlist) break;
-        const int tile = list[1 + li];
+        const int tile = __builtin_amdgcn_readfirstlane(list[1 + li]);
         const int bx = tile % g.ntx, by = tile / g.ntx;
         for (int q = threadIdx.x; q < 2 * rcells; q += blockDim.x) are[q] = 0.0;
         cell_runs(keys, tile_start[tile], tile_start[tile + 1], (uint32_t)g.kr.lower(tile, ntiles),
@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
         __syncthreads();
         const int li = s_tile;
         if (li >= nlist) break;
-        const int tile = list[1 + li];
+        const int tile = __builtin_amdgcn_readfirstlane(list[1 + li]);
         const int bx = tile % g.ntx, by = tile / g.ntx;
         const int x0 = bx * g.tx, y0 = by * g.ty;
         // the tile and its halo of every plane
@@ -1405,9 +1405,11 @@ bool degrid_dense_supported(const GridArgs &g, int kw)
            (size_t)degrid_lds_bytes(g, kw, 8) + kDegrid2Static <= 163840 - 64;
 }
 
-// One tile per workgroup (list == nullptr: tile blockIdx.x), or a
-// persistent grid taking the tiles of list (k_sparse_tiles) one at a time.
-template <typename T, int KW, int PPT>
+// One tile per workgroup (LISTED false: tile blockIdx.x), or a persistent
+// grid taking the tiles of list (k_sparse_tiles) one at a time.  Two
+// instantiations, so the one-tile form keeps its scalar registers free of
+// the work counter's state.
+template <typename T, int KW, int PPT, bool LISTED>
 __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ geo,
                                                 const cx_t<T> *__restrict__ fac,
                                                 const int32_t *__restrict__ perm,
@@ -1416,21 +1418,21 @@ __global__ __launch_bounds__(1024) void k_degrid(const geo_t<T> *__restrict__ ge
                                                 cx_t<T> *__restrict__ out, GridArgs g, int is2d,
                                                 int32_t *__restrict__ list, int64_t ntiles)
 {
-    __shared__ int s_tile;
-    const int nlist = list ? list[0] : 1;
-    for (int round = 0;; round++) {
-        int tile = blockIdx.x;
-        if (list) {
+    if constexpr (!LISTED) {
+        degrid_tile<T, KW, PPT>(geo, fac, perm, tile_start, fw, out, g, is2d, blockIdx.x);
+    } else {
+        __shared__ int s_tile;
+        const int nlist = list[0];
+        for (;;) {
             if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
             __syncthreads();
             const int li = s_tile;
             if (li >= nlist) break;
-            tile = list[1 + li];
-        } else if (round > 0) {
-            break;
+            // uniform across the workgroup: tile-derived addresses stay scalar
+            degrid_tile<T, KW, PPT>(geo, fac, perm, tile_start, fw, out, g, is2d,
+                                    __builtin_amdgcn_readfirstlane(list[1 + li]));
+            __syncthreads();  // the staged planes and s_tile are reused
         }
-        degrid_tile<T, KW, PPT>(geo, fac, perm, tile_start, fw, out, g, is2d, tile);
-        __syncthreads();  // the staged planes and s_tile are reused
     }
 }
 
@@ -1467,13 +1469,16 @@ static int degrid_kw(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *per
         const int per_cu = std::max(1, std::min((int)(163840 / std::max<size_t>(lds, 1)), 2048 / threads));
         nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
     }
-    if (lds > 65536)
-        CG_HIP(hipFuncSetAttribute((const void *)k_degrid<T, KW, 4>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_degrid<T, KW, 4>), dim3(nblk), dim3(threads), lds, s, geo, fac, perm, ts,
-                       fw, out, g, is2d, list, ntiles);
-    CG_HIP(hipGetLastError());
-    return CG_OK;
+    auto go = [&](const void *fn, auto kern) -> int {
+        if (lds > 65536)
+            CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(threads), lds, s, geo, fac, perm, ts, fw, out, g,
+                           is2d, list, ntiles);
+        CG_HIP(hipGetLastError());
+        return CG_OK;
+    };
+    if (list) return go((const void *)k_degrid<T, KW, 4, true>, k_degrid<T, KW, 4, true>);
+    return go((const void *)k_degrid<T, KW, 4, false>, k_degrid<T, KW, 4, false>);
 }
 
 template <typename T, typename TV>

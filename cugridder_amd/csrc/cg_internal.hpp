@@ -116,11 +116,15 @@ int launch_gather_records_xy(const float4 *rec, const int32_t *perm, int64_t m, 
 
 // ---- sort (cg_sort.hip): stable LSD radix sort of (key, val) pairs by the
 // low `key_bits` bits.  tmp_keys/tmp_vals have m entries; hist needs
-// radix_hist_size(m) ints.  Result ends in keys/vals.
+// radix_hist_size(m) ints.  Result ends in keys/vals (copy_back), or, with
+// copy_back = 0, in keys/vals after an even and tmp after an odd number of
+// passes (radix_passes): a caller that starts an odd sort in its tmp
+// buffers gets the result in its keys without a copy.
 int64_t radix_hist_size(int64_t m);
+int radix_passes(int key_bits);
 int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys,
                       int32_t *tmp_vals, int64_t m, int key_bits, int32_t *hist,
-                      int32_t *scan_tmp, hipStream_t s);
+                      int32_t *scan_tmp, hipStream_t s, int copy_back = 1);
 int64_t scan_tmp_size(int64_t n);
 // exclusive scan of n ints in place
 int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *scan_tmp, hipStream_t s);

@@ -245,12 +245,22 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     }
 }
 
+int radix_passes(int key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
+
 int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys, int32_t *tmp_vals,
-                      int64_t m, int key_bits, int32_t *hist, int32_t *scan_tmp, hipStream_t s)
+                      int64_t m, int key_bits, int32_t *hist, int32_t *scan_tmp, hipStream_t s,
+                      int copy_back)
 {
-    if (m <= 1) return CG_OK;
+    const int passes = radix_passes(key_bits);
+    if (m <= 1) {
+        // the one element (if any) goes where the passes would have left it
+        if (!copy_back && (passes & 1) && m == 1) {
+            CG_HIP(hipMemcpyAsync(tmp_keys, keys, sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+            CG_HIP(hipMemcpyAsync(tmp_vals, vals, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        }
+        return CG_OK;
+    }
     int nblocks = (int)ceil_div(m, kSortTile);
-    int passes = (key_bits + kRadixBits - 1) / kRadixBits;
     uint32_t *ka = keys, *kb = tmp_keys;
     int32_t *va = vals, *vb = tmp_vals;
     for (int p = 0; p < passes; p++) {
@@ -266,7 +276,7 @@ int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys, int32_t
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (ka != keys) {
+    if (copy_back && ka != keys) {
         CG_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, s));
         CG_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * m, hipMemcpyDeviceToDevice, s));
     }
