@@ -151,56 +151,61 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *key
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// One pass: the block ranks its 4096 items by digit in LDS (stable: round,
-// then wavefront, then lane order), then writes them out in that order, so
-// the items of one digit leave as one contiguous run at the digit's global
-// offset for this block (coalesced stores instead of one scattered store
-// per item).  offs is the scanned histogram (digit-major, block-minor): the
-// block's count of digit d is the difference of consecutive offsets.
+// One pass: the block ranks its 4096 items by digit in LDS and writes them
+// out so that the items of one digit leave as one contiguous run at the
+// digit's global offset for this block (coalesced stores instead of one
+// scattered store per item).  Wavefront w owns the contiguous quarter
+// [1024 w, 1024 w + 1024) of the tile, 64 items per round: it ranks each
+// item among its own items of the same digit with ballots (a match on each
+// digit bit) and a running count per digit in its LDS row, with no
+// workgroup barrier between rounds.  One prefix over the four rows per
+// digit then places the quarters in order, so the order of a digit's items
+// is the input order (stable).  offs is the scanned histogram (digit-major,
+// block-minor).
+__device__ __forceinline__ void wave_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t *kin, const int32_t *vin, uint32_t *kout, int32_t *vout, int64_t m, int shift,
     const int32_t *offs, int nblocks)
 {
     constexpr int NW = kSortThreads / 64;
+    constexpr int QW = kSortTile / NW;  // items of a wavefront's quarter
+    constexpr int R = QW / 64;          // rounds
+    static_assert(kSortThreads == kRadix, "one thread per digit");
     __shared__ int s_gbase[kRadix];
     __shared__ int s_lstart[kRadix];
-    __shared__ int s_lrun[kRadix];
     __shared__ int s_wcnt[NW][kRadix];
-    __shared__ int s_rtot[kRadix];
     __shared__ int s_wave[65];
     __shared__ uint32_t s_key[kSortTile];
     __shared__ int32_t s_val[kSortTile];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    const int64_t hidx = (int64_t)tid * nblocks + blockIdx.x;
-    const int64_t hlen = (int64_t)kRadix * nblocks;
-    const int g0 = offs[hidx];
-    const int g1 = hidx + 1 < hlen ? offs[hidx + 1] : (int)m;
-    s_gbase[tid] = g0;
-    int tot;
-    const int ls = block_excl_scan(g1 - g0, s_wave, &tot);
-    s_lstart[tid] = ls;
-    s_lrun[tid] = ls;
-    const uint64_t lt = __lanemask_lt();
-    const int64_t base = blockIdx.x * (int64_t)kSortTile;
-    // all loads first: one memory latency per block, not one per round
-    uint32_t kk[kSortRounds];
-    int32_t vv[kSortRounds];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    s_gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
 #pragma unroll
-    for (int r = 0; r < kSortRounds; r++) {
-        const int64_t i = base + r * kSortThreads + tid;
+    for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
+    const uint64_t lt = __lanemask_lt();
+    const int64_t tbase = blockIdx.x * (int64_t)kSortTile;
+    const int64_t qbase = tbase + (int64_t)wave * QW;
+    // all loads first: one memory latency per block, not one per round
+    uint32_t kk[R];
+    int32_t vv[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t i = qbase + 64 * r + lane;
         kk[r] = i < m ? kin[i] : 0u;
         vv[r] = i < m ? vin[i] : 0;
     }
+    __syncthreads();  // counters zeroed
+    int *cnt = s_wcnt[wave];
+    int rk[R];
 #pragma unroll
-    for (int r = 0; r < kSortRounds; r++) {
-        const int64_t i = base + r * kSortThreads + tid;
-        const bool valid = i < m;
-        const uint32_t key = kk[r];
-        const int32_t val = vv[r];
-        const int d = (int)((key >> shift) & (kRadix - 1));
-#pragma unroll
-        for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
-        __syncthreads();
+    for (int r = 0; r < R; r++) {
+        const bool valid = qbase + 64 * r + lane < m;
+        const int d = (int)((kk[r] >> shift) & (kRadix - 1));
         uint64_t mask = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kRadixBits; b++) {
@@ -208,32 +213,40 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
             const uint64_t bb = __ballot(bit);
             mask &= bit ? bb : ~bb;
         }
-        const int rank = __popcll(mask & lt);
-        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(mask);
-        __syncthreads();
-        {
-            int acc = 0;
-#pragma unroll
-            for (int w = 0; w < NW; w++) {
-                const int t = s_wcnt[w][tid];
-                s_wcnt[w][tid] = acc;
-                acc += t;
-            }
-            s_rtot[tid] = acc;
-        }
-        __syncthreads();
-        if (valid) {
-            const int lp = s_lrun[d] + s_wcnt[wave][d] + rank;
-            if (lp >= 0 && lp < kSortTile) {  // always true for a consistent histogram
-                s_key[lp] = key;
-                s_val[lp] = val;
-            }
-        }
-        __syncthreads();
-        s_lrun[tid] += s_rtot[tid];
-        __syncthreads();
+        const int before = __popcll(mask & lt);
+        const int prev = cnt[d];
+        rk[r] = prev + before;
+        wave_fence();  // every lane has read its digit's count
+        if (valid && before == 0) cnt[d] = prev + __popcll(mask);
+        wave_fence();
     }
-    const int nvalid = (int)std::min<int64_t>(kSortTile, m - base);
+    __syncthreads();
+    {
+        // quarters before this one, per digit, and the block's digit starts
+        int acc = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const int t = s_wcnt[w][tid];
+            s_wcnt[w][tid] = acc;
+            acc += t;
+        }
+        int tot;
+        s_lstart[tid] = block_excl_scan(acc, s_wave, &tot);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        if (qbase + 64 * r + lane < m) {
+            const int d = (int)((kk[r] >> shift) & (kRadix - 1));
+            const int lp = s_lstart[d] + cnt[d] + rk[r];
+            if (lp >= 0 && lp < kSortTile) {  // always true for a consistent histogram
+                s_key[lp] = kk[r];
+                s_val[lp] = vv[r];
+            }
+        }
+    }
+    __syncthreads();
+    const int nvalid = (int)std::min<int64_t>(kSortTile, m - tbase);
     for (int p = tid; p < nvalid; p += kSortThreads) {
         const uint32_t key = s_key[p];
         const int d = (int)((key >> shift) & (kRadix - 1));
