@@ -716,6 +716,32 @@ __device__ __forceinline__ Runs item_runs(const int *run, int q, int tx, int &lx
     return R;
 }
 
+// Work items of a tile in descending order of their point counts (ties by
+// index), so the waves' dynamic queue starts on the largest: ord[0..nitem)
+// (nitem <= kDenseItems); cnt is scratch.  Uses the whole workgroup; ends
+// with a barrier.
+constexpr int kDenseItems = 256;
+template <int QS>
+__device__ __forceinline__ void order_items(const int *run, int nitem, int tx, int *cnt,
+                                            unsigned short *ord)
+{
+    for (int i = threadIdx.x; i < nitem; i += blockDim.x) {
+        int lx0, ly0;
+        cnt[i] = item_runs<QS>(run, i, tx, lx0, ly0).n;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nitem; i += blockDim.x) {
+        const int ci = cnt[i];
+        int rank = 0;
+        for (int j = 0; j < nitem; j++) {
+            const int cj = cnt[j];
+            rank += cj > ci || (cj == ci && j < i);
+        }
+        ord[rank] = (unsigned short)i;
+    }
+    __syncthreads();
+}
+
 template <int KW, int QS, int KM>
 __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__restrict__ geo,
                                                                 const float2 *__restrict__ val,
@@ -739,6 +765,8 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
     __shared__ unsigned short s_lst[NWAVES][kDense2Max];
     __shared__ int s_cnt[NWAVES][33], s_wb[NWAVES][33];
     __shared__ int s_next, s_tile;
+    __shared__ int s_icnt[kDenseItems];
+    __shared__ unsigned short s_ord[kDenseItems];
     const int ncell = g.tx * g.ty;
     const int nitem = ncell / (QS * QS);
     const int nlist = list[0];
@@ -781,13 +809,14 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
                   g.cshift, ncell, run);
         if (threadIdx.x == 0) s_next = 0;
         __syncthreads();
+        order_items<QS>(run, nitem, g.tx, s_icnt, s_ord);
         for (;;) {
             int it = 0;
             if (lane == 0) it = atomicAdd(&s_next, 1);
             it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
             if (it >= nitem) break;
             int lx0, ly0;
-            Runs R = item_runs<QS>(run, it, g.tx, lx0, ly0);
+            Runs R = item_runs<QS>(run, __builtin_amdgcn_readfirstlane((int)s_ord[it]), g.tx, lx0, ly0);
             R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
             R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
             R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
@@ -898,7 +927,7 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
 }
 
 // static LDS of k_grid_dense2 (sorted lists and window counters of 8 waves)
-constexpr size_t kDense2Static = (size_t)(kDense2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4) + 64;
+constexpr size_t kDense2Static = (size_t)(kDense2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4) + 64 + 6 * kDenseItems;
 
 static size_t dense_lds_bytes(const GridArgs &g, int kw)
 {
@@ -909,7 +938,7 @@ static size_t dense_lds_bytes(const GridArgs &g, int kw)
 bool grid_dense_supported(const GridArgs &g, int kw, int elem)
 {
     return elem == 8 && kw >= 2 && kw <= 8 && g.nf3 >= 2 * kw && g.nf3 <= 32 && g.cshift > 0 &&
-           g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 &&
+           g.tx * g.ty <= kDenseItems && g.tx + kw - 1 <= g.nf1 && g.ty + kw - 1 <= g.nf2 &&
            dense_lds_bytes(g, kw) + kDense2Static <= 163840 - 64;
 }
 
@@ -1151,6 +1180,8 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
     __shared__ int s_cnt[NWAVES][33], s_wb[NWAVES][33];
     __shared__ int s_run[257];
     __shared__ int s_next, s_tile;
+    __shared__ int s_icnt[kDenseItems];
+    __shared__ unsigned short s_ord[kDenseItems];
     // per-wave exchange of a group's kernel values: ky[point][y], kz[plane][point]
     __shared__ __align__(16) float s_ky[NWAVES][16 * 8];
     __shared__ __align__(16) float s_kz[NWAVES][8 * 16];
@@ -1207,13 +1238,14 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
                   g.cshift, ncell, s_run);
         if (threadIdx.x == 0) s_next = 0;
         __syncthreads();
+        order_items<QS>(s_run, nitem, g.tx, s_icnt, s_ord);
         for (;;) {
             int it = 0;
             if (lane == 0) it = atomicAdd(&s_next, 1);
             it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
             if (it >= nitem) break;
             int lx0, ly0;
-            Runs R = item_runs<QS>(s_run, it, g.tx, lx0, ly0);
+            Runs R = item_runs<QS>(s_run, __builtin_amdgcn_readfirstlane((int)s_ord[it]), g.tx, lx0, ly0);
             R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
             R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
             R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
@@ -1363,7 +1395,8 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
 
 // static LDS of k_degrid_dense
 constexpr size_t kDegrid2Static =
-    (size_t)(kDegrid2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4 + 2 * 128 * 4) + 257 * 4 + 64;
+    (size_t)(kDegrid2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4 + 2 * 128 * 4) + 257 * 4 + 64 +
+    6 * kDenseItems;
 
 template <int KW>
 static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *perm,
