@@ -420,6 +420,24 @@ def dft_accuracy(torch, W, d_img, v_out, npix_sample=64, nrow_sample=64, seed=0)
             "reference": "FP64 direct sum on the GPU (torch), sampled pixels / rows"}
 
 
+def copy_bandwidth(torch, dev, nbytes=1 << 31, reps=3):
+    """Device-to-device copy rate on this GPU (GB/s, bytes read plus
+    written): the practical HBM ceiling the roofline is also compared with."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
 def traffic_from_profiles(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
@@ -475,6 +493,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("copy probe: 1 GiB read + 1 GiB written")
+    copy_gbs = copy_bandwidth(torch, dev)
+    log("device copy: %.0f GB/s (read + write)" % copy_gbs)
     ctx = Context(timing=True, dense_min=DENSE_MIN, degrid_dense_min=DEGRID_DENSE_MIN)
 
     learned_info = None
@@ -601,6 +621,11 @@ def main():
         roof = {"kernel": "k_grid2d_mfma (type 1)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None, "algorithmic_bytes": nbytes, "avg_ms": round(t_grid, 4)}
+
+    if roof is not None:
+        roof["copy_GBs"] = round(copy_gbs, 1)
+        if roof.get("unit") == "GB/s":
+            roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 4)
 
     accuracy = None
     if args.workload in ("wstack", "plain", "learned") and not args.no_accuracy:
