@@ -258,6 +258,7 @@ struct Quad {
     int nq;
     double xq[32], gq[32];
 };
+constexpr int kChebMax = 96;
 Quad make_quad(const ConvOpts &o);
 // phi[k] = 2 sum_i g_i cos(pi k (x_i - N + 1)/(N - 1)), N = nf/2 + 1
 // (src/fourier/deconv.cu:12-26), k in [0, nf/2]
@@ -273,6 +274,11 @@ struct WArgs {
     int zb, zn, accumulate;
     double pix_x, pix_y, o_center, i_center, gamma, h;
     Quad qd;
+    // psihat(z') over the image's z' range [cheb_a, cheb_b] as a Chebyshev
+    // series of cheb_n terms (0: evaluate the quadrature sum per pixel)
+    int cheb_n;
+    double cheb_a, cheb_b;
+    double cheb[kChebMax];
 };
 WArgs make_wargs(const WPlan &p);
 // z' and psihat(z') on the (N2/2+1) x (N1/2+1) quarter image

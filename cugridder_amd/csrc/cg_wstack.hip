@@ -16,6 +16,69 @@
 
 namespace cg {
 
+// psihat(z') = sum_i g_i 2 cos(x_i z') (the quadrature of the kernel's
+// Fourier transform, deconv.cu:49-65), FP64 on the host
+static double psihat_at(const Quad &q, double zp)
+{
+    double s = 0.0;
+    for (int i = 0; i < q.nq; i++) s += q.gq[i] * 2.0 * std::cos(q.xq[i] * zp);
+    return s;
+}
+
+// Chebyshev series of psihat(z') over the z' range of the image's pixels:
+// z = n - 1 falls monotonically with l^2 + m^2 inside the horizon, so the
+// range is z' at the centre and at the quarter image's far corner.  The
+// smallest series (16 .. kChebMax terms) within 1e-14 of the direct sum on
+// 2049 points of the range is kept; none (images reaching the horizon, or
+// a series that does not converge) leaves cheb_n = 0.
+static void fit_psihat(WArgs &w)
+{
+    w.cheb_n = 0;
+    const double lx = (w.N2 / 2) * w.pix_x, my = (w.N1 / 2) * w.pix_y;
+    const double r2 = lx * lx + my * my;
+    if (!(r2 < 1.0)) return;
+    const double zfar = -r2 / (std::sqrt(1.0 - r2) + 1.0);
+    const double z0 = (0.0 - w.o_center) * w.gamma * w.h, z1 = (zfar - w.o_center) * w.gamma * w.h;
+    const double a = std::min(z0, z1), b = std::max(z0, z1);
+    if (!(b - a > 1e-9 * std::max(1.0, std::fabs(a) + std::fabs(b)))) return;
+    double ref[2049], scale = 1.0;
+    for (int t = 0; t < 2049; t++) {
+        ref[t] = psihat_at(w.qd, a + (b - a) * t / 2048.0);
+        scale = std::max(scale, std::fabs(ref[t]));
+    }
+    for (int n : {16, 24, 32, 48, 64, kChebMax}) {
+        double f[kChebMax], c[kChebMax];
+        for (int j = 0; j < n; j++) {
+            const double x = std::cos(kPi * (j + 0.5) / n);
+            f[j] = psihat_at(w.qd, a + (x + 1.0) * 0.5 * (b - a));
+        }
+        for (int k = 0; k < n; k++) {
+            double s = 0.0;
+            for (int j = 0; j < n; j++) s += f[j] * std::cos(kPi * k * (j + 0.5) / n);
+            c[k] = 2.0 * s / n;
+        }
+        c[0] *= 0.5;
+        double err = 0.0;
+        for (int t = 0; t < 2049; t++) {
+            const double x = (2.0 * (a + (b - a) * t / 2048.0) - (a + b)) / (b - a);
+            double b1 = 0.0, b2 = 0.0;
+            for (int k = n - 1; k >= 1; k--) {
+                const double tt = 2.0 * x * b1 - b2 + c[k];
+                b2 = b1;
+                b1 = tt;
+            }
+            err = std::max(err, std::fabs(x * b1 - b2 + c[0] - ref[t]));
+        }
+        if (err <= 1e-14 * scale) {
+            w.cheb_n = n;
+            w.cheb_a = a;
+            w.cheb_b = b;
+            for (int k = 0; k < n; k++) w.cheb[k] = c[k];
+            return;
+        }
+    }
+}
+
 WArgs make_wargs(const WPlan &p)
 {
     WArgs w{};
@@ -31,6 +94,7 @@ WArgs make_wargs(const WPlan &p)
     w.gamma = p.gamma;
     w.h = p.h;
     w.qd = make_quad(p.conv);
+    fit_psihat(w);
     if (!(p.nf2 > 1 && fft_col_split(p.nf2, w.cna, w.cnb))) w.cna = w.cnb = 0;
     w.skip_dead = fft_spectrum_pruned<float>(p.nf2) ? 1 : 0;
     return w;
@@ -75,7 +139,19 @@ __global__ __launch_bounds__(256) void k_quarter(WArgs w, double *zq, double *ph
     const double z = t < 0 ? sqrt(-t) - 1.0 : -(l * l + m * m) / (sqrt(t) + 1.0);
     const double zp = (z - w.o_center) * w.gamma * w.h;
     double s = 0.0;
-    for (int i = 0; i < w.qd.nq; i++) s += w.qd.gq[i] * 2.0 * cos(w.qd.xq[i] * zp);
+    if (w.cheb_n > 0) {
+        // Clenshaw recurrence of the host-fitted series
+        const double x = (2.0 * zp - (w.cheb_a + w.cheb_b)) / (w.cheb_b - w.cheb_a);
+        double b1 = 0.0, b2 = 0.0;
+        for (int k = w.cheb_n - 1; k >= 1; k--) {
+            const double t2 = 2.0 * x * b1 - b2 + w.cheb[k];
+            b2 = b1;
+            b1 = t2;
+        }
+        s = x * b1 - b2 + w.cheb[0];
+    } else {
+        for (int i = 0; i < w.qd.nq; i++) s += w.qd.gq[i] * 2.0 * cos(w.qd.xq[i] * zp);
+    }
     zq[q] = zp;
     phi3[q] = s;
 }
