@@ -149,7 +149,12 @@ int cg_ctx_destroy(void *ctx);
  * "mfma2d" (1: complex64 2-D NUFFT type 1 grids 16 x 16 quadrants as MFMA
  * products when the tiling allows; 0: LDS-atomic gridding),
  * "dense_min" (points per gridding bin from which the complex64 gridder
- * uses per-start-cell MFMA products; 0 disables), "row_mask" (1: the FFT
+ * uses MFMA products over 2 x 2 start cells; 0 disables),
+ * "degrid_dense_min" (points per degridding tile from which the complex64
+ * degridder uses MFMA products; 0 disables; default 2048),
+ * "tap_poly" (1: the complex64 MFMA kernels take kernel values from per-tap
+ * polynomials fitted at plan time instead of the exp2 form; a learned
+ * kernel always does), "row_mask" (1: the FFT
  * skips grid rows that no point reaches; 0: transforms every row),
  * "prerec" (1: complex64 records are built in the caller's order and
  * gathered as aligned slots; 0: one gather pass; results are identical),
