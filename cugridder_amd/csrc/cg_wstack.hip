@@ -12,6 +12,8 @@
 // footprint); the other columns are never read by the inverse FFT, so the
 // grid needs no memset.  Both index the spectrum's column axis in the order
 // the column FFT leaves it (split order for long columns).
+#include <vector>
+
 #include "cg_internal.hpp"
 
 namespace cg {
@@ -28,9 +30,41 @@ static double psihat_at(const Quad &q, double zp)
 // Chebyshev series of psihat(z') over the z' range of the image's pixels:
 // z = n - 1 falls monotonically with l^2 + m^2 inside the horizon, so the
 // range is z' at the centre and at the quarter image's far corner.  The
-// smallest series (16 .. kChebMax terms) within 1e-14 of the direct sum on
-// 2049 points of the range is kept; none (images reaching the horizon, or
-// a series that does not converge) leaves cheb_n = 0.
+// shortest series (16 .. kChebMax terms) whose last three coefficients are
+// below 1e-14 of the coefficient sum and which agrees with the direct sum
+// to 1e-14 at 17 points of the range is kept; none (images reaching the
+// horizon, or a series that does not converge) leaves cheb_n = 0.  Host
+// cost: a few hundred cosines per plan (the DCT's cosines are a table).
+static constexpr int kChebSizes[] = {16, 24, 32, 48, 64, kChebMax};
+
+static const std::vector<double> &cheb_cos_table(int n)
+{
+    static const std::vector<std::vector<double>> tabs = [] {
+        std::vector<std::vector<double>> t;
+        for (int m : kChebSizes) {
+            std::vector<double> c((size_t)m * m);
+            for (int k = 0; k < m; k++)
+                for (int j = 0; j < m; j++) c[(size_t)k * m + j] = std::cos(kPi * k * (j + 0.5) / m);
+            t.push_back(std::move(c));
+        }
+        return t;
+    }();
+    int i = 0;
+    while (kChebSizes[i] != n) i++;
+    return tabs[i];
+}
+
+static double clenshaw(const double *c, int n, double x)
+{
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = n - 1; k >= 1; k--) {
+        const double t = 2.0 * x * b1 - b2 + c[k];
+        b2 = b1;
+        b1 = t;
+    }
+    return x * b1 - b2 + c[0];
+}
+
 static void fit_psihat(WArgs &w)
 {
     w.cheb_n = 0;
@@ -41,41 +75,34 @@ static void fit_psihat(WArgs &w)
     const double z0 = (0.0 - w.o_center) * w.gamma * w.h, z1 = (zfar - w.o_center) * w.gamma * w.h;
     const double a = std::min(z0, z1), b = std::max(z0, z1);
     if (!(b - a > 1e-9 * std::max(1.0, std::fabs(a) + std::fabs(b)))) return;
-    double ref[2049], scale = 1.0;
-    for (int t = 0; t < 2049; t++) {
-        ref[t] = psihat_at(w.qd, a + (b - a) * t / 2048.0);
-        scale = std::max(scale, std::fabs(ref[t]));
-    }
-    for (int n : {16, 24, 32, 48, 64, kChebMax}) {
+    for (int n : kChebSizes) {
+        const std::vector<double> &ct = cheb_cos_table(n);
         double f[kChebMax], c[kChebMax];
-        for (int j = 0; j < n; j++) {
-            const double x = std::cos(kPi * (j + 0.5) / n);
-            f[j] = psihat_at(w.qd, a + (x + 1.0) * 0.5 * (b - a));
-        }
+        for (int j = 0; j < n; j++) f[j] = psihat_at(w.qd, a + (ct[(size_t)n + j] + 1.0) * 0.5 * (b - a));
+        double csum = 0.0;
         for (int k = 0; k < n; k++) {
             double s = 0.0;
-            for (int j = 0; j < n; j++) s += f[j] * std::cos(kPi * k * (j + 0.5) / n);
+            for (int j = 0; j < n; j++) s += f[j] * ct[(size_t)k * n + j];
             c[k] = 2.0 * s / n;
+            csum += std::fabs(c[k]);
         }
         c[0] *= 0.5;
-        double err = 0.0;
-        for (int t = 0; t < 2049; t++) {
-            const double x = (2.0 * (a + (b - a) * t / 2048.0) - (a + b)) / (b - a);
-            double b1 = 0.0, b2 = 0.0;
-            for (int k = n - 1; k >= 1; k--) {
-                const double tt = 2.0 * x * b1 - b2 + c[k];
-                b2 = b1;
-                b1 = tt;
-            }
-            err = std::max(err, std::fabs(x * b1 - b2 + c[0] - ref[t]));
+        const double tail = std::max(std::fabs(c[n - 1]), std::max(std::fabs(c[n - 2]), std::fabs(c[n - 3])));
+        if (!(tail <= 1e-14 * csum)) continue;
+        double err = 0.0, scale = 1.0;
+        for (int t = 0; t <= 16; t++) {
+            const double zp = a + (b - a) * t / 16.0;
+            const double ref = psihat_at(w.qd, zp);
+            scale = std::max(scale, std::fabs(ref));
+            err = std::max(err, std::fabs(clenshaw(c, n, (2.0 * zp - (a + b)) / (b - a)) - ref));
         }
         if (err <= 1e-14 * scale) {
             w.cheb_n = n;
             w.cheb_a = a;
             w.cheb_b = b;
             for (int k = 0; k < n; k++) w.cheb[k] = c[k];
-            return;
         }
+        return;
     }
 }
 
