@@ -54,6 +54,7 @@ struct Ctx {
     int tile_grid2d = 32;  // 2-D (one plane) gridding tiles
     int mfma2d = 1;        // 2-D complex64 gridding as per-quadrant MFMA products
     int tile_degrid = 16;
+    int tile_degrid2d = 32;  // 2-D NUFFT type 2 degridding tiles (two radix passes at 8192^2)
     int dense_min = 64;      // points per bin that send it to the MFMA gridding path (0: off)
     int degrid_dense_min = 2048;  // points per degridding tile for the MFMA degridder (0: off)
     int tap_poly = 0;        // complex64 MFMA kernels: kernel values from per-tap polynomials
@@ -192,6 +193,7 @@ static void env_overrides(Ctx &C)
     if (const char *e = std::getenv("CG_BAND_FFT")) C.band_fft = std::atoi(e);
     if (const char *e = std::getenv("CG_COMPACT_KEYS")) C.compact_keys = std::atoi(e);
     if (const char *e = std::getenv("CG_TILE_DEGRID")) C.tile_degrid = std::atoi(e);
+    if (const char *e = std::getenv("CG_TILE_DEGRID2D")) C.tile_degrid2d = std::atoi(e);
     if (const char *e = std::getenv("CG_GRID_LIMIT_MB")) C.grid_limit_mb = std::atoi(e);
     if (const char *e = std::getenv("CG_TAP_POLY")) C.tap_poly = std::atoi(e);
 }
@@ -1140,7 +1142,7 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
     const int eb = (int)sizeof(cx_t<T>);
     const int kw = conv.kw;
     Tiling t = type == 1 ? choose_tiling(P.nf1, P.nf2, 1, kw, 16, C.grid_lds_budget, C.tile_grid2d, 0)
-                         : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid, kw - 1);
+                         : choose_tiling(P.nf1, P.nf2, 1, kw, eb, C.lds_budget, C.tile_degrid2d, kw - 1);
     C.last_plan = P;
     C.last_tiling = t;
     CoordArgs a{};
@@ -1599,6 +1601,7 @@ int cg_ctx_set_option(void *ctx, const char *name, int value)
     else if (!std::strcmp(name, "band_fft")) C->band_fft = value;
     else if (!std::strcmp(name, "compact_keys")) C->compact_keys = value;
     else if (!std::strcmp(name, "tile_degrid")) C->tile_degrid = value;
+    else if (!std::strcmp(name, "tile_degrid2d")) C->tile_degrid2d = value;
     else if (!std::strcmp(name, "stop_after")) C->stop_after = value;
     else if (!std::strcmp(name, "dense_min")) C->dense_min = value;
     else if (!std::strcmp(name, "degrid_dense_min")) C->degrid_dense_min = value;
@@ -1968,7 +1971,7 @@ int cg_nufft2d_noalloc(void *stream, int type, int prec, int M, const void *d_x,
     }
     Ctx &C = *D.ctx;
     const size_t need = nufft2d_workspace(type, prec, M, N1, N2, epsilon, sigma, C.grid_lds_budget,
-                                          C.lds_budget, C.tile_grid2d, C.tile_degrid);
+                                          C.lds_budget, C.tile_grid2d, C.tile_degrid2d);
     if (need == 0) return CG_ERR_ARG;
     if (need > kScratchBytes) {
         if (!std::getenv("CG_QUIET"))

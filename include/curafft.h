@@ -146,6 +146,7 @@ int cg_ctx_destroy(void *ctx);
  * staging bytes), "grid_lds_budget" (gridding accumulator bytes),
  * "tile_grid", "tile_degrid" (preferred tile edge, power of two <= 32),
  * "tile_grid2d" (the same for 2-D NUFFT type 1, <= 64; default 32),
+ * "tile_degrid2d" (the same for 2-D NUFFT type 2; default 32),
  * "mfma2d" (1: complex64 2-D NUFFT type 1 grids 16 x 16 quadrants as MFMA
  * products when the tiling allows; 0: LDS-atomic gridding),
  * "dense_min" (points per gridding bin from which the complex64 gridder

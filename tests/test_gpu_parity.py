@@ -142,11 +142,11 @@ def _clustered_xy(rng, m):
 @pytest.mark.parametrize("tile,N1,N2,eps", [(8, 96, 64, 1e-5), (16, 96, 64, 1e-5), (32, 96, 64, 1e-5),
                                              (64, 96, 64, 1e-5), (32, 32, 48, 2e-6), (16, 40, 24, 1e-3)])
 def test_nufft2d_tiles_and_records(gpu, tile, N1, N2, eps):
-    """complex64 2-D NUFFT with clustered points: every type-1 gridding tile
-    size against the FP32-ABI oracle; the MFMA quadrant gridder (tiles of 16
-    or 32) against the LDS-atomic one; the two record paths bit-identical;
-    type 2 (whose tiling the options leave alone) unchanged.  32 x 48 modes
-    give a grid of two tiles along x at tile 32."""
+    """complex64 2-D NUFFT with clustered points: every gridding and
+    degridding tile size (type 1 and type 2) against the FP32-ABI oracle;
+    the MFMA quadrant gridder (tiles of 16 or 32) against the LDS-atomic one;
+    the two record paths bit-identical, and type 2 unchanged by the type-1
+    options.  32 x 48 modes give a grid of two tiles along x at tile 32."""
     torch = gpu
     from cugridder_amd import Context
     rng = np.random.default_rng(40 + tile + N1)
@@ -157,7 +157,7 @@ def test_nufft2d_tiles_and_records(gpu, tile, N1, N2, eps):
     tx, ty, tc, tf = (torch.from_numpy(a).cuda() for a in (x, y, c, fk))
     out = {}
     for prerec, mfma in ((1, 1), (0, 1), (1, 0)):
-        ctx = Context(tile_grid2d=tile, prerec=prerec, mfma2d=mfma)
+        ctx = Context(tile_grid2d=tile, tile_degrid2d=tile, prerec=prerec, mfma2d=mfma)
         try:
             f1 = ctx.nufft2d1(tx, ty, tc, N1, N2, eps).cpu().numpy()
             c2 = ctx.nufft2d2(tx, ty, tf, eps).cpu().numpy()
