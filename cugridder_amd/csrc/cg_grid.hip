@@ -76,10 +76,13 @@ struct KernelEvalN {
 };
 
 // Kernel-value source of the complex64 MFMA kernels, fixed per
-// instantiation: the ES exp2 form, the learned exp2 form, or the per-tap
-// polynomials (tap_on).
+// instantiation: the ES exp2 form or the per-tap polynomials (tp_on; KernelEvalN
+// also takes kKerLearned for the exact learned form).
 enum : int { kKerES = 0, kKerLearned = 1, kKerPoly = 2 };
-inline int ker_mode(const GridArgs &g) { return g.tp_on ? kKerPoly : g.learned ? kKerLearned : kKerES; }
+// the MFMA kernels are instantiated for the ES form and the polynomials: a
+// learned kernel always comes with its polynomials (grid_args), so a
+// learned launch without them is refused
+inline int ker_mode(const GridArgs &g) { return g.tp_on ? kKerPoly : g.learned ? -1 : kKerES; }
 
 // FP32 kernel values psi(off0 + t) / e^beta (the normalisation of
 // KernelEvalN) from per-tap polynomials in x = 2 off0 + kw - 1 (host fit,
@@ -969,9 +972,8 @@ static int grid_dense2_kw(const float4 *geo, const float2 *val, const uint32_t *
     auto pick = [&](auto qs) -> int {
         constexpr int QS = decltype(qs)::value;
         if (km == kKerPoly) return go((const void *)k_grid_dense2<KW, QS, kKerPoly>, k_grid_dense2<KW, QS, kKerPoly>);
-        if (km == kKerLearned)
-            return go((const void *)k_grid_dense2<KW, QS, kKerLearned>, k_grid_dense2<KW, QS, kKerLearned>);
-        return go((const void *)k_grid_dense2<KW, QS, kKerES>, k_grid_dense2<KW, QS, kKerES>);
+        if (km == kKerES) return go((const void *)k_grid_dense2<KW, QS, kKerES>, k_grid_dense2<KW, QS, kKerES>);
+        return CG_ERR_ARG;
     };
     if constexpr (KW <= 7)
         if (g.tx % 2 == 0 && g.ty % 2 == 0) return pick(std::integral_constant<int, 2>{});
@@ -1422,9 +1424,8 @@ static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *
     auto pick = [&](auto qs) -> int {
         constexpr int QS = decltype(qs)::value;
         if (km == kKerPoly) return go((const void *)k_degrid_dense<KW, QS, kKerPoly>, k_degrid_dense<KW, QS, kKerPoly>);
-        if (km == kKerLearned)
-            return go((const void *)k_degrid_dense<KW, QS, kKerLearned>, k_degrid_dense<KW, QS, kKerLearned>);
-        return go((const void *)k_degrid_dense<KW, QS, kKerES>, k_degrid_dense<KW, QS, kKerES>);
+        if (km == kKerES) return go((const void *)k_degrid_dense<KW, QS, kKerES>, k_degrid_dense<KW, QS, kKerES>);
+        return CG_ERR_ARG;
     };
     if constexpr (KW <= 7)
         if (g.tx % 2 == 0 && g.ty % 2 == 0) return pick(std::integral_constant<int, 2>{});

@@ -165,6 +165,39 @@ def test_learned_imaging_vs_dft(L, trained):
         es.close()
 
 
+def test_learned_kernel_on_the_mfma_paths(L, trained):
+    """A learned kernel in the MFMA gridder and degridder (every bin dense)
+    takes per-tap polynomials fitted to its values; against the sparse
+    kernels, which evaluate the learned exponent exactly, and against the
+    FP64 direct DFT."""
+    import torch
+    from cugridder_amd import Context
+    kw, eps, n = 6, 1e-5, 128
+    uvw, vis, fov, freq, uvw_t, vis_t = _problem(L, n, 4096, seed=5)
+    pix = fov * np.pi / 180 / n
+    ref = O.dft_ms2dirty(uvw, freq, vis, None, n, n, pix, pix).reshape(n, n)
+    rng = np.random.default_rng(6)
+    img = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    img_t = torch.from_numpy(img).to(uvw_t.device)
+    v_ref = O.dft_dirty2ms(uvw, freq, img.astype(np.complex128), None, pix, pix)
+    out = {}
+    for dense in (1, 0):
+        ctx = Context(dense_min=dense, degrid_dense_min=dense)
+        try:
+            ctx.set_kernel(kw, trained[kw][0], eps)
+            d = ctx.ms2dirty(uvw_t, vis_t, None, n, n, fov, freq, eps).cpu().numpy()
+            assert ctx.last_plan()["kw"] == kw
+            v = ctx.dirty2ms(uvw_t, img_t, None, fov, freq, eps).cpu().numpy()
+            out[dense] = (d, v)
+        finally:
+            ctx.close()
+    (d1, v1), (d0, v0) = out[1], out[0]
+    print("MFMA vs sparse: %.3e %.3e; vs DFT %.3e %.3e"
+          % (rel_l2(d1, d0), rel_l2(v1, v0), rel_l2(d1, ref), rel_l2(v1, v_ref)))
+    assert rel_l2(d1, d0) <= 2e-6 and rel_l2(v1, v0) <= 2e-6
+    assert rel_l2(d1, ref) <= eps and rel_l2(v1, v_ref) <= eps
+
+
 def test_certification_accepts_and_rejects(L, trained):
     """The ES judge (support 7, eps 1e-6) accepts the support-6 kernel at
     1e-5 (the learned workload's bar) and rejects supports 5 and 4 there
