@@ -636,7 +636,7 @@ static int exec_ms2dirty(Ctx &C, const CallPlan &cp, const TU *uvw, const cx_t<T
         CG_TRY(launch_grid<T>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val), as<int32_t>(C.tstart),
                               as<cx_t<T>>(C.grid), g, P.conv.kw, C.stream));
         if (cp.dense_min > 0) {
-            CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
+            CG_TRY(ensure(C.dlist, sizeof(int32_t) * dense_list_ints(t.ntiles(), g.m)));
             CG_TRY(clk.mark(ST_COUNT + 1));
             CG_TRY(launch_grid_dense((const geo_t<float> *)C.geo.p, (const cx_t<float> *)C.val.p,
                                      as<uint32_t>(C.keys), as<int32_t>(C.tstart), as<int32_t>(C.dlist),
@@ -815,7 +815,7 @@ static int exec_dirty2ms(Ctx &C, const CallPlan &cp, const TU *uvw, cx_t<T> *vis
         g.dense_min = 0;
         if (sizeof(T) == 4 && M > 0 && C.degrid_dense_min > 0 && degrid_dense_supported(g, kw))
             g.dense_min = C.degrid_dense_min;
-        CG_TRY(ensure(C.dlist, sizeof(int32_t) * (t.ntiles() + 2)));
+        CG_TRY(ensure(C.dlist, sizeof(int32_t) * dense_list_ints(t.ntiles(), g.m)));
         if (M > 0)
             CG_TRY((launch_degrid<T, cx_t<T>>(as<geo_t<T>>(C.geo), as<cx_t<T>>(C.val),
                                               as<int32_t>(C.vals), as<int32_t>(C.tstart),

@@ -579,15 +579,40 @@ static int grid_kw(const geo_t<T> *geo, const cx_t<T> *val, const int32_t *ts, c
 // footprint region in LDS (FP64), and the region once per bin into the grid
 // with FP32 global atomics, on top of what the sparse kernel wrote.
 
+// Work units of the dense kernels: every bin with at least dense_min points,
+// cut into chunks of at most `chunk` consecutive sorted points, so that one
+// bin holding a large share of the points (a small image with many rows) is
+// spread over many workgroups.  list[0] counts the units, list[1] is the
+// dense kernels' work counter, unit e is (bin, first point offset) at
+// list[2 + 2 e], list[3 + 2 e] (any order).  Needs dense_list_ints ints.
 __global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ tile_start,
-                                                    int64_t ntiles, int dense_min,
+                                                    int64_t ntiles, int dense_min, int chunk,
                                                     int32_t *__restrict__ list)
 {
-    // list[0] counts the dense bins, list[1..] holds them (any order);
-    // list[ntiles + 1] is the work counter of the dense kernels
     const int64_t t = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (t >= ntiles) return;
-    if (tile_start[t + 1] - tile_start[t] >= dense_min) list[1 + atomicAdd(list, 1)] = (int32_t)t;
+    const int n = tile_start[t + 1] - tile_start[t];
+    if (n < dense_min || n == 0) return;
+    const int nu = (n + chunk - 1) / chunk;
+    const int e0 = atomicAdd(list, nu);
+    for (int u = 0; u < nu; u++) {
+        list[2 + 2 * (e0 + u)] = (int32_t)t;
+        list[3 + 2 * (e0 + u)] = u * chunk;
+    }
+}
+
+int dense_chunk(int64_t m)
+{
+    // about 1024 units over the rows (four per CU), at least 4096 points each
+    int64_t c = 4096;
+    while (c < m / 1024 && c < (1 << 30)) c *= 2;
+    return (int)c;
+}
+
+int64_t dense_list_ints(int64_t ntiles, int64_t m)
+{
+    const int64_t units = ntiles + (m + dense_chunk(m) - 1) / dense_chunk(m);
+    return std::max<int64_t>(2 + 2 * units, ntiles + 2);  // also the sparse tiles' list
 }
 
 __global__ void k_zero_i32(int32_t *p, int64_t second)
@@ -745,13 +770,47 @@ __device__ __forceinline__ void order_items(const int *run, int nitem, int tx, i
     __syncthreads();
 }
 
+// The waves' queue over a work unit: pieces of at most psz consecutive
+// points of one item, items in order_items' order, so a unit whose points
+// crowd into a few start cells still occupies every wave.  piece[p] =
+// ordered item | part << 8 (points [part psz, min(n, (part + 1) psz)) of
+// it); empty items have no piece.  The pieces number at most nitem + unit
+// points / psz, i.e. <= kDensePieces when psz >= unit points / 16 (parts
+// per item <= 17).  Call after order_items (cnt, ord); the first wave
+// builds the table (a prefix sum over the items by lane shuffles); ends
+// with a barrier.
+constexpr int kDensePieces = kDenseItems + 16;
+__device__ __forceinline__ void order_pieces(const int *cnt, const unsigned short *ord, int nitem,
+                                             int psz, unsigned short *piece, int *npiece)
+{
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int base = 0;
+        for (int k0 = 0; k0 < nitem; k0 += 64) {
+            const int k = k0 + lane;
+            const int parts = k < nitem ? (cnt[ord[k]] + psz - 1) / psz : 0;
+            int x = parts;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(x, d);
+                if (lane >= d) x += y;
+            }
+            const int first = base + x - parts;
+            for (int q = 0; q < parts; q++) piece[first + q] = (unsigned short)(k | q << 8);
+            base += __shfl(x, 63);
+        }
+        if (lane == 0) *npiece = base;
+    }
+    __syncthreads();
+}
+
 template <int KW, int QS, int KM>
 __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__restrict__ geo,
                                                                 const float2 *__restrict__ val,
                                                                 const uint32_t *__restrict__ keys,
                                                                 const int32_t *__restrict__ tile_start,
                                                                 int32_t *__restrict__ list,
-                                                                int64_t ntiles,
+                                                                int chunk, int psz,
                                                                 float2 *__restrict__ fw, GridArgs g)
 {
     constexpr int S = 9 - KW;             // planes between windows
@@ -770,6 +829,8 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
     __shared__ int s_next, s_tile;
     __shared__ int s_icnt[kDenseItems];
     __shared__ unsigned short s_ord[kDenseItems];
+    __shared__ unsigned short s_piece[kDensePieces];
+    __shared__ int s_npiece;
     const int ncell = g.tx * g.ty;
     const int nitem = ncell / (QS * QS);
     const int nlist = list[0];
@@ -800,26 +861,33 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
     const int e1pos = r & 7, e1shift = isx ? 0 : 8;
     const int src_kx = isx ? lane : lane - 8;  // x value of A row r
 
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+        if (threadIdx.x == 0) s_tile = atomicAdd(&list[1], 1);
         __syncthreads();
         const int li = s_tile;
         if (li >= nlist) break;
-        const int tile = __builtin_amdgcn_readfirstlane(list[1 + li]);
+        // the unit: points [beg, end) of bin `tile`
+        const int tile = __builtin_amdgcn_readfirstlane(list[2 + 2 * li]);
+        const int tb = tile_start[tile], te = tile_start[tile + 1];
+        const int beg = __builtin_amdgcn_readfirstlane(tb + list[3 + 2 * li]);
+        const int end = __builtin_amdgcn_readfirstlane((int)min((int64_t)te, (int64_t)beg + chunk));
         const int bx = tile % g.ntx, by = tile / g.ntx;
         for (int q = threadIdx.x; q < 2 * rcells; q += blockDim.x) are[q] = 0.0;
-        cell_runs(keys, tile_start[tile], tile_start[tile + 1], (uint32_t)g.kr.lower(tile, ntiles),
-                  g.cshift, ncell, run);
+        cell_runs(keys, beg, end, (uint32_t)g.kr.lower(tile, ntiles), g.cshift, ncell, run);
         if (threadIdx.x == 0) s_next = 0;
         __syncthreads();
         order_items<QS>(run, nitem, g.tx, s_icnt, s_ord);
+        order_pieces(s_icnt, s_ord, nitem, psz, s_piece, &s_npiece);
+        const int npiece = s_npiece;
         for (;;) {
             int it = 0;
             if (lane == 0) it = atomicAdd(&s_next, 1);
             it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-            if (it >= nitem) break;
+            if (it >= npiece) break;
+            const int pc = __builtin_amdgcn_readfirstlane((int)s_piece[it]);
             int lx0, ly0;
-            Runs R = item_runs<QS>(run, __builtin_amdgcn_readfirstlane((int)s_ord[it]), g.tx, lx0, ly0);
+            Runs R = item_runs<QS>(run, __builtin_amdgcn_readfirstlane((int)s_ord[pc & 255]), g.tx, lx0, ly0);
             R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
             R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
             R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
@@ -830,8 +898,10 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
             const float2 *vi = val + R.a0;
             const unsigned d1 = (unsigned)(R.a1 - R.a0 - R.n0);
             const int l0 = isx ? lx0 : ly0;
-            for (int c0 = 0; c0 < R.n; c0 += kDense2Max) {
-                const int n = min(kDense2Max, R.n - c0);
+            // the piece: item points [pb, pe)
+            const int pb = (pc >> 8) * psz, pe = min(R.n, pb + psz);
+            for (int c0 = pb; c0 < pe; c0 += kDense2Max) {
+                const int n = min(kDense2Max, pe - c0);
                 sort_by_window<S>(geo, R, c0, n, lst, cnt, wb, lane);
                 // ---- windows present
                 const bool has = lane < NW && wb[lane + 1] > wb[lane];
@@ -930,7 +1000,8 @@ __global__ __launch_bounds__(kDense2Threads) void k_grid_dense2(const float4 *__
 }
 
 // static LDS of k_grid_dense2 (sorted lists and window counters of 8 waves)
-constexpr size_t kDense2Static = (size_t)(kDense2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4) + 64 + 6 * kDenseItems;
+constexpr size_t kDense2Static = (size_t)(kDense2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4) + 64 + 6 * kDenseItems +
+                                 2 * kDensePieces + 4;
 
 static size_t dense_lds_bytes(const GridArgs &g, int kw)
 {
@@ -952,19 +1023,23 @@ static int grid_dense2_kw(const float4 *geo, const float2 *val, const uint32_t *
                           hipStream_t s)
 {
     const int64_t ntiles = (int64_t)g.ntx * g.nty;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
+    const int chunk = dense_chunk(g.m);
+    const int64_t units = ntiles + ceil_div((int64_t)g.m, (int64_t)chunk);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, 1);
     hipLaunchKernelGGL(k_dense_bins, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, ts,
-                       ntiles, g.dense_min, list);
+                       ntiles, g.dense_min, chunk, list);
     CG_HIP(hipGetLastError());
     const size_t lds = dense_lds_bytes(g, KW);
     const int per_cu = std::max<int>(1, (int)(163840 / (lds + kDense2Static)));
-    const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)256 * per_cu);
+    const unsigned nblk = (unsigned)std::min<int64_t>(units, (int64_t)256 * per_cu);
+    // pieces: about one per wave of a full unit
+    const int psz = (int)ceil_div(chunk, kDense2Threads / 64);
     // quads of start cells where their footprints fit the 8 x 8 frame
     auto go = [&](const void *fn, auto kern) -> int {
         if (lds > 65536)
             CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(kern, dim3(nblk), dim3(kDense2Threads), lds, s, geo, val, keys, ts, list,
-                           ntiles, fw, g);
+                           chunk, psz, fw, g);
         CG_HIP(hipGetLastError());
         return CG_OK;
     };
@@ -1166,7 +1241,7 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
                                                                   const uint32_t *__restrict__ keys,
                                                                   const int32_t *__restrict__ tile_start,
                                                                   int32_t *__restrict__ list,
-                                                                  int64_t ntiles,
+                                                                  int chunk, int psz,
                                                                   const float2 *__restrict__ fw,
                                                                   float2 *__restrict__ out, GridArgs g)
 {
@@ -1184,6 +1259,8 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
     __shared__ int s_next, s_tile;
     __shared__ int s_icnt[kDenseItems];
     __shared__ unsigned short s_ord[kDenseItems];
+    __shared__ unsigned short s_piece[kDensePieces];
+    __shared__ int s_npiece;
     // per-wave exchange of a group's kernel values: ky[point][y], kz[plane][point]
     __shared__ __align__(16) float s_ky[NWAVES][16 * 8];
     __shared__ __align__(16) float s_kz[NWAVES][8 * 16];
@@ -1216,12 +1293,17 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
     int *cnt = s_cnt[wave], *wb = s_wb[wave];
     float *ky_t = s_ky[wave], *kz_t = s_kz[wave];
 
+    const int64_t ntiles = (int64_t)g.ntx * g.nty;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(&list[ntiles + 1], 1);
+        if (threadIdx.x == 0) s_tile = atomicAdd(&list[1], 1);
         __syncthreads();
         const int li = s_tile;
         if (li >= nlist) break;
-        const int tile = __builtin_amdgcn_readfirstlane(list[1 + li]);
+        // the unit: points [beg, end) of bin `tile`
+        const int tile = __builtin_amdgcn_readfirstlane(list[2 + 2 * li]);
+        const int tb = tile_start[tile], te = tile_start[tile + 1];
+        const int beg = __builtin_amdgcn_readfirstlane(tb + list[3 + 2 * li]);
+        const int end = __builtin_amdgcn_readfirstlane((int)min((int64_t)te, (int64_t)beg + chunk));
         const int bx = tile % g.ntx, by = tile / g.ntx;
         const int x0 = bx * g.tx, y0 = by * g.ty;
         // the tile and its halo of every plane
@@ -1236,25 +1318,29 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
             if (gy >= g.nf2) gy -= g.nf2;
             lds[(lz * hy + y) * hxs + x] = fw[((size_t)lz * g.nf2 + gy) * g.nf1 + gx];
         }
-        cell_runs(keys, tile_start[tile], tile_start[tile + 1], (uint32_t)g.kr.lower(tile, ntiles),
-                  g.cshift, ncell, s_run);
+        cell_runs(keys, beg, end, (uint32_t)g.kr.lower(tile, ntiles), g.cshift, ncell, s_run);
         if (threadIdx.x == 0) s_next = 0;
         __syncthreads();
         order_items<QS>(s_run, nitem, g.tx, s_icnt, s_ord);
+        order_pieces(s_icnt, s_ord, nitem, psz, s_piece, &s_npiece);
+        const int npiece = s_npiece;
         for (;;) {
             int it = 0;
             if (lane == 0) it = atomicAdd(&s_next, 1);
             it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-            if (it >= nitem) break;
+            if (it >= npiece) break;
+            const int pc = __builtin_amdgcn_readfirstlane((int)s_piece[it]);
             int lx0, ly0;
-            Runs R = item_runs<QS>(s_run, __builtin_amdgcn_readfirstlane((int)s_ord[it]), g.tx, lx0, ly0);
+            Runs R = item_runs<QS>(s_run, __builtin_amdgcn_readfirstlane((int)s_ord[pc & 255]), g.tx, lx0, ly0);
             R.a0 = __builtin_amdgcn_readfirstlane(R.a0);
             R.n0 = __builtin_amdgcn_readfirstlane(R.n0);
             R.a1 = __builtin_amdgcn_readfirstlane(R.a1);
             R.n = __builtin_amdgcn_readfirstlane(R.n);
-            if (R.n < kDegrid2Item) {
+            // the piece: item points [pb, pe)
+            const int pb = (pc >> 8) * psz, pe = min(R.n, pb + psz);
+            if (pe - pb < kDegrid2Item) {
                 // one lane per point: all KW^3 taps from LDS
-                for (int i = lane; i < R.n; i += 64) {
+                for (int i = pb + lane; i < pe; i += 64) {
                     const int row = R.row(i);
                     const float4 G = geo[row];
                     const uint32_t pk = __float_as_uint(G.w);
@@ -1303,8 +1389,8 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
             auto rec = [&](int i) -> unsigned {
                 return (unsigned)i < (unsigned)R.n0 ? (unsigned)i : (unsigned)i + d1;
             };
-            for (int c0 = 0; c0 < R.n; c0 += kDense2Max) {
-                const int n = min(kDense2Max, R.n - c0);
+            for (int c0 = pb; c0 < pe; c0 += kDense2Max) {
+                const int n = min(kDense2Max, pe - c0);
                 sort_by_window<S>(geo, R, c0, n, lst, cnt, wb, lane);
                 const bool has = lane < NW && wb[lane + 1] > wb[lane];
                 const uint64_t hm = __ballot(has);
@@ -1398,7 +1484,7 @@ __global__ __launch_bounds__(kDegrid2Threads) void k_degrid_dense(const float4 *
 // static LDS of k_degrid_dense
 constexpr size_t kDegrid2Static =
     (size_t)(kDegrid2Threads / 64) * (2 * kDense2Max + 2 * 33 * 4 + 2 * 128 * 4) + 257 * 4 + 64 +
-    6 * kDenseItems;
+    6 * kDenseItems + 2 * kDensePieces + 4;
 
 template <int KW>
 static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *perm,
@@ -1406,17 +1492,21 @@ static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *
                            float2 *out, const GridArgs &g, hipStream_t s)
 {
     const int64_t ntiles = (int64_t)g.ntx * g.nty;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, ntiles + 1);
+    const int chunk = dense_chunk(g.m);
+    const int64_t units = ntiles + ceil_div((int64_t)g.m, (int64_t)chunk);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, list, 1);
     hipLaunchKernelGGL(k_dense_bins, dim3((unsigned)ceil_div(ntiles, 256)), dim3(256), 0, s, ts,
-                       ntiles, g.dense_min, list);
+                       ntiles, g.dense_min, chunk, list);
     CG_HIP(hipGetLastError());
     const size_t lds = (size_t)degrid_lds_bytes(g, KW, 8);
-    const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, 256);
+    const unsigned nblk = (unsigned)std::min<int64_t>(units, 256);
+    // pieces: about one per wave of a full unit
+    const int psz = (int)ceil_div(chunk, kDegrid2Threads / 64);
     auto go = [&](const void *fn, auto kern) -> int {
         if (lds > 65536)
             CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(kern, dim3(nblk), dim3(kDegrid2Threads), lds, s, geo, fac, perm, keys, ts,
-                           list, ntiles, fw, out, g);
+                           list, chunk, psz, fw, out, g);
         CG_HIP(hipGetLastError());
         return CG_OK;
     };

@@ -188,7 +188,11 @@ int launch_degrid(const geo_t<T> *geo, const cx_t<T> *fac, const int32_t *perm,
 int grid_lds_bytes(const GridArgs &g, int elem);
 // Dense bins (fp32, kw <= 8, nf3 <= 32, cshift > 0): the sparse kernel
 // launched with g.dense_min > 0 leaves them out; this adds them to fw with
-// per-start-cell MFMA products.  list needs ntiles + 2 ints of scratch.
+// per-start-cell MFMA products.  list needs dense_list_ints(ntiles, g.m)
+// ints of scratch.  The dense kernels' work units are chunks of at most
+// dense_chunk(g.m) sorted points of one bin.
+int dense_chunk(int64_t m);
+int64_t dense_list_ints(int64_t ntiles, int64_t m);
 bool grid_dense_supported(const GridArgs &g, int kw, int elem);
 int launch_grid_dense(const geo_t<float> *geo, const cx_t<float> *val, const uint32_t *keys,
                       const int32_t *tile_start, int32_t *list, cx_t<float> *fw, const GridArgs &g,
@@ -197,7 +201,7 @@ int degrid_lds_bytes(const GridArgs &g, int kw, int elem);
 // Dense bins of the complex64 degridder (kw <= 8, nf3 <= 32, start cell in
 // the sort key, whole grid staged, no slabs): k_degrid launched with
 // g.dense_min > 0 leaves the bins with at least that many points to this
-// one, which interpolates them on MFMA.  list needs ntiles + 2 ints.
+// one, which interpolates them on MFMA.  list needs dense_list_ints ints.
 bool degrid_dense_supported(const GridArgs &g, int kw);
 int launch_degrid_dense(const float4 *geo, const float2 *fac, const int32_t *perm,
                         const uint32_t *keys, const int32_t *tile_start, int32_t *list,

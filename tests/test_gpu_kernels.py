@@ -134,20 +134,26 @@ def test_gridding_stage_vs_oracle(gpu, single, case):
     assert rel_l2(got, ref) < (3e-6 if single else 1e-13)
 
 
-@pytest.mark.parametrize("case", [(3000, 64, 96, 10.0, 1e-6, 1), (20000, 256, 256, 2.0, 1e-5, 1),
-                                  (131072, 1024, 1024, 2.0, 1e-5, 64),
-                                  (20000, 128, 128, 5.0, 1e-3, 1), (50000, 200, 200, 3.0, 1e-4, 1)])
+@pytest.mark.parametrize("case", [(3000, 64, 96, 10.0, 1e-6, 1, None),
+                                  (20000, 256, 256, 2.0, 1e-5, 1, None),
+                                  (131072, 1024, 1024, 2.0, 1e-5, 64, 1500.0),
+                                  (20000, 128, 128, 5.0, 1e-3, 1, None),
+                                  (50000, 200, 200, 3.0, 1e-4, 1, None),
+                                  # crowded: ~33K points in each of a few bins,
+                                  # so bins are cut into 4096-point units and
+                                  # quads into several pieces
+                                  (300000, 256, 256, 2.0, 1e-5, 1, 130.0)])
 def test_dense_gridding_vs_oracle(gpu, case):
     """Bins holding at least dense_min points are gridded by MFMA products
     over quads of start cells (complex64, kw 4 to 7): the planes must match
     the oracle's spreading and the all-sparse path."""
     torch = gpu
     from cugridder_amd import Context
-    nrow, nx, ny, fov, eps, dmin = case
-    if nrow == 131072:
+    nrow, nx, ny, fov, eps, dmin, sig = case
+    if sig is not None:
         rng = np.random.default_rng(1)
         uvw = np.empty((nrow, 3))
-        uvw[:, :2] = rng.normal(0, 1500, (nrow, 2))
+        uvw[:, :2] = rng.normal(0, sig, (nrow, 2))
         uvw[:, 2] = rng.uniform(-300, 300, nrow)
         ms = (rng.standard_normal(nrow) + 1j * rng.standard_normal(nrow)).astype(np.complex64)
         wgt = rng.uniform(0.5, 1.5, nrow).astype(np.float32)
@@ -171,8 +177,12 @@ def test_dense_gridding_vs_oracle(gpu, case):
     assert rel_l2(grids[dmin], grids[0]) < 3e-6
 
 
-@pytest.mark.parametrize("case", [(131072, 1024, 1024, 2.0, 1e-5), (60000, 256, 256, 2.0, 1e-6),
-                                  (60000, 128, 160, 5.0, 1e-3), (60000, 200, 200, 3.0, 1e-4)])
+@pytest.mark.parametrize("case", [(131072, 1024, 1024, 2.0, 1e-5, 1500.0),
+                                  (60000, 256, 256, 2.0, 1e-6, 300.0),
+                                  (60000, 128, 160, 5.0, 1e-3, 300.0),
+                                  (60000, 200, 200, 3.0, 1e-4, 300.0),
+                                  # crowded: bins cut into units, quads into pieces
+                                  (300000, 256, 256, 2.0, 1e-5, 130.0)])
 def test_dense_degridding_vs_oracle(gpu, case):
     """dirty2ms with the tiles of many points interpolated by MFMA products
     over quads of start cells (complex64, kw 4 to 7, quads below the point
@@ -180,10 +190,10 @@ def test_dense_degridding_vs_oracle(gpu, case):
     oracle."""
     torch = gpu
     from cugridder_amd import Context
-    nrow, nx, ny, fov, eps = case
+    nrow, nx, ny, fov, eps, sig = case
     rng = np.random.default_rng(nrow + nx)
     uvw = np.empty((nrow, 3))
-    uvw[:, :2] = rng.normal(0, 1500 if nx == 1024 else 300, (nrow, 2))
+    uvw[:, :2] = rng.normal(0, sig, (nrow, 2))
     uvw[:, 2] = rng.uniform(-300, 300, nrow)
     f0 = 1e8
     wgt = rng.uniform(0.5, 1.5, nrow).astype(np.float32)
