@@ -926,12 +926,12 @@ __device__ __forceinline__ void band_pass(float2 *buf, const float2 *mt)
     lds_barrier();
 }
 
-// the last Stockham pass (radix R, stage PP = M / R), handing each output
-// to out(slot, element index, value) instead of writing it back to LDS.
-// Output r of the thread's k-th butterfly is element i + PP r (i its
-// butterfly index) and slot k R + r; the column of a thread's butterflies is
-// its own (1024 is a multiple of CB).  The caller may rewrite buf after this
-// returns (every read precedes its barrier).
+// the last Stockham pass (radix R, stage PP = M / R), handing the outputs
+// of each butterfly to out(k, j, u) instead of writing them back to LDS:
+// u[r] (r < R) is element j + PP r of the transform, slot k R + r of the
+// thread (k its butterfly, j = its butterfly index i); the column of a
+// thread's butterflies is its own (1024 is a multiple of CB).  The caller
+// may rewrite buf after this returns (every read precedes its barrier).
 template <int R, int PP, int DIR, int M, int CB, typename F>
 __device__ __forceinline__ void band_pass_out(const float2 *buf, const float2 *mt, F &&out)
 {
@@ -958,9 +958,7 @@ __device__ __forceinline__ void band_pass_out(const float2 *buf, const float2 *m
             u[k][r] = cmul<float>(u[k][r], w);
         }
         dft_small<float, R, DIR>(u[k]);
-        const int j = (i - kk) * R + kk;
-#pragma unroll
-        for (int r = 0; r < R; r++) out(k * R + r, j + r * PP, u[k][r]);
+        out(k, (i - kk) * R + kk, u[k]);
     }
 }
 
@@ -995,13 +993,21 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
     const int lc = cok ? live.at(c) : 0;
     float2 *pl = data + (size_t)blockIdx.y * n2 * n1;
     const int mask = n2 - 1;
-    auto dead = [&](int k) { return k >= spec_a && k < spec_b; };
     // split-order row of frequency k (na = 2^nash)
     auto srow = [&](int k) { return (k & ((1 << nash) - 1)) * nb + (k >> nash); };
     auto twid = [&](int n, int ka) {
         float2 w = tab.get((n * ka) & mask);
         if (DIR < 0) w.y = -w.y;
         return w;
+    };
+    // spectrum addresses: a uniform base per output r (per input m) plus one
+    // 32-bit byte offset per lane (a plane is < 4 GiB, checked at launch), so
+    // that the loads and stores take the scalar-base form; dead frequencies
+    // k are those with (unsigned)(k - spec_a) < dw
+    char *const plb = reinterpret_cast<char *>(pl);
+    const unsigned dw = spec_b > spec_a ? (unsigned)(spec_b - spec_a) : 0u;
+    auto lane_off = [&](int k) {
+        return ((uint32_t)srow(k) * (uint32_t)n1 + (uint32_t)lc) * (uint32_t)sizeof(float2);
     };
     // residue e + S j of this thread: band offset o = (e + S j - bs) mod M,
     // row (bs + o) mod n2 when o < blen
@@ -1026,15 +1032,17 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
 #pragma unroll
             for (int j = 0; j < 16; j++) v[j] = cmul<T>(xb[j], twid(bs + ((eo + S * j - bs) & (M - 1)), ka));
         } else {
+            // input m is frequency k0 + m F (F = np S, a multiple of the
+            // split factor na), at split-order row srow(k0) + m (F / na)
+            const int k0 = ka + np * eo, F = np * S;
+            const uint32_t boff = lane_off(k0);
+            const size_t mstride = (size_t)(F >> nash) * n1 * sizeof(float2);
+            const int d0 = k0 - spec_a;
 #pragma unroll
             for (int m = 0; m < 16; m++) {
-                // branch-free: a dead frequency loads row 0 (frequency 0, live
-                // and cached) and is selected away, so only live rows are read
-                // from HBM; lc < n1 for idle columns too
-                const int k = ka + np * (eo + S * m);
-                const bool dk = dead(k);
-                const float2 t = pl[(size_t)(dk ? 0 : srow(k)) * n1 + lc];
-                v[m] = (cok && !dk) ? t : make_float2(0.f, 0.f);
+                v[m] = make_float2(0.f, 0.f);
+                if (cok && (unsigned)(d0 + m * F) >= dw)
+                    v[m] = *reinterpret_cast<const float2 *>(plb + m * mstride + boff);
             }
         }
         dft_small<T, 16, DIR>(v);
@@ -1049,11 +1057,16 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
             // split factor na (powers of two, na <= 256 <= n2 / 8), so its
             // split-order row is srow(k0) + r np 256 / na
             constexpr int R = M == 1024 ? 4 : 8;
-            const int rstep = (np * 256) >> nash;
-            auto store = [&](int slot, int kb, float2 val) {
-                const int r = slot % R, k0 = ka + np * (kb - 256 * r);
-                const int k = k0 + np * 256 * r;
-                if (cok && !dead(k)) pl[(size_t)(srow(k0) + r * rstep) * n1 + lc] = val;
+            const int T = np * 256;
+            const size_t rstride = (size_t)(T >> nash) * n1 * sizeof(float2);
+            auto store = [&](int, int j, const float2 *u) {
+                const int k0 = ka + np * j;
+                const uint32_t boff = lane_off(k0);
+                const int d0 = k0 - spec_a;
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    if (cok && (unsigned)(d0 + r * T) >= dw)
+                        *reinterpret_cast<float2 *>(plb + r * rstride + boff) = u[r];
             };
             if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, mt, store);
             else band_pass_out<8, 256, DIR, M, CB>(buf, mt, store);
@@ -1062,10 +1075,15 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
             // the last pass's outputs (residue i + 256 r of butterfly i, slot
             // k R + r) are accumulated where they are computed: this thread
             // owns the band rows of those residues
-            auto acc = [&](int slot, int rr, float2 val) {
-                const float2 z = cmul<T>(val, twid(bs + ((rr - bs) & (M - 1)), ka));
-                xb[slot].x += z.x;
-                xb[slot].y += z.y;
+            constexpr int R = M == 1024 ? 4 : 8;
+            auto acc = [&](int kb, int j, const float2 *u) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int rr = j + 256 * r;
+                    const float2 z = cmul<T>(u[r], twid(bs + ((rr - bs) & (M - 1)), ka));
+                    xb[kb * R + r].x += z.x;
+                    xb[kb * R + r].y += z.y;
+                }
             };
             if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, mt, acc);
             else band_pass_out<8, 256, DIR, M, CB>(buf, mt, acc);
@@ -1475,9 +1493,10 @@ static int fft_cols(cx_t<T> *data, int batch, int n2, int n1, const cx_t<T> *tw2
         // M = 2048 (8 columns); the transform buffer is 128 KB either way
         const int M = band_len <= 1024 ? 1024 : 2048, CB = M == 1024 ? 16 : 8;
         const size_t lds = sizeof(float2) * ((size_t)M * CB + M + (1 << kBandTwSh) + (n2 >> kBandTwSh));
+        // (byte offsets inside a plane are 32-bit in the kernel)
         if (band_len > 0 && band_len <= M && lds <= (size_t)kLdsMax && band_s >= 0 && band_s < n2 &&
             fft_spectrum_pruned<T>(n2) && (n2 & (n2 - 1)) == 0 &&
-            n2 >= 2 * M && fft_col_split(n2, na, nb)) {
+            (size_t)n2 * n1 * sizeof(float2) <= 0xffffffffu && n2 >= 2 * M && fft_col_split(n2, na, nb)) {
             int nblk = (nlive + CB - 1) / CB;
             if (M == 2048) nblk = (nblk + 15) / 16 * 16;  // whole groups of the block remap
             const dim3 grid((unsigned)nblk, (unsigned)batch);
