@@ -1066,7 +1066,7 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if (cok && (unsigned)(d0 + r * T) >= dw)
-                        *reinterpret_cast<float2 *>(plb + r * rstride + boff) = u[r];
+                        store_nt(reinterpret_cast<float2 *>(plb + r * rstride + boff), u[r]);
             };
             if constexpr (M == 1024) band_pass_out<4, 256, DIR, M, CB>(buf, mt, store);
             else band_pass_out<8, 256, DIR, M, CB>(buf, mt, store);

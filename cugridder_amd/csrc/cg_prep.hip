@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, c
     double u = (double)uvw[3 * r], v = (double)uvw[3 * r + 1], w = (double)uvw[3 * r + 2];
     Foot f = make_foot(u, v, w, a, ps.suv, ps.sw);
     uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8) | ((uint32_t)f.z0 << 16);
-    geo[i] = make_geo<T>(f.d1, f.d2, f.d3, pk);
+    store_nt(geo + i, make_geo<T>(f.d1, f.d2, f.d3, pk));
     double wt = wgt ? (double)wgt[j] : 1.0;
     double ws = w * ps.sw;
     double2 c;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_records(const TU *uvw, const TV *vis, c
         sincos(-(wp * a.gamma + a.i_center) * a.o_center, &s, &co);
         c = make_double2(wt * co, wt * s);
     }
-    val[i] = mkc((T)c.x, (T)c.y);
+    store_nt(val + i, mkc((T)c.x, (T)c.y));
 }
 
 // The same records for complex64 in two passes: every point's record is
@@ -500,9 +500,9 @@ __global__ __launch_bounds__(256) void k_gather_records(const float4 *__restrict
     if (i >= nrow) return;
     int64_t j = perm[i];
     if (j < 0 || j >= nrow) j = 0;  // a valid permutation never takes this
-    geo[i] = rec[2 * j];
+    store_nt(geo + i, rec[2 * j]);
     const float4 t = rec[2 * j + 1];
-    val[i] = make_float2(t.x, t.y);
+    store_nt(val + i, make_float2(t.x, t.y));
 }
 
 int launch_gather_records(const float4 *rec, const int32_t *perm, int64_t nrow, float4 *geo,

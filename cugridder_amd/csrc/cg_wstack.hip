@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_dirty2ms_pro(const TV *__restrict__ dir
     double2 ph = make_double2(cv, sv);
     for (int p = 0; p < zn; p++) {
         const double2 v = zmul(f, ph);
-        dst[plane * p] = mkc((T)v.x, (T)v.y);
+        store_nt(dst + plane * p, mkc((T)v.x, (T)v.y));
         ph = zmul(ph, step);
     }
 }
