@@ -548,12 +548,12 @@ __global__ __launch_bounds__(256) void k_records_xy(const TU *x, const TU *y, co
     if (j < 0 || j >= m) j = 0;  // a valid permutation never takes this
     Foot f = make_foot_xy((double)x[j], (double)y[j], a);
     uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8);
-    geo[i] = make_geo<T>(f.d1, f.d2, 0.0, pk);
+    store_nt(geo + i, make_geo<T>(f.d1, f.d2, 0.0, pk));
     if (c) {
         double2 v = ld_c(c, j);
-        val[i] = mkc((T)v.x, (T)v.y);
+        store_nt(val + i, mkc((T)v.x, (T)v.y));
     } else {
-        val[i] = mkc((T)1, (T)0);
+        store_nt(val + i, mkc((T)1, (T)0));
     }
 }
 
@@ -596,12 +596,12 @@ __global__ __launch_bounds__(256) void k_gather_records_xy(const float4 *__restr
     int64_t j = perm[i];
     if (j < 0 || j >= m) j = 0;  // a valid permutation never takes this
     if (has_c) {
-        geo[i] = rec[2 * j];
+        store_nt(geo + i, rec[2 * j]);
         const float4 t = rec[2 * j + 1];
-        val[i] = make_float2(t.x, t.y);
+        store_nt(val + i, make_float2(t.x, t.y));
     } else {
-        geo[i] = rec[j];
-        val[i] = make_float2(1.f, 0.f);
+        store_nt(geo + i, rec[j]);
+        store_nt(val + i, make_float2(1.f, 0.f));
     }
 }
 
