@@ -623,9 +623,12 @@ __global__ void k_zero_i32(int32_t *p, int64_t second)
     }
 }
 
-// Dense bins, second form.  Work items are quads, 2 x 2 start cells (single
-// cells when kw = 8): with kw <= 7 the footprints of a quad's points lie in
-// one 8 x 8 tap frame.  A quad's points (two runs of the sorted order) are
+// Dense bins, second form.  A workgroup takes work units (k_dense_bins: a
+// bin, or a chunk of a crowded one) from a device counter; its waves take
+// pieces (order_pieces) of the unit's work items.  Work items are quads,
+// 2 x 2 start cells (single cells when kw = 8): with kw <= 7 the footprints
+// of a quad's points lie in one 8 x 8 tap frame.  A quad's points (two runs
+// of the sorted order) are
 // first ordered by w-window (window w holds the points whose first plane z0
 // lies in [S w, S w + S), S = 9 - kw, so every footprint of the window fits
 // the eight planes [S w, S w + 8)) with a stable counting sort of the wave in
@@ -1217,9 +1220,10 @@ __device__ __forceinline__ void degrid_tile(const geo_t<T> *__restrict__ geo,
 }
 
 // Dense bins of the degridder (complex64), the adjoint of k_grid_dense2.  A
-// workgroup stages its tile plus the (kw-1) halo of every plane in LDS (as
-// k_degrid does) and its waves take quads (2 x 2 start cells; cells at kw =
-// 8).  A quad with at least kDegrid2Item points is ordered by w-window
+// workgroup takes work units as k_grid_dense2 does, stages the unit's tile
+// plus the (kw-1) halo of every plane in LDS (as k_degrid does) and its
+// waves take pieces of quads (2 x 2 start cells; cells at kw = 8).  A piece
+// with at least kDegrid2Item points is ordered by w-window
 // (sort_by_window) and each window's points are interpolated sixteen at a
 // time as one 16 x 16 product on v_mfma_f32_16x16x4f32:
 //   R[j][col] = sum_tap W[j][tap] F[tap][col],  W = kx_j(x - dx_j) ky_j(y - dy_j)
@@ -1230,7 +1234,7 @@ __device__ __forceinline__ void degrid_tile(const geo_t<T> *__restrict__ geo,
 // a DPP reduction across the lanes of each 16-lane row.  Each lane evaluates
 // six kernel values of one point per group (two each along x, y and z); the
 // y and z values reach the lanes that need them through a small per-wave
-// LDS table.  Quads with fewer points give each point a lane.
+// LDS table.  Pieces with fewer points give each point a lane.
 constexpr int kDegrid2Threads = 1024;
 constexpr int kDegrid2Item = 48;
 
