@@ -603,8 +603,8 @@ __global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ 
 
 int dense_chunk(int64_t m)
 {
-    // about 1024 units over the rows (four per CU), at least 4096 points each
-    int64_t c = 4096;
+    // about 1024 units over the rows (four per CU), at least 512 points each
+    int64_t c = 512;
     while (c < m / 1024 && c < (1 << 30)) c *= 2;
     return (int)c;
 }
@@ -1504,8 +1504,9 @@ static int degrid_dense_kw(const float4 *geo, const float2 *fac, const int32_t *
     CG_HIP(hipGetLastError());
     const size_t lds = (size_t)degrid_lds_bytes(g, KW, 8);
     const unsigned nblk = (unsigned)std::min<int64_t>(units, 256);
-    // pieces: about one per wave of a full unit
-    const int psz = (int)ceil_div(chunk, kDegrid2Threads / 64);
+    // pieces: about one per wave of a full unit, at least 64 points (the
+    // MFMA path's share; the piece table needs psz >= chunk / 16)
+    const int psz = std::max<int>(64, (int)ceil_div(chunk, kDegrid2Threads / 64));
     auto go = [&](const void *fn, auto kern) -> int {
         if (lds > 65536)
             CG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
