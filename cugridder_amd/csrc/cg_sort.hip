@@ -128,12 +128,24 @@ int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *tmp, hipStream_t s)
 
 int64_t radix_hist_size(int64_t m) { return (int64_t)kRadix * std::max<int64_t>(1, ceil_div(m, kSortTile)); }
 
+// The tile a workgroup of the radix passes takes: workgroups are dealt to
+// the 8 XCDs round-robin, so workgroup b runs on XCD b % 8; XCD x takes the
+// contiguous tiles [x q + min(x, r), ...) (nblocks = 8 q + r), so that the
+// histogram entries of adjacent tiles, and the adjacent runs of a digit that
+// adjacent tiles write, meet in one L2.  A bijection of [0, nblocks).
+__device__ __forceinline__ int radix_tile(int nblocks)
+{
+    const int b = blockIdx.x, x = b & 7, q = nblocks >> 3, r = nblocks & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *keys, int64_t m,
                                                              int shift, int32_t *hist, int nblocks)
 {
     __shared__ int cnt[kRadix];
     cnt[threadIdx.x] = 0;
-    int64_t base = blockIdx.x * (int64_t)kSortTile;
+    const int tile = radix_tile(nblocks);
+    int64_t base = tile * (int64_t)kSortTile;
     // all loads first: one memory latency per block, not one per round
     uint32_t kk[kSortRounds];
 #pragma unroll
@@ -148,7 +160,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *key
         if (i < m) atomicAdd(&cnt[(kk[k] >> shift) & (kRadix - 1)], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+    hist[(int64_t)threadIdx.x * nblocks + tile] = cnt[threadIdx.x];
 }
 
 // One pass: the block ranks its 4096 items by digit in LDS and writes them
@@ -184,11 +196,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     __shared__ uint32_t s_key[kSortTile];
     __shared__ int32_t s_val[kSortTile];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    s_gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+    const int tile = radix_tile(nblocks);
+    s_gbase[tid] = offs[(int64_t)tid * nblocks + tile];
 #pragma unroll
     for (int w = 0; w < NW; w++) s_wcnt[w][tid] = 0;
     const uint64_t lt = __lanemask_lt();
-    const int64_t tbase = blockIdx.x * (int64_t)kSortTile;
+    const int64_t tbase = tile * (int64_t)kSortTile;
     const int64_t qbase = tbase + (int64_t)wave * QW;
     // all loads first: one memory latency per block, not one per round
     uint32_t kk[R];
