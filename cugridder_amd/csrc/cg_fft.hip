@@ -1018,6 +1018,24 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
         xb[j] = make_float2(0.f, 0.f);
         if (DIR > 0 && o < blen && cok && (!rowmask || rowmask[n])) xb[j] = pl[(size_t)n * n1 + lc];
     }
+    // inverse: input m of sub-transform ka is frequency k0 + m F (k0 = ka +
+    // np e, F = np S, a multiple of the split factor na), at split-order row
+    // srow(k0) + m (F / na); the inputs of ka + 1 are loaded while ka is
+    // transformed (the loop's barriers would otherwise wait on them)
+    float2 vn[16];
+    auto load_inputs = [&](int ka, int eo) {
+        const int k0 = ka + np * eo, F = np * S;
+        const uint32_t boff = lane_off(k0);
+        const size_t mstride = (size_t)(F >> nash) * n1 * sizeof(float2);
+        const int d0 = k0 - spec_a;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            vn[m] = make_float2(0.f, 0.f);
+            if (cok && (unsigned)(d0 + m * F) >= dw)
+                vn[m] = *reinterpret_cast<const float2 *>(plb + m * mstride + boff);
+        }
+    };
+    if constexpr (DIR < 0) load_inputs(0, e);
     lds_barrier();  // twiddle table
     for (int ka = 0; ka < np; ka++) {
         // recomputing the band rows every iteration is cheaper than keeping
@@ -1032,18 +1050,9 @@ __global__ __launch_bounds__(1024) void k_fft_cols_band(float2 *data, int n2, in
 #pragma unroll
             for (int j = 0; j < 16; j++) v[j] = cmul<T>(xb[j], twid(bs + ((eo + S * j - bs) & (M - 1)), ka));
         } else {
-            // input m is frequency k0 + m F (F = np S, a multiple of the
-            // split factor na), at split-order row srow(k0) + m (F / na)
-            const int k0 = ka + np * eo, F = np * S;
-            const uint32_t boff = lane_off(k0);
-            const size_t mstride = (size_t)(F >> nash) * n1 * sizeof(float2);
-            const int d0 = k0 - spec_a;
 #pragma unroll
-            for (int m = 0; m < 16; m++) {
-                v[m] = make_float2(0.f, 0.f);
-                if (cok && (unsigned)(d0 + m * F) >= dw)
-                    v[m] = *reinterpret_cast<const float2 *>(plb + m * mstride + boff);
-            }
+            for (int m = 0; m < 16; m++) v[m] = vn[m];
+            if (ka + 1 < np) load_inputs(ka + 1, eo);
         }
         dft_small<T, 16, DIR>(v);
 #pragma unroll
