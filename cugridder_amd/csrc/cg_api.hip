@@ -317,11 +317,14 @@ static int sort_rows(Ctx &C, const TU *uvw, int64_t M, const CoordArgs &a, const
     const bool odd = radix_passes(bits) & 1;
     uint32_t *k0 = as<uint32_t>(odd ? C.keys2 : C.keys), *k1 = as<uint32_t>(odd ? C.keys : C.keys2);
     int32_t *v0 = as<int32_t>(odd ? C.vals2 : C.vals), *v1 = as<int32_t>(odd ? C.vals : C.vals2);
+    // the permutation starts as the identity, generated by the first pass
     if (rec)
-        CG_TRY((launch_keys_prerecords<TU, float2, float>(uvw, vis, wgt, M, a, 0, k0, v0, rec, C.stream)));
+        CG_TRY((launch_keys_prerecords<TU, float2, float>(uvw, vis, wgt, M, a, 0, k0, nullptr, rec,
+                                                           C.stream)));
     else
-        CG_TRY(launch_keys<TU>(uvw, M, a, k0, v0, C.stream));
-    CG_TRY(launch_radix_sort(k0, v0, k1, v1, M, bits, as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream, 0));
+        CG_TRY(launch_keys<TU>(uvw, M, a, k0, nullptr, C.stream));
+    CG_TRY(launch_radix_sort(k0, v0, k1, v1, M, bits, as<int32_t>(C.hist), as<int32_t>(C.scan), C.stream, 0,
+                             1));
     CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a), as<int32_t>(C.tstart),
                               C.stream));
     return CG_OK;
@@ -1187,11 +1190,11 @@ int nufft2d_dev(Ctx &C, int type, int64_t M, const TU *x, const TU *y, cx_t<T> *
         if constexpr (sizeof(T) == 4 && std::is_same<TU, float>::value) {
             if (rec)
                 CG_TRY(launch_keys_prerecords_xy(x, y, type == 1 ? (const float2 *)c : nullptr, M, a,
-                                                 k0, v0, rec, C.stream));
+                                                 k0, nullptr, rec, C.stream));
         }
-        if (!rec) CG_TRY(launch_keys_xy<TU>(x, y, M, a, k0, v0, C.stream));
+        if (!rec) CG_TRY(launch_keys_xy<TU>(x, y, M, a, k0, nullptr, C.stream));
         CG_TRY(launch_radix_sort(k0, v0, k1, v1, M, bits, as<int32_t>(C.hist), as<int32_t>(C.scan),
-                                 C.stream, 0));
+                                 C.stream, 0, 1));
         CG_TRY(launch_tile_bounds(as<uint32_t>(C.keys), M, ntiles, a.cshift, key_rect(a),
                                   as<int32_t>(C.tstart), C.stream));
     } else {

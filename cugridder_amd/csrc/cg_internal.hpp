@@ -60,7 +60,7 @@ template <typename TU>
 int launch_wminmax(const TU *uvw, int64_t nrow, double scale_w, double *d_part,
                    double &wmin, double &wmax, hipStream_t s, double *uv = nullptr);
 // keys[j] = (tile of the footprint start of row j) << cshift | start cell
-// inside the tile; vals[j] = j
+// inside the tile; vals[j] = j (vals may be null)
 template <typename TU>
 int launch_keys(const TU *uvw, int64_t nrow, const CoordArgs &a, uint32_t *keys,
                 int32_t *vals, hipStream_t s);
@@ -119,12 +119,14 @@ int launch_gather_records_xy(const float4 *rec, const int32_t *perm, int64_t m, 
 // radix_hist_size(m) ints.  Result ends in keys/vals (copy_back), or, with
 // copy_back = 0, in keys/vals after an even and tmp after an odd number of
 // passes (radix_passes): a caller that starts an odd sort in its tmp
-// buffers gets the result in its keys without a copy.
+// buffers gets the result in its keys without a copy.  identity_vals: vals
+// are not read, the first pass takes value i for item i (the key kernels
+// then need not write them).
 int64_t radix_hist_size(int64_t m);
 int radix_passes(int key_bits);
 int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys,
                       int32_t *tmp_vals, int64_t m, int key_bits, int32_t *hist,
-                      int32_t *scan_tmp, hipStream_t s, int copy_back = 1);
+                      int32_t *scan_tmp, hipStream_t s, int copy_back = 1, int identity_vals = 0);
 int64_t scan_tmp_size(int64_t n);
 // exclusive scan of n ints in place
 int launch_exclusive_scan(int32_t *data, int64_t n, int32_t *scan_tmp, hipStream_t s);

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_keys(const TU *uvw, int64_t nrow, Coord
     const PointScale ps = point_scale(j, a);
     Foot f = make_foot((double)uvw[3 * ps.row], (double)uvw[3 * ps.row + 1], 0.0, b, ps.suv, ps.sw);
     keys[j] = tile_key(f, a);
-    vals[j] = (int32_t)j;
+    if (vals) vals[j] = (int32_t)j;
 }
 
 template <typename TU>
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(256) void k_keys_xy(const TU *x, const TU *y, int64
     if (j >= m) return;
     Foot f = make_foot_xy((double)x[j], (double)y[j], a);
     keys[j] = tile_key(f, a);
-    vals[j] = (int32_t)j;
+    if (vals) vals[j] = (int32_t)j;
 }
 
 template <typename TU>
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void k_keys_prerecords(const TU *uvw, const TV
     const PointScale ps = point_scale(j, a);
     const Foot f = make_foot((double)uvw[3 * ps.row], (double)uvw[3 * ps.row + 1], 0.0, b, ps.suv, ps.sw);
     keys[j] = tile_key(f, a);
-    vals[j] = (int32_t)j;
+    if (vals) vals[j] = (int32_t)j;
     prerecord(uvw, vis, wgt, j, a, adjoint, rec);
 }
 
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_keys_prerecords_xy(const float *__restr
     if (j >= m) return;
     const Foot f = make_foot_xy((double)x[j], (double)y[j], a);
     keys[j] = tile_key(f, a);
-    vals[j] = (int32_t)j;
+    if (vals) vals[j] = (int32_t)j;
     const uint32_t pk = (uint32_t)(f.x0 % a.tx) | ((uint32_t)(f.y0 % a.ty) << 8);
     const float4 G = make_geo<float>(f.d1, f.d2, 0.0, pk);
     if (c) {

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     for (int r = 0; r < R; r++) {
         const int64_t i = qbase + 64 * r + lane;
         kk[r] = i < m ? kin[i] : 0u;
-        vv[r] = i < m ? vin[i] : 0;
+        vv[r] = i < m ? (vin ? vin[i] : (int32_t)i) : 0;  // no vin: the identity
     }
     __syncthreads();  // counters zeroed
     int *cnt = s_wcnt[wave];
@@ -273,11 +273,23 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
 
 int radix_passes(int key_bits) { return (key_bits + kRadixBits - 1) / kRadixBits; }
 
+__global__ __launch_bounds__(256) void k_iota(int32_t *v, int64_t m)
+{
+    const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (i < m) v[i] = (int32_t)i;
+}
+
 int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys, int32_t *tmp_vals,
                       int64_t m, int key_bits, int32_t *hist, int32_t *scan_tmp, hipStream_t s,
-                      int copy_back)
+                      int copy_back, int identity_vals)
 {
     const int passes = radix_passes(key_bits);
+    if (identity_vals && (m <= 1 || passes == 0)) {
+        // no pass reads the values: write the identity where the result is
+        if (m > 0)
+            hipLaunchKernelGGL(k_iota, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, vals, m);
+        CG_HIP(hipGetLastError());
+    }
     if (m <= 1) {
         // the one element (if any) goes where the passes would have left it
         if (!copy_back && (passes & 1) && m == 1) {
@@ -296,8 +308,9 @@ int launch_radix_sort(uint32_t *keys, int32_t *vals, uint32_t *tmp_keys, int32_t
         CG_HIP(hipGetLastError());
         int r = launch_exclusive_scan(hist, (int64_t)kRadix * nblocks, scan_tmp, s);
         if (r) return r;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, ka, va, kb, vb,
-                           m, shift, hist, nblocks);
+        // with identity_vals the first pass generates the values
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, ka,
+                           p == 0 && identity_vals ? nullptr : va, kb, vb, m, shift, hist, nblocks);
         CG_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
