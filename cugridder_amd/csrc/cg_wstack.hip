@@ -259,12 +259,24 @@ __global__ __launch_bounds__(256) void k_ms2dirty_epi(const cx_t<T> *__restrict_
     sincos(zp * (w.zb - w.nf3 / 2), &sv, &cv);
     double2 ph = make_double2(cv, sv);
     double2 acc = make_double2(0.0, 0.0);
-    for (int p = 0; p < zn; p++) {
-        const double2 f = to_d2(src[plane * p]);
+    auto add_plane = [&](cx_t<T> v) {
+        const double2 f = to_d2(v);
         acc.x += f.x * ph.x - f.y * ph.y;
         acc.y += f.x * ph.y + f.y * ph.x;
         ph = zmul(ph, step);
+    };
+    // the planes in batches of eight loads in flight, then their sums in
+    // plane order (a load waited for per plane leaves the stage bound by
+    // latency, not bandwidth)
+    int p = 0;
+    for (; p + 8 <= zn; p += 8) {
+        cx_t<T> fb[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) fb[k] = src[plane * (p + k)];
+#pragma unroll
+        for (int k = 0; k < 8; k++) add_plane(fb[k]);
     }
+    for (; p < zn; p++) add_plane(src[plane * p]);
     const double l = dr * w.pix_x, m = dc * w.pix_y;
     double nm1, n;
     pixel_n(l, m, nm1, n);
