@@ -593,7 +593,7 @@ __global__ __launch_bounds__(256) void k_dense_bins(const int32_t *__restrict__ 
     if (t >= ntiles) return;
     const int n = tile_start[t + 1] - tile_start[t];
     if (n < dense_min || n == 0) return;
-    const int nu = (n + chunk - 1) / chunk;
+    const int nu = (int)(((int64_t)n + chunk - 1) / chunk);
     const int e0 = atomicAdd(list, nu);
     for (int u = 0; u < nu; u++) {
         list[2 + 2 * (e0 + u)] = (int32_t)t;
