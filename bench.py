@@ -154,9 +154,9 @@ def make_nufft2d(torch, dev, M=1 << 25, N=4096, seed=2):
 
 def setup_learned(torch, ctx, W, args, kw=6):
     """Trains the support-kw kernel on the GPU (2^20 seeded visibilities of a
-    256^2 patch), certifies it at the workload's eps against the ES judge
+    256^2 patch), certifies it at the workload's eps against the ES certifier
     (support 7) and installs it in ctx when accepted; also times one step of
-    the ES support-7 path (eps 1e-6, the judge's accuracy) for comparison.
+    the ES support-7 path (eps 1e-6, the certifier's accuracy) for comparison.
     Untimed set-up: returns what the JSON line reports about it."""
     from cugridder_amd import learned
     W["eps"] = 1e-5
@@ -188,7 +188,7 @@ def setup_learned(torch, ctx, W, args, kw=6):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3 / reps, es.last_plan()["kw"]
 
-    # the judge's accuracy (eps 1e-6, support 7) and the ES kernel at the
+    # the certifier's accuracy (eps 1e-6, support 7) and the ES kernel at the
     # workload's own eps (1e-5: support 6, the learned kernel's support)
     es_ms, es_kw = es_time(1e-6)
     es5_ms, es5_kw = es_time(W["eps"])
@@ -196,9 +196,9 @@ def setup_learned(torch, ctx, W, args, kw=6):
     log("ES support %d at eps 1e-6: %.1f ms per step; ES support %d at eps %g: %.1f ms per step"
         % (es_kw, es_ms, es5_kw, W["eps"], es5_ms))
     return {"kw": kw, "accepted": cert["accepted"], "certified_rel_l2": [cert["ms2dirty"],
-            cert["dirty2ms"]], "judge": "ES support 7, eps 1e-6", "train_s": round(t_train, 2),
-            "es_judge_kw": es_kw, "es_judge_ms_per_step": round(es_ms, 2),
-            "es_judge_Mvis_s": round(W["M"] / (es_ms * 1e-3) / 1e6, 3),
+            cert["dirty2ms"]], "certifier": "ES support 7, eps 1e-6", "train_s": round(t_train, 2),
+            "es_certifier_kw": es_kw, "es_certifier_ms_per_step": round(es_ms, 2),
+            "es_certifier_Mvis_s": round(W["M"] / (es_ms * 1e-3) / 1e6, 3),
             "es_same_eps_kw": es5_kw, "es_same_eps_ms_per_step": round(es5_ms, 2),
             "es_same_eps_Mvis_s": round(W["M"] / (es5_ms * 1e-3) / 1e6, 3)}
 

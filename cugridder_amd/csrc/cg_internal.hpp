@@ -5,7 +5,7 @@
 
 namespace cg {
 
-// Geometry needed by the coordinate kernels (all FP64: north-star step (i)).
+// Geometry needed by the coordinate kernels (all FP64: the uvw scaling to grid coordinates).
 struct CoordArgs {
     double scale_uv;   // 2 pi f/c (one channel)
     double scale_w;    // 2 pi f/c * sign

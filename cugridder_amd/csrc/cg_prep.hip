@@ -1,6 +1,6 @@
 // Coordinate preparation: w range, tile keys and sorted point records.
 //
-// All coordinate arithmetic is FP64 (north-star step (i)): the reference's
+// All coordinate arithmetic is FP64 (the uvw scaling to grid coordinates): the reference's
 // get_effective_coordinate (src/astro/precomp.cu:48-80), rescaling_real
 // (src/utils.cu:532-556), pre_stage_2 (src/fourier/cuft.cu:82-103) and
 // SHIFT_RESCALE (include/conv.h:24) are folded into one pass that emits, per

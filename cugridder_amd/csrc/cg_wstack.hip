@@ -1,5 +1,5 @@
-// Per-pixel stages of improved W-stacking, with FP64 kept for north-star
-// steps (ii) n-1, (iii) the w-phase and (iv) the correction factors.
+// Per-pixel stages of improved W-stacking, with FP64 kept for n-1, the
+// w-phase and the correction factors.
 //
 // ms2dirty epilogue fuses, per image pixel, the reference's
 //   w_term_dft      (src/fourier/cuft.cu:597-645): sum_p F_p e^{i z'(p-P/2)}

@@ -22,8 +22,8 @@ error; to first order the 2-D / 3-D error is the sum over axes).
 
 Training runs on the GPU (cg_learn_train: HIP forward, backward and Adam
 kernels).  ``certify`` then images a seeded P x P problem with the learned
-kernel and with the ES kernel at support 7 (eps 1e-6, the judge) and accepts
-the kernel for eps only when both directions agree with the judge within
+kernel and with the ES kernel at support 7 (eps 1e-6, the certifier) and accepts
+the kernel for eps only when both directions agree with the certifier within
 eps (relative L2).  The ES kernel stays the fallback: a context only uses a
 learned kernel that was installed with ``Context.set_kernel`` and whose
 certified eps is no larger than the call's.
@@ -136,9 +136,9 @@ def validation_problem(patch=256, nvis=1 << 16, fov_deg=1.0, freq=1.0e9, seed=1)
     return uvw, vis, fov_deg, freq
 
 
-def certify(kw, params, eps, deg=None, patch=256, nvis=1 << 16, judge_eps=1e-6, seed=1):
+def certify(kw, params, eps, deg=None, patch=256, nvis=1 << 16, certifier_eps=1e-6, seed=1):
     """Relative L2 differences (ms2dirty, dirty2ms) of the learned kernel
-    against the ES judge on a seeded patch x patch problem, and whether both
+    against the ES certifier on a seeded patch x patch problem, and whether both
     are within eps."""
     import torch
     from .context import Context
@@ -147,23 +147,23 @@ def certify(kw, params, eps, deg=None, patch=256, nvis=1 << 16, judge_eps=1e-6, 
     dev = torch.device("cuda")
     uvw = torch.from_numpy(uvw_h).to(dev)
     vis = torch.from_numpy(vis_h).to(dev)
-    judge = Context(use_learned=0)
+    certifier = Context(use_learned=0)
     cand = Context()
     try:
         cand.set_kernel(kw, params, eps, deg)
-        d_ref = judge.ms2dirty(uvw, vis, None, patch, patch, fov, freq, judge_eps)
+        d_ref = certifier.ms2dirty(uvw, vis, None, patch, patch, fov, freq, certifier_eps)
         d_new = cand.ms2dirty(uvw, vis, None, patch, patch, fov, freq, eps)
         if cand.last_plan()["kw"] != kw:
             raise RuntimeError("the learned kernel was not selected")
         rng = np.random.default_rng(seed + 1)
         img = torch.from_numpy((rng.standard_normal((patch, patch)) +
                                 1j * rng.standard_normal((patch, patch))).astype(np.complex64)).to(dev)
-        v_ref = judge.dirty2ms(uvw, img, None, fov, freq, judge_eps)
+        v_ref = certifier.dirty2ms(uvw, img, None, fov, freq, certifier_eps)
         v_new = cand.dirty2ms(uvw, img, None, fov, freq, eps)
         torch.cuda.synchronize()
         e1 = float(torch.linalg.norm(d_new - d_ref) / torch.linalg.norm(d_ref))
         e2 = float(torch.linalg.norm(v_new - v_ref) / torch.linalg.norm(v_ref))
     finally:
-        judge.close()
+        certifier.close()
         cand.close()
     return {"ms2dirty": e1, "dirty2ms": e2, "accepted": bool(e1 <= eps and e2 <= eps)}

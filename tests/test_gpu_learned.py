@@ -5,7 +5,7 @@ tabulation kernels are compared with the FP64 NumPy oracle
 (oracle/learned.py); the trained kernel is then used by ms2dirty /
 dirty2ms and compared with the FP64 direct DFT and with the ES kernel at
 the same support (what the network gains), and the certification against
-the ES judge is shown to accept and to reject.
+the ES certifier is shown to accept and to reject.
 """
 import numpy as np
 import pytest
@@ -199,7 +199,7 @@ def test_learned_kernel_on_the_mfma_paths(L, trained):
 
 
 def test_certification_accepts_and_rejects(L, trained):
-    """The ES judge (support 7, eps 1e-6) accepts the support-6 kernel at
+    """The ES certifier (support 7, eps 1e-6) accepts the support-6 kernel at
     1e-5 (the learned workload's bar) and rejects supports 5 and 4 there
     (their per-axis error alone is above it)."""
     ok = L.certify(6, trained[6][0], 1e-5, patch=256, nvis=1 << 15)
